@@ -1,0 +1,124 @@
+"""Throughput of the plain FP64 RHS + SSP-RK3 path on one MI355X.
+
+Workload: decaying compressible isotropic turbulence, periodic [0, 2pi]^3,
+256^3 cells, gamma = 1.4, Pr = 0.72, Sutherland mu, Re = 100, Ma_t = 0.6,
+WCNS5 interpolation + HLLC flux, CFL 0.5, seeded synthetic initial state.
+One step = record U0 + CFL time step + three RK3 stages, entirely on the
+device (no host synchronisation inside the timed region).
+
+Prints one JSON line; see --help.  --gpus above 1 is rejected: this build
+runs one GPU per process and measures one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-RK3-steps/s @256^3: FP64 plain vs mixed-FP32 at stated err; adjoint/fwd"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--scheme", default="wcns5", help="wcns5 | weno5_js | mdcd | mdcd_linear")
+    ap.add_argument("--riemann", default="HLLC")
+    ap.add_argument("--mach-t", type=float, default=0.6)
+    ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU checker on a bounded sample")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    return ap.parse_args()
+
+
+def cpu_baseline(prim, grid, params, threads):
+    """One full RK3 step of the same workload through the CPU checker (the
+    restatement of the reference's pass structure), on `threads` host cores."""
+    from oracle.ns3d_oracle import OracleField, set_threads
+
+    used = set_threads(threads)
+    ref = OracleField(grid, params)
+    ref.initialize_primitive(*prim)
+    t0 = time.perf_counter()
+    dt = ref.compute_timestep()
+    ref.record_conserved_to0()
+    ref.runge_kutta_3(dt)
+    el = time.perf_counter() - t0
+    cells = grid.global_nx * grid.global_ny * grid.global_nz
+    ref.close()
+    return {"value": cells / el / 1e6, "unit": "Mcell-RK3-steps/s", "cores": used, "kind": "port",
+            "sample": f"1 RK3 step (CFL + 3 stages) of the {grid.global_nx}^3 workload, {el:.2f} s wall"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from ns3d_wcns_amd import DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
+    from ns3d_wcns_amd.isotropic import isotropic_state
+
+    torch.cuda.set_device(0)
+    n = args.n
+    prim, ma = isotropic_state(n, mach_t=args.mach_t, seed=12345)
+    P = SolverParams(Ma=ma, Re=100.0, Pr=0.72, cfl=0.5, interpolation=SCHEME_ALIASES[args.scheme],
+                     riemann_solver=RiemannSolver[args.riemann]).derive()
+    G = periodic_box(n)
+    stream = torch.cuda.current_stream()
+    dev = DeviceField(G, P)
+    dev.set_stream(stream.cuda_stream)
+    dev.initialize_primitive(*prim)
+    dts = torch.zeros(args.warmup + args.steps, dtype=torch.float64, device="cuda")
+    base = dts.data_ptr()
+    for s in range(args.warmup):
+        dev.advance_steps(1, base + 8 * s)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    dev.advance_steps(args.steps, base + 8 * args.warmup)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop) / args.steps
+    cells = n ** 3
+    value = cells / (ms * 1e-3) / 1e6
+    rho = dev.interior(dev.download("rho"))
+    finite = bool(np.isfinite(rho).all())
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcell-RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"isotropic turbulence {n}^3 periodic, Ma_t={args.mach_t}, Re=100, "
+                               f"{args.scheme.upper()}+{args.riemann}, CFL 0.5, plain FP64",
+                   "cells": cells, "parallelism": "1 GPU"},
+        "wall_s": round(wall, 3),
+        "state_finite": finite,
+    }
+    if args.cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(prim, G, P, args.cpu_threads)
+    dev.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

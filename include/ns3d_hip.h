@@ -1,0 +1,193 @@
+/*
+ * ns3d_hip.h -- C ABI of the MI355X (gfx950) RHS + SSP-RK3 library for one
+ * periodic box of the WCNS compressible Navier-Stokes solver.
+ *
+ * Every entry point below replaces one function of the reference's call
+ * surface (ns3d_func.h / field_structures.h of wylx-2/ns3d_wcns); the
+ * reference function is cited next to each declaration.  Signatures use plain
+ * pointers, sizes and POD structs only.
+ *
+ * Model
+ *   - A context owns device-resident SoA fields in the reference's ghosted
+ *     layout: sx = nx + 2*ng cells per x line (ng = 3), x fastest,
+ *     idx3 = (k*sy + j)*sx + i (field_structures.h:64-67).
+ *   - Host arrays handed to upload/download use that same layout, so a
+ *     Field3D's std::vector<double>::data() can be passed directly.
+ *   - All work is enqueued on the context's HIP stream (ns3d_set_stream);
+ *     functions that return values to the host synchronise that stream.
+ *   - Status: 0 on success, a negative NS3D_E_* code on failure; the message
+ *     is available from ns3d_last_error() (per thread).  The reference
+ *     reports errors on std::cerr and returns early (InviscousFlux.cpp:19-31);
+ *     here the caller gets the code and the message instead.
+ *   - Boundary conditions: periodic on all six sides (the one-rank periodic
+ *     case of boundary.cpp:14-54).  Non-periodic BCs are rejected.
+ */
+#ifndef NS3D_HIP_H
+#define NS3D_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define NS3D_OK 0
+#define NS3D_E_ARG (-1)         /* bad argument (null pointer, size) */
+#define NS3D_E_UNSUPPORTED (-2) /* configuration outside this library */
+#define NS3D_E_HIP (-3)         /* HIP runtime error */
+#define NS3D_E_NOMEM (-4)       /* device allocation failed */
+
+/* ---- enums: same values as SolverParams' enums (field_structures.h:120-146) ---- */
+#define NS3D_INTERP_WENO5 0       /* interpolation = WENO5: WCNS5 weights 1/16,5/8,5/16 (interpolation.cpp:86-115) */
+#define NS3D_INTERP_ZERO 1        /* rejected: reads past the ghost layers in the reference */
+#define NS3D_INTERP_MDCD_LINEAR 2 /* interpolation.cpp:118-133 */
+#define NS3D_INTERP_MDCD_HYBRID 3 /* interpolation.cpp:135-209 */
+#define NS3D_INTERP_WENO5_JS 4    /* weno5_reconstruction, Jiang-Shu weights (reconstruction.cpp:69-100) */
+
+#define NS3D_RIEMANN_ROE 0
+#define NS3D_RIEMANN_RUSANOV 1
+#define NS3D_RIEMANN_HLLC 2
+#define NS3D_RIEMANN_HLL 3
+#define NS3D_RIEMANN_HLLC_P 4
+#define NS3D_RIEMANN_AUSM 5
+
+#define NS3D_PRECISION_FP64 0  /* plain FP64 method */
+#define NS3D_PRECISION_MIXED 1 /* FP32 interpolation/flux/viscous stencils, FP64 state and accumulation */
+
+/* Numerical fields of SolverParams (field_structures.h:86-186).  Derived
+ * values (mu, Cv, Cp, Rgas) must already be set, as
+ * read_solver_params_from_file does (initialization.cpp:194-197). */
+typedef struct ns3d_params {
+    double gamma, Pr, Ma, Re, cfl;
+    double mu, S_ref, Cv, Cp, Rgas;
+    double mdcd_diss, mdcd_disp;
+    int interpolation;  /* NS3D_INTERP_* */
+    int riemann_solver; /* NS3D_RIEMANN_* */
+    int char_recon;     /* 0 component-wise, 1 characteristic-wise */
+    int pad_;
+} ns3d_params;
+
+/* GridDesc + LocalDesc of one periodic rank (field_structures.h:33-60). */
+typedef struct ns3d_grid {
+    int nx, ny, nz; /* interior cells; each >= 6 */
+    int ng;         /* ghost layers; must be 3 (initialization.cpp:175-191) */
+    double dx, dy, dz;
+} ns3d_grid;
+
+typedef struct ns3d_ctx ns3d_ctx;
+
+/* Field ids for upload/download and device-pointer access. */
+enum ns3d_field {
+    NS3D_RHO = 0, NS3D_RHOU, NS3D_RHOV, NS3D_RHOW, NS3D_E,         /* conserved */
+    NS3D_RHO0, NS3D_RHOU0, NS3D_RHOV0, NS3D_RHOW0, NS3D_E0,        /* recordConservedTo0 copy */
+    NS3D_U, NS3D_V, NS3D_W, NS3D_P, NS3D_T,                        /* primitives */
+    NS3D_RHS_RHO, NS3D_RHS_RHOU, NS3D_RHS_RHOV, NS3D_RHS_RHOW, NS3D_RHS_E,
+    NS3D_NUM_CELL_FIELDS
+};
+
+/* ---- lifetime ---- */
+/* Field3D::allocate (field_structures.h:250-373): rho = 1, all else 0. */
+int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params, int precision);
+int ns3d_destroy(ns3d_ctx *ctx);
+/* stream: a hipStream_t passed as void* (NULL = the null stream). */
+int ns3d_set_stream(ns3d_ctx *ctx, void *stream);
+int ns3d_set_params(ns3d_ctx *ctx, const ns3d_params *params);
+const char *ns3d_last_error(void);
+/* Cells per field including ghosts (sx*sy*sz). */
+size_t ns3d_cell_count(const ns3d_ctx *ctx);
+int ns3d_synchronize(ns3d_ctx *ctx);
+
+/* ---- data movement (ghosted host arrays of ns3d_cell_count doubles) ---- */
+int ns3d_upload(ns3d_ctx *ctx, int field, const double *host);
+int ns3d_download(ns3d_ctx *ctx, int field, double *host);
+/* Device address of a field (ghosted layout), for zero-copy use. */
+int ns3d_field_device_ptr(ns3d_ctx *ctx, int field, double **dptr);
+
+/* ---- Field3D members (field_structures.h:444-497) ---- */
+int ns3d_conserved_to_primitive(ns3d_ctx *ctx); /* Field3D::conservedToPrimitive :444-465 */
+int ns3d_primitive_to_conserved(ns3d_ctx *ctx); /* Field3D::primitiveToConserved :468-488 */
+int ns3d_record_conserved_to0(ns3d_ctx *ctx);   /* Field3D::recordConservedTo0 :491-497 */
+
+/* ---- boundary (boundary.cpp, field_structures.h:511-1056) ---- */
+/* apply_boundary (boundary.cpp:14-54 -> exchange_halos_physical,
+ * field_structures.h:936-994): periodic copy of rho,u,v,w,p into the three
+ * ghost layers of each face (tangential range interior only). */
+int ns3d_apply_boundary(ns3d_ctx *ctx);
+/* apply_boundary_halfnode_flux (boundary.cpp:207-244) and
+ * exchange_halos_viscous_flux (field_structures.h:877-932): on one periodic
+ * rank the copied values are the periodic images the consuming kernels read
+ * by wrapped indexing, so these complete without moving data. */
+int ns3d_apply_boundary_halfnode_flux(ns3d_ctx *ctx);
+int ns3d_exchange_halos_viscous_flux(ns3d_ctx *ctx);
+
+/* ---- inviscid flux (InviscousFlux.cpp) ---- */
+/* compute_invis_flux (InviscousFlux.cpp:10-144) with
+ * WCNS_Riemann_InviscidFlux (:146-268): the numerical flux at every face of
+ * every interior line, into the library's face arrays. */
+int ns3d_compute_invis_flux(ns3d_ctx *ctx);
+/* compute_invis_dflux (InviscousFlux.cpp:1027-1056): rhs -= 6th-order
+ * half-node difference of the face fluxes, x then y then z. */
+int ns3d_compute_invis_dflux(ns3d_ctx *ctx);
+/* Face fluxes in the reference's face layout (idx_fx/idx_fy/idx_fz,
+ * field_structures.h:73-82) for dir 0/1/2 and component 0..4 (mass, momx,
+ * momy, momz, E); host array of the face-array size, interior lines filled,
+ * including the ng-1 periodic ghost faces per side. */
+int ns3d_download_face_flux(ns3d_ctx *ctx, int dir, int comp, double *host);
+size_t ns3d_face_count(const ns3d_ctx *ctx, int dir);
+
+/* ---- viscous flux (ViscousFlux.cpp) ---- */
+int ns3d_compute_gradients(ns3d_ctx *ctx);      /* compute_gradients :109-148 */
+int ns3d_compute_viscous_flux(ns3d_ctx *ctx);   /* compute_viscous_flux :151-208 */
+int ns3d_compute_vis_flux(ns3d_ctx *ctx);       /* compute_vis_flux :210-255 */
+/* gradient q = 0..11 (du_dx du_dy du_dz dv_dx ... dT_dz); viscous flux
+ * dir 0/1/2 (Fv = x, Hv = y, Gv = z) and comp 1..4.  Ghosted cell layout,
+ * interior filled (viscous fluxes also in the face ghost layers). */
+int ns3d_download_gradient(ns3d_ctx *ctx, int q, double *host);
+int ns3d_download_viscous_flux(ns3d_ctx *ctx, int dir, int comp, double *host);
+
+/* ---- RHS, time step, RK3 (range_kutta.cpp, timestep.cpp) ---- */
+/* compute_rhs (range_kutta.cpp:11-43): RHS fields = inviscid + viscous. */
+int ns3d_compute_rhs(ns3d_ctx *ctx);
+/* runge_kutta_3 (range_kutta.cpp:46-92): three Shu-Osher stages from the
+ * recorded U0, each followed by C2P -> apply_boundary -> P2C. */
+int ns3d_runge_kutta_3(ns3d_ctx *ctx, double dt);
+/* Same, reading dt from device memory (no host synchronisation). */
+int ns3d_runge_kutta_3_dev(ns3d_ctx *ctx, const double *d_dt);
+/* compute_timestep (timestep.cpp:5-23): cfl * min over the interior of
+ * dx / (|u| + a), x-velocity and dx only.  Synchronises. */
+int ns3d_compute_timestep(ns3d_ctx *ctx, double *dt_out);
+/* Same, result left in device memory at d_dt. */
+int ns3d_compute_timestep_dev(ns3d_ctx *ctx, double *d_dt);
+
+/* time_advance (maincycle.cpp:7-113) without monitors and output: per step
+ * record U0, dt from the CFL (or dt_fixed when > 0), clip dt to
+ * output_timefreq / total_time, stop when dt <= 1e-12, runge_kutta_3.
+ * dt_out (may be NULL) receives each step's dt; *steps_taken the count.
+ * Synchronises once per step to run the reference's host-side dt logic. */
+int ns3d_time_advance(ns3d_ctx *ctx, int max_steps, double dt_fixed, double total_time,
+                      double output_timefreq, double *dt_out, int *steps_taken);
+/* nsteps of (record U0, CFL dt, RK3) entirely on the device; d_dt_hist (device,
+ * nsteps doubles, may be NULL) receives each step's dt.  No clipping: the
+ * caller's total time and output interval must lie beyond the run. */
+int ns3d_advance_steps(ns3d_ctx *ctx, int nsteps, double *d_dt_hist);
+
+/* ---- observables (postprocess.cpp:103-128) ---- */
+/* compute_total_energy: sum over the interior of 0.5*rho*|u|^2*dx*dy*dz
+ * from the primitive arrays.  Synchronises. */
+int ns3d_total_kinetic_energy(ns3d_ctx *ctx, double *out);
+
+/* ---- scalar kernels over batches (device pointers), for known-answer tests ---- */
+/* interpolate_select (interpolation.cpp:13-77): stencils n x 6, flag +1 / -1. */
+int ns3d_interpolate_batch(const ns3d_params *params, const double *d_stencils, int n, double flag,
+                           double *d_out, void *stream);
+/* X_Riemann_solver (InviscousFlux.cpp:270-846): UL, UR n x 5 conserved,
+ * normal along axis dim; F n x 5. */
+int ns3d_riemann_batch(const ns3d_params *params, const double *d_UL, const double *d_UR, int n, int dim,
+                       double *d_F, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NS3D_HIP_H */

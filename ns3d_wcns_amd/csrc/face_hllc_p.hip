@@ -1,0 +1,5 @@
+// Face-flux kernels for the hllc_p Riemann solver (all directions, interpolations
+// and interpolation variables).
+#include "ns3d_face_impl.h"
+
+NS3D_INSTANTIATE_FACE(ns3d::R_HLLC_P)

@@ -1,0 +1,673 @@
+// ns3d_capi.hip -- the extern "C" boundary of libns3d_hip (include/ns3d_hip.h):
+// context lifetime, device memory layout, data movement and the orchestration
+// of the kernels behind each reference entry point.
+//
+// Device memory of one context (doubles):
+//   20 ghosted cell fields (U, U0, Q = u v w p T, RHS), sx*sy*sz each, in the
+//      reference's layout so that upload/download are plain copies;
+//   15 compact face-flux fields (3 directions x 5 components), nx*ny*nz each;
+//    9 compact viscous-flux fields, nx*ny*nz each;
+//   12 compact gradient fields, allocated on first use of compute_gradients;
+//   reduction partials (one per 64x4 block) and a few device scalars.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ns3d_hip.h"
+#include "ns3d_kernels.h"
+
+using namespace ns3d;
+
+struct ns3d_ctx {
+    Box b;
+    ns3d_grid grid;
+    ns3d_params P;
+    int precision;
+    hipStream_t stream;
+    size_t ncell, nint;
+    double *slab;
+    double *cell[NS3D_NUM_CELL_FIELDS];
+    double *face[3][5];
+    double *visc[NVISC];
+    double *grad_slab;
+    double *grad[12];
+    double *partial;
+    double *scal;  // [0] dt, [1] reduction result
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define NS3D_HIP(call)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return fail(NS3D_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define NS3D_LAUNCHED(what)                                                                          \
+    do {                                                                                             \
+        hipError_t e_ = hipGetLastError();                                                           \
+        if (e_ != hipSuccess) return fail(NS3D_E_HIP, "%s launch: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+#define NS3D_CTX(c)                                                     \
+    do {                                                                \
+        if (!(c)) return fail(NS3D_E_ARG, "%s: null context", __func__); \
+    } while (0)
+
+int check_params(const ns3d_params *p) {
+    if (!p) return fail(NS3D_E_ARG, "null params");
+    if (p->interpolation == NS3D_INTERP_ZERO)
+        return fail(NS3D_E_UNSUPPORTED,
+                    "interpolation ZERO is not supported: its 2-point stencil sets one ghost layer while the "
+                    "6th-order half-node difference reads three");
+    if (p->interpolation != NS3D_INTERP_WENO5 && p->interpolation != NS3D_INTERP_MDCD_LINEAR &&
+        p->interpolation != NS3D_INTERP_MDCD_HYBRID && p->interpolation != NS3D_INTERP_WENO5_JS)
+        return fail(NS3D_E_ARG, "unknown interpolation %d", p->interpolation);
+    if (p->riemann_solver < NS3D_RIEMANN_ROE || p->riemann_solver > NS3D_RIEMANN_AUSM)
+        return fail(NS3D_E_ARG, "Unknown Riemann solver %d", p->riemann_solver);
+    if (!(p->gamma > 1.0)) return fail(NS3D_E_ARG, "gamma must exceed 1 (got %g)", p->gamma);
+    return NS3D_OK;
+}
+
+SchemeCoef<double> coef64(const ns3d_params &P) {
+    SchemeCoef<double> c;
+    const double diss = P.mdcd_diss, disp = P.mdcd_disp;
+    c.gamma = P.gamma;
+    // linear MDCD weights, interpolation.cpp:128-132
+    c.mdcd_lin[0] = (3.0 * disp + 3.0 * diss) / 8.0;
+    c.mdcd_lin[1] = (-18.0 * disp - 30 * diss - 1.0) / 16.0;
+    c.mdcd_lin[2] = (12.0 * disp + 60.0 * diss + 9.0) / 16.0;
+    c.mdcd_lin[3] = (12.0 * disp - 60.0 * diss + 9.0) / 16.0;
+    c.mdcd_lin[4] = (-18.0 * disp + 30.0 * diss - 1.0) / 16.0;
+    c.mdcd_lin[5] = (3.0 * disp - 3.0 * diss) / 8.0;
+    // MDCD-WENO linear weights, interpolation.cpp:182-185
+    c.mdcd_d[0] = 1.5 * (disp + diss);
+    c.mdcd_d[1] = 0.5 - 1.5 * (disp - 3 * diss);
+    c.mdcd_d[2] = 0.5 - 1.5 * (disp + 3 * diss);
+    c.mdcd_d[3] = 1.5 * (disp - diss);
+    return c;
+}
+
+SchemeCoef<float> coef32(const ns3d_params &P) {
+    const SchemeCoef<double> d = coef64(P);
+    SchemeCoef<float> c;
+    c.gamma = (float)d.gamma;
+    for (int q = 0; q < 6; ++q) c.mdcd_lin[q] = (float)d.mdcd_lin[q];
+    for (int q = 0; q < 4; ++q) c.mdcd_d[q] = (float)d.mdcd_d[q];
+    return c;
+}
+
+double **conserved(ns3d_ctx *c) { return &c->cell[NS3D_RHO]; }
+double **recorded(ns3d_ctx *c) { return &c->cell[NS3D_RHO0]; }
+double **prims(ns3d_ctx *c) { return &c->cell[NS3D_U]; }
+double **rhs(ns3d_ctx *c) { return &c->cell[NS3D_RHS_RHO]; }
+
+int run_faces(ns3d_ctx *c) {
+    FaceArgs<double> a64{};
+    FaceArgs<float> a32{};
+    for (int n = 0; n < 5; ++n) a64.U[n] = a32.U[n] = conserved(c)[n];
+    for (int n = 0; n < 4; ++n) a64.Q[n] = a32.Q[n] = prims(c)[n];
+    a64.b = a32.b = c->b;
+    a64.c = coef64(c->P);
+    a32.c = coef32(c->P);
+    a64.ref = a32.ref = MixedRef{0.0, 0.0};
+    for (int dir = 0; dir < 3; ++dir) {
+        for (int n = 0; n < 5; ++n) a64.F[n] = a32.F[n] = c->face[dir][n];
+        const int interp = c->P.interpolation;
+        const bool chr = c->P.char_recon != 0;
+        switch (c->P.riemann_solver) {
+            case R_ROE: launch_face<R_ROE>(dir, interp, chr, c->precision, a64, a32, c->stream); break;
+            case R_RUSANOV: launch_face<R_RUSANOV>(dir, interp, chr, c->precision, a64, a32, c->stream); break;
+            case R_HLLC: launch_face<R_HLLC>(dir, interp, chr, c->precision, a64, a32, c->stream); break;
+            case R_HLL: launch_face<R_HLL>(dir, interp, chr, c->precision, a64, a32, c->stream); break;
+            case R_HLLC_P: launch_face<R_HLLC_P>(dir, interp, chr, c->precision, a64, a32, c->stream); break;
+            default: launch_face<R_AUSM>(dir, interp, chr, c->precision, a64, a32, c->stream); break;
+        }
+        NS3D_LAUNCHED("face flux");
+    }
+    return NS3D_OK;
+}
+
+ViscArgs visc_args(ns3d_ctx *c, bool with_grad, bool with_flux) {
+    ViscArgs a{};
+    const double *q[4] = {c->cell[NS3D_U], c->cell[NS3D_V], c->cell[NS3D_W], c->cell[NS3D_T]};
+    for (int n = 0; n < 4; ++n) a.Q[n] = q[n];
+    for (int n = 0; n < 12; ++n) a.G[n] = with_grad ? c->grad[n] : nullptr;
+    for (int n = 0; n < NVISC; ++n) a.V[n] = with_flux ? c->visc[n] : nullptr;
+    a.b = c->b;
+    a.i60[0] = 1.0 / (60.0 * c->grid.dx);
+    a.i60[1] = 1.0 / (60.0 * c->grid.dy);
+    a.i60[2] = 1.0 / (60.0 * c->grid.dz);
+    a.mu = c->P.mu;
+    a.S_ref = c->P.S_ref;
+    a.cp_pr = c->P.Cp / c->P.Pr;
+    return a;
+}
+
+AsmArgs asm_args(ns3d_ctx *c, int stage, int mode, const double *d_dt, double dt) {
+    AsmArgs a{};
+    for (int d = 0; d < 3; ++d)
+        for (int n = 0; n < 5; ++n) a.F[d][n] = c->face[d][n];
+    for (int n = 0; n < NVISC; ++n) a.V[n] = c->visc[n];
+    for (int n = 0; n < 5; ++n) {
+        a.U[n] = conserved(c)[n];
+        a.U0[n] = recorded(c)[n];
+        a.Q[n] = prims(c)[n];
+        a.RHS[n] = rhs(c)[n];
+    }
+    a.b = c->b;
+    a.ih[0] = 1.0 / c->grid.dx;
+    a.ih[1] = 1.0 / c->grid.dy;
+    a.ih[2] = 1.0 / c->grid.dz;
+    for (int d = 0; d < 3; ++d) a.i60[d] = a.ih[d] / 60.0;
+    a.gamma = c->P.gamma;
+    a.Rgas = c->P.Rgas;
+    a.dt = d_dt;
+    a.dtv = dt;
+    a.stage = stage;
+    a.mode = mode;
+    return a;
+}
+
+ConvArgs conv_args(ns3d_ctx *c) {
+    ConvArgs a{};
+    for (int n = 0; n < 5; ++n) {
+        a.U[n] = conserved(c)[n];
+        a.Q[n] = prims(c)[n];
+    }
+    a.n = (long)c->ncell;
+    a.gamma = c->P.gamma;
+    a.Rgas = c->P.Rgas;
+    return a;
+}
+
+int halo(ns3d_ctx *c, bool all) {
+    HaloArgs h{};
+    h.b = c->b;
+    if (all) {  // conserved and primitives after a fused stage
+        for (int n = 0; n < 5; ++n) h.f[n] = conserved(c)[n];
+        for (int n = 0; n < 5; ++n) h.f[5 + n] = prims(c)[n];
+        h.nf = 10;
+    } else {  // exchange_halos_physical: rho, u, v, w, p
+        h.f[0] = c->cell[NS3D_RHO];
+        for (int n = 0; n < 4; ++n) h.f[1 + n] = prims(c)[n];
+        h.nf = 5;
+    }
+    launch_halo(h, c->stream);
+    NS3D_LAUNCHED("halo");
+    return NS3D_OK;
+}
+
+int rk3(ns3d_ctx *c, const double *d_dt, double dt) {
+    int rc;
+    for (int stage = 1; stage <= 3; ++stage) {
+        if ((rc = run_faces(c))) return rc;
+        launch_visc(visc_args(c, false, true), c->stream);
+        NS3D_LAUNCHED("viscous");
+        launch_assemble(asm_args(c, stage, ASM_INVISCID | ASM_VISCOUS, d_dt, dt), c->stream);
+        NS3D_LAUNCHED("assemble");
+        if ((rc = halo(c, true))) return rc;
+    }
+    return NS3D_OK;
+}
+
+int record0(ns3d_ctx *c) {
+    for (int n = 0; n < 5; ++n)
+        NS3D_HIP(hipMemcpyAsync(recorded(c)[n], conserved(c)[n], c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+    return NS3D_OK;
+}
+
+int cfl(ns3d_ctx *c, double *d_out) {
+    CflArgs a{};
+    a.rho = c->cell[NS3D_RHO];
+    a.rhou = c->cell[NS3D_RHOU];
+    a.E = c->cell[NS3D_E];
+    a.partial = c->partial;
+    a.dt_out = d_out;
+    a.b = c->b;
+    a.gamma = c->P.gamma;
+    a.dx = c->grid.dx;
+    a.cfl = c->P.cfl;
+    launch_cfl(a, c->stream);
+    NS3D_LAUNCHED("cfl");
+    return NS3D_OK;
+}
+
+// compact -> reference layout scatter of one periodic field (host side)
+void scatter_cells(const ns3d_ctx *c, const std::vector<double> &compact, double *host, bool face_ghosts) {
+    const Box &b = c->b;
+    std::memset(host, 0, c->ncell * sizeof(double));
+    auto at = [&](int i, int j, int k) { return compact[b.oidx(i, j, k)]; };
+    for (int k = 0; k < b.nz; ++k)
+        for (int j = 0; j < b.ny; ++j)
+            for (int i = 0; i < b.nx; ++i) host[b.gidx(i, j, k)] = at(i, j, k);
+    if (!face_ghosts) return;
+    auto w = [](int i, int n) { return ((i % n) + n) % n; };
+    for (int k = 0; k < b.nz; ++k)
+        for (int j = 0; j < b.ny; ++j)
+            for (int g = 1; g <= NG; ++g) {
+                host[b.gidx(-g, j, k)] = at(w(-g, b.nx), j, k);
+                host[b.gidx(b.nx - 1 + g, j, k)] = at(w(b.nx - 1 + g, b.nx), j, k);
+            }
+    for (int k = 0; k < b.nz; ++k)
+        for (int i = 0; i < b.nx; ++i)
+            for (int g = 1; g <= NG; ++g) {
+                host[b.gidx(i, -g, k)] = at(i, w(-g, b.ny), k);
+                host[b.gidx(i, b.ny - 1 + g, k)] = at(i, w(b.ny - 1 + g, b.ny), k);
+            }
+    for (int j = 0; j < b.ny; ++j)
+        for (int i = 0; i < b.nx; ++i)
+            for (int g = 1; g <= NG; ++g) {
+                host[b.gidx(i, j, -g)] = at(i, j, w(-g, b.nz));
+                host[b.gidx(i, j, b.nz - 1 + g)] = at(i, j, w(b.nz - 1 + g, b.nz));
+            }
+}
+
+int fetch_compact(ns3d_ctx *c, const double *dptr, std::vector<double> &out) {
+    out.resize(c->nint);
+    NS3D_HIP(hipMemcpyAsync(out.data(), dptr, c->nint * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ensure_grad(ns3d_ctx *c) {
+    if (c->grad_slab) return NS3D_OK;
+    if (hipMalloc(&c->grad_slab, 12 * c->nint * sizeof(double)) != hipSuccess) {
+        c->grad_slab = nullptr;
+        (void)hipGetLastError();
+        return fail(NS3D_E_NOMEM, "gradient arrays: out of device memory");
+    }
+    for (int q = 0; q < 12; ++q) c->grad[q] = c->grad_slab + q * c->nint;
+    NS3D_HIP(hipMemsetAsync(c->grad_slab, 0, 12 * c->nint * sizeof(double), c->stream));
+    return NS3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ns3d_last_error(void) { return g_err.c_str(); }
+
+int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params, int precision) {
+    if (!out || !grid) return fail(NS3D_E_ARG, "ns3d_create: null argument");
+    *out = nullptr;
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (grid->ng != NG) return fail(NS3D_E_UNSUPPORTED, "ghost layers must be %d (got %d)", NG, grid->ng);
+    if (grid->nx < 6 || grid->ny < 6 || grid->nz < 6)
+        return fail(NS3D_E_ARG, "each interior extent must be >= 6 (got %d x %d x %d)", grid->nx, grid->ny,
+                    grid->nz);
+    const double ghosted = (double)(grid->nx + 2 * NG) * (grid->ny + 2 * NG) * (grid->nz + 2 * NG);
+    if (ghosted >= 2147483647.0)
+        return fail(NS3D_E_ARG, "box of %g cells exceeds the reference's 32-bit cell index", ghosted);
+    if (!(grid->dx > 0 && grid->dy > 0 && grid->dz > 0)) return fail(NS3D_E_ARG, "grid spacing must be positive");
+    if (precision != NS3D_PRECISION_FP64)
+        return fail(NS3D_E_UNSUPPORTED, "precision %d is not available in this build", precision);
+    ns3d_ctx *c = new ns3d_ctx();
+    c->grid = *grid;
+    c->P = *params;
+    c->precision = precision;
+    c->stream = nullptr;
+    Box &b = c->b;
+    b.nx = grid->nx; b.ny = grid->ny; b.nz = grid->nz;
+    b.sx = b.nx + 2 * NG; b.sy = b.ny + 2 * NG; b.sz = b.nz + 2 * NG;
+    c->ncell = (size_t)b.sx * b.sy * b.sz;
+    c->nint = (size_t)b.nx * b.ny * b.nz;
+    const size_t npart = (size_t)ke_partial_count(b);
+    const size_t total = NS3D_NUM_CELL_FIELDS * c->ncell + (15 + NVISC) * c->nint + npart + 16;
+    if (hipMalloc(&c->slab, total * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return fail(NS3D_E_NOMEM, "out of device memory for %zu doubles", total);
+    }
+    double *p = c->slab;
+    for (int f = 0; f < NS3D_NUM_CELL_FIELDS; ++f, p += c->ncell) c->cell[f] = p;
+    for (int d = 0; d < 3; ++d)
+        for (int n = 0; n < 5; ++n, p += c->nint) c->face[d][n] = p;
+    for (int n = 0; n < NVISC; ++n, p += c->nint) c->visc[n] = p;
+    c->partial = p;
+    p += npart;
+    c->scal = p;
+    c->grad_slab = nullptr;
+    if (hipMemset(c->slab, 0, total * sizeof(double)) != hipSuccess) {
+        (void)hipFree(c->slab);
+        delete c;
+        return fail(NS3D_E_HIP, "hipMemset failed");
+    }
+    launch_fill(c->cell[NS3D_RHO], (long)c->ncell, 1.0, nullptr);  // Field3D::allocate: rho = 1
+    if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(c->slab);
+        delete c;
+        return fail(NS3D_E_HIP, "initialisation failed");
+    }
+    *out = c;
+    return NS3D_OK;
+}
+
+int ns3d_destroy(ns3d_ctx *c) {
+    if (!c) return NS3D_OK;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->grad_slab) (void)hipFree(c->grad_slab);
+    (void)hipFree(c->slab);
+    delete c;
+    return NS3D_OK;
+}
+
+int ns3d_set_stream(ns3d_ctx *c, void *stream) {
+    NS3D_CTX(c);
+    c->stream = (hipStream_t)stream;
+    return NS3D_OK;
+}
+
+int ns3d_set_params(ns3d_ctx *c, const ns3d_params *params) {
+    NS3D_CTX(c);
+    int rc = check_params(params);
+    if (rc) return rc;
+    c->P = *params;
+    return NS3D_OK;
+}
+
+size_t ns3d_cell_count(const ns3d_ctx *c) { return c ? c->ncell : 0; }
+
+int ns3d_synchronize(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_upload(ns3d_ctx *c, int field, const double *host) {
+    NS3D_CTX(c);
+    if (field < 0 || field >= NS3D_NUM_CELL_FIELDS || !host) return fail(NS3D_E_ARG, "ns3d_upload: bad field or pointer");
+    NS3D_HIP(hipMemcpyAsync(c->cell[field], host, c->ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_download(ns3d_ctx *c, int field, double *host) {
+    NS3D_CTX(c);
+    if (field < 0 || field >= NS3D_NUM_CELL_FIELDS || !host)
+        return fail(NS3D_E_ARG, "ns3d_download: bad field or pointer");
+    NS3D_HIP(hipMemcpyAsync(host, c->cell[field], c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_field_device_ptr(ns3d_ctx *c, int field, double **dptr) {
+    NS3D_CTX(c);
+    if (field < 0 || field >= NS3D_NUM_CELL_FIELDS || !dptr) return fail(NS3D_E_ARG, "bad field or pointer");
+    *dptr = c->cell[field];
+    return NS3D_OK;
+}
+
+int ns3d_conserved_to_primitive(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    launch_c2p(conv_args(c), c->stream);
+    NS3D_LAUNCHED("conservedToPrimitive");
+    return NS3D_OK;
+}
+
+int ns3d_primitive_to_conserved(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    launch_p2c(conv_args(c), c->stream);
+    NS3D_LAUNCHED("primitiveToConserved");
+    return NS3D_OK;
+}
+
+int ns3d_record_conserved_to0(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    return record0(c);
+}
+
+int ns3d_apply_boundary(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    return halo(c, false);
+}
+
+int ns3d_apply_boundary_halfnode_flux(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    return NS3D_OK;
+}
+
+int ns3d_exchange_halos_viscous_flux(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    return NS3D_OK;
+}
+
+int ns3d_compute_invis_flux(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    return run_faces(c);
+}
+
+int ns3d_compute_invis_dflux(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    launch_assemble(asm_args(c, 0, ASM_INVISCID | ASM_ACCUMULATE, nullptr, 0.0), c->stream);
+    NS3D_LAUNCHED("invis_dflux");
+    return NS3D_OK;
+}
+
+size_t ns3d_face_count(const ns3d_ctx *c, int dir) {
+    if (!c || dir < 0 || dir > 2) return 0;
+    const Box &b = c->b;
+    return dir == 0 ? (size_t)(b.sx - 1) * b.sy * b.sz
+                    : (dir == 1 ? (size_t)b.sx * (b.sy - 1) * b.sz : (size_t)b.sx * b.sy * (b.sz - 1));
+}
+
+int ns3d_download_face_flux(ns3d_ctx *c, int dir, int comp, double *host) {
+    NS3D_CTX(c);
+    if (dir < 0 || dir > 2 || comp < 0 || comp > 4 || !host) return fail(NS3D_E_ARG, "bad face/component");
+    std::vector<double> f;
+    int rc = fetch_compact(c, c->face[dir][comp], f);
+    if (rc) return rc;
+    const Box &b = c->b;
+    std::memset(host, 0, ns3d_face_count(c, dir) * sizeof(double));
+    const int n = dir == 0 ? b.nx : (dir == 1 ? b.ny : b.nz);
+    auto w = [](int i, int m) { return ((i % m) + m) % m; };
+    // reference face index q (between ghosted cells q and q+1) = compact face q - NG
+    for (int k = 0; k < b.nz; ++k)
+        for (int j = 0; j < b.ny; ++j)
+            for (int i = 0; i < b.nx; ++i) {
+                if (dir == 0 && i == 0) {
+                    for (int q = 0; q < b.sx - 1; ++q)
+                        host[((long)(k + NG) * b.sy + (j + NG)) * (b.sx - 1) + q] = f[b.oidx(w(q - NG, n), j, k)];
+                } else if (dir == 1 && j == 0) {
+                    for (int q = 0; q < b.sy - 1; ++q)
+                        host[((long)(k + NG) * (b.sy - 1) + q) * b.sx + (i + NG)] = f[b.oidx(i, w(q - NG, n), k)];
+                } else if (dir == 2 && k == 0) {
+                    for (int q = 0; q < b.sz - 1; ++q)
+                        host[((long)q * b.sy + (j + NG)) * b.sx + (i + NG)] = f[b.oidx(i, j, w(q - NG, n))];
+                }
+            }
+    return NS3D_OK;
+}
+
+int ns3d_compute_gradients(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    int rc = ensure_grad(c);
+    if (rc) return rc;
+    launch_visc(visc_args(c, true, false), c->stream);
+    NS3D_LAUNCHED("gradients");
+    return NS3D_OK;
+}
+
+int ns3d_compute_viscous_flux(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    int rc = ensure_grad(c);
+    if (rc) return rc;
+    launch_visc_from_grad(visc_args(c, true, true), c->stream);
+    NS3D_LAUNCHED("viscous flux");
+    return NS3D_OK;
+}
+
+int ns3d_compute_vis_flux(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    launch_assemble(asm_args(c, 0, ASM_VISCOUS | ASM_ACCUMULATE, nullptr, 0.0), c->stream);
+    NS3D_LAUNCHED("vis_flux");
+    return NS3D_OK;
+}
+
+int ns3d_download_gradient(ns3d_ctx *c, int q, double *host) {
+    NS3D_CTX(c);
+    if (q < 0 || q >= 12 || !host) return fail(NS3D_E_ARG, "bad gradient index");
+    int rc = ensure_grad(c);
+    if (rc) return rc;
+    std::vector<double> f;
+    if ((rc = fetch_compact(c, c->grad[q], f))) return rc;
+    scatter_cells(c, f, host, false);
+    return NS3D_OK;
+}
+
+int ns3d_download_viscous_flux(ns3d_ctx *c, int dir, int comp, double *host) {
+    NS3D_CTX(c);
+    if (dir < 0 || dir > 2 || comp < 0 || comp > 4 || !host) return fail(NS3D_E_ARG, "bad viscous flux index");
+    if (comp == 0) {
+        std::memset(host, 0, c->ncell * sizeof(double));
+        return NS3D_OK;
+    }
+    static const int slot[3][4] = {{VXX, VXY, VXZ, VEX}, {VXY, VYY, VYZ, VEY}, {VXZ, VYZ, VZZ, VEZ}};
+    std::vector<double> f;
+    int rc = fetch_compact(c, c->visc[slot[dir][comp - 1]], f);
+    if (rc) return rc;
+    scatter_cells(c, f, host, true);
+    return NS3D_OK;
+}
+
+int ns3d_compute_rhs(ns3d_ctx *c) {
+    NS3D_CTX(c);
+    for (int n = 0; n < 5; ++n) NS3D_HIP(hipMemsetAsync(rhs(c)[n], 0, c->ncell * sizeof(double), c->stream));
+    int rc = run_faces(c);
+    if (rc) return rc;
+    launch_visc(visc_args(c, false, true), c->stream);
+    NS3D_LAUNCHED("viscous");
+    launch_assemble(asm_args(c, 0, ASM_INVISCID | ASM_VISCOUS, nullptr, 0.0), c->stream);
+    NS3D_LAUNCHED("assemble");
+    return NS3D_OK;
+}
+
+int ns3d_runge_kutta_3(ns3d_ctx *c, double dt) {
+    NS3D_CTX(c);
+    return rk3(c, nullptr, dt);
+}
+
+int ns3d_runge_kutta_3_dev(ns3d_ctx *c, const double *d_dt) {
+    NS3D_CTX(c);
+    if (!d_dt) return fail(NS3D_E_ARG, "null device dt");
+    return rk3(c, d_dt, 0.0);
+}
+
+int ns3d_compute_timestep(ns3d_ctx *c, double *dt_out) {
+    NS3D_CTX(c);
+    if (!dt_out) return fail(NS3D_E_ARG, "null dt_out");
+    int rc = cfl(c, c->scal);
+    if (rc) return rc;
+    NS3D_HIP(hipMemcpyAsync(dt_out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_compute_timestep_dev(ns3d_ctx *c, double *d_dt) {
+    NS3D_CTX(c);
+    if (!d_dt) return fail(NS3D_E_ARG, "null device dt");
+    return cfl(c, d_dt);
+}
+
+int ns3d_time_advance(ns3d_ctx *c, int max_steps, double dt_fixed, double total_time, double output_timefreq,
+                      double *dt_out, int *steps_taken) {
+    NS3D_CTX(c);
+    double t = 0.0, out_t = output_timefreq;
+    int taken = 0, rc;
+    for (int step = 1; step <= max_steps; ++step) {
+        double dt = dt_fixed;
+        if ((rc = record0(c))) return rc;
+        const double eps = 1e-12;
+        if (std::abs(total_time - t) <= eps) break;
+        if (dt_fixed < 0.0 && (rc = ns3d_compute_timestep(c, &dt))) return rc;
+        if (t >= out_t - eps) out_t += output_timefreq;
+        if (t + dt > total_time) dt = total_time - t;
+        if (t + dt > out_t) {
+            dt = out_t - t;
+            out_t += output_timefreq;
+        }
+        if (dt <= eps) {
+            fprintf(stderr, "dt too small or zero; aborting to avoid stall at Time=%g\n", t);
+            break;
+        }
+        t += dt;
+        if ((rc = rk3(c, nullptr, dt))) return rc;
+        if (dt_out) dt_out[taken] = dt;
+        ++taken;
+        if (t >= total_time) break;
+    }
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    if (steps_taken) *steps_taken = taken;
+    return NS3D_OK;
+}
+
+int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
+    NS3D_CTX(c);
+    int rc;
+    for (int s = 0; s < nsteps; ++s) {
+        double *d_dt = d_dt_hist ? d_dt_hist + s : c->scal;
+        if ((rc = record0(c))) return rc;
+        if ((rc = cfl(c, d_dt))) return rc;
+        if ((rc = rk3(c, d_dt, 0.0))) return rc;
+    }
+    return NS3D_OK;
+}
+
+int ns3d_total_kinetic_energy(ns3d_ctx *c, double *out) {
+    NS3D_CTX(c);
+    if (!out) return fail(NS3D_E_ARG, "null output");
+    KeArgs a{};
+    a.rho = c->cell[NS3D_RHO];
+    a.u = c->cell[NS3D_U];
+    a.v = c->cell[NS3D_V];
+    a.w = c->cell[NS3D_W];
+    a.partial = c->partial;
+    a.b = c->b;
+    a.vol = c->grid.dx * c->grid.dy * c->grid.dz;
+    launch_ke(a, c->scal + 1, c->stream);
+    NS3D_LAUNCHED("kinetic energy");
+    NS3D_HIP(hipMemcpyAsync(out, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_interpolate_batch(const ns3d_params *params, const double *d_st, int n, double flag, double *d_out,
+                           void *stream) {
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!d_st || !d_out))) return fail(NS3D_E_ARG, "bad batch");
+    if (n == 0) return NS3D_OK;
+    launch_interp_batch(params->interpolation, coef64(*params), d_st, n, flag, d_out, (hipStream_t)stream);
+    NS3D_LAUNCHED("interpolate batch");
+    return NS3D_OK;
+}
+
+int ns3d_riemann_batch(const ns3d_params *params, const double *d_UL, const double *d_UR, int n, int dim,
+                       double *d_F, void *stream) {
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (n < 0 || dim < 0 || dim > 2 || (n > 0 && (!d_UL || !d_UR || !d_F))) return fail(NS3D_E_ARG, "bad batch");
+    if (n == 0) return NS3D_OK;
+    launch_riemann_batch(params->riemann_solver, dim, params->gamma, d_UL, d_UR, n, d_F, (hipStream_t)stream);
+    NS3D_LAUNCHED("riemann batch");
+    return NS3D_OK;
+}
+
+}  // extern "C"

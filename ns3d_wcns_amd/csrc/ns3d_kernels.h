@@ -1,0 +1,84 @@
+// ns3d_kernels.h -- argument blocks and launchers of the cell-wise kernels.
+#pragma once
+#include <algorithm>
+
+#include "ns3d_common.h"
+
+namespace ns3d {
+
+// Compact slots of the nine distinct viscous-flux components.  The reference
+// stores twelve (Fvflux_* = x, Hvflux_* = y, Gvflux_* = z for momx, momy, momz,
+// E; ViscousFlux.cpp:187-206); the off-diagonal stresses appear twice there.
+enum { VXX = 0, VYY, VZZ, VXY, VXZ, VYZ, VEX, VEY, VEZ, NVISC };
+
+struct ViscArgs {
+    const double *Q[4];  // ghosted u, v, w, T
+    double *G[12];       // compact gradients (optional: nullptr for the fused path)
+    double *V[NVISC];    // compact viscous fluxes (nullptr: gradients only)
+    Box b;
+    double i60[3];       // 1 / (60 h)
+    double mu, S_ref, cp_pr;
+};
+
+enum { ASM_INVISCID = 1, ASM_VISCOUS = 2, ASM_ACCUMULATE = 4 };
+
+struct AsmArgs {
+    const double *F[3][5];    // compact face fluxes per direction
+    const double *V[NVISC];   // compact viscous fluxes
+    double *U[5];             // ghosted conserved (read at stage 2/3, written at 1..3)
+    const double *U0[5];      // ghosted recorded state
+    double *Q[5];             // ghosted u, v, w, p, T (written at stage 1..3)
+    double *RHS[5];           // ghosted RHS (stage 0)
+    Box b;
+    double ih[3], i60[3];
+    double gamma, Rgas;
+    const double *dt;         // device scalar, or nullptr to use dtv
+    double dtv;
+    int stage;                // 0: store RHS; 1..3: RK3 stage
+    int mode;                 // ASM_* bits
+};
+
+struct ConvArgs {
+    double *U[5];
+    double *Q[5];
+    long n;
+    double gamma, Rgas;
+};
+
+struct HaloArgs {
+    double *f[10];
+    int nf;
+    Box b;
+};
+
+struct CflArgs {
+    const double *rho, *rhou, *E;
+    double *partial;
+    double *dt_out;
+    Box b;
+    double gamma, dx, cfl;
+};
+
+struct KeArgs {
+    const double *rho, *u, *v, *w;
+    double *partial;
+    Box b;
+    double vol;
+};
+
+long ke_partial_count(const Box &b);
+void launch_visc(const ViscArgs &a, hipStream_t s);
+void launch_visc_from_grad(const ViscArgs &a, hipStream_t s);
+void launch_assemble(const AsmArgs &a, hipStream_t s);
+void launch_c2p(const ConvArgs &a, hipStream_t s);
+void launch_p2c(const ConvArgs &a, hipStream_t s);
+void launch_fill(double *p, long n, double v, hipStream_t s);
+void launch_halo(const HaloArgs &h, hipStream_t s);
+void launch_cfl(const CflArgs &a, hipStream_t s);
+void launch_ke(const KeArgs &a, double *out, hipStream_t s);
+void launch_interp_batch(int interp, const SchemeCoef<double> &c, const double *st, int n, double flag, double *out,
+                         hipStream_t s);
+void launch_riemann_batch(int rs, int dir, double g, const double *UL, const double *UR, int n, double *F,
+                          hipStream_t s);
+
+}  // namespace ns3d

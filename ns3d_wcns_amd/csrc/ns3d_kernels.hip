@@ -1,0 +1,473 @@
+// ns3d_kernels.hip -- cell-wise kernels of the RHS + RK3 path: viscous fluxes,
+// RHS assembly fused with the SSP-RK3 stage update and the C2P/P2C round trip,
+// periodic halo fill, CFL reduction and the kinetic-energy sum.
+#include "ns3d_kernels.h"
+
+namespace ns3d {
+
+// 6th-order central first derivative of a 7-point line without its centre,
+// times 1/(60 h) (diff_6th_central, ViscousFlux.cpp:27-29).
+__device__ __forceinline__ double d6c(double fm3, double fm2, double fm1, double fp1, double fp2, double fp3,
+                                      double i60h) {
+    return (fp3 - 9.0 * fp2 + 45.0 * fp1 - 45.0 * fm1 + 9.0 * fm2 - fm3) * i60h;
+}
+
+// 6th-order midpoint-to-node difference of faces i-5/2..i+5/2 times 1/h
+// (diff_6th_central_half, InviscousFlux.cpp:990-992).
+__device__ __forceinline__ double d6h(double f0, double f1, double f2, double f3, double f4, double f5, double ih) {
+    return ((f3 - f2) * (75.0 / 64.0) + (f4 - f1) * (-25.0 / 384.0) + (f5 - f0) * (3.0 / 640.0)) * ih;
+}
+
+// Sutherland viscosity mu(T) = mu_ref T^1.5 (1 + S) / (T + S)
+// (SolverParams::get_mu, field_structures.h:96-106).
+__device__ __forceinline__ double sutherland(double mu, double S, double T) {
+    return mu * (T * sqrt(T)) * (1.0 + S) / (T + S);
+}
+
+// ---------------------------------------------------------------- viscous
+// Velocity/temperature gradients, Sutherland mu, stresses and heat flux
+// (compute_gradients + compute_viscous_flux, ViscousFlux.cpp:109-208) in one
+// pass; writes the nine distinct viscous flux components in compact layout.
+__global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.b.nx || j >= a.b.ny) return;
+    const long c = a.b.gidx(i, j, k);
+    const long sx = a.b.sx, sxy = (long)a.b.sx * a.b.sy;
+    double g[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double *f = a.Q[q];
+        g[q][0] = d6c(f[c - 3], f[c - 2], f[c - 1], f[c + 1], f[c + 2], f[c + 3], a.i60[0]);
+        g[q][1] = d6c(f[c - 3 * sx], f[c - 2 * sx], f[c - sx], f[c + sx], f[c + 2 * sx], f[c + 3 * sx], a.i60[1]);
+        g[q][2] = d6c(f[c - 3 * sxy], f[c - 2 * sxy], f[c - sxy], f[c + sxy], f[c + 2 * sxy], f[c + 3 * sxy], a.i60[2]);
+    }
+    const long o = a.b.oidx(i, j, k);
+    if (a.G[0]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) a.G[3 * q + d][o] = g[q][d];
+    }
+    if (!a.V[0]) return;
+    const double T = a.Q[3][c];
+    const double mu = sutherland(a.mu, a.S_ref, T);
+    const double divU = g[0][0] + g[1][1] + g[2][2];
+    const double txx = 2.0 * mu * g[0][0] - (2.0 / 3.0) * mu * divU;
+    const double tyy = 2.0 * mu * g[1][1] - (2.0 / 3.0) * mu * divU;
+    const double tzz = 2.0 * mu * g[2][2] - (2.0 / 3.0) * mu * divU;
+    const double txy = mu * (g[0][1] + g[1][0]);
+    const double txz = mu * (g[0][2] + g[2][0]);
+    const double tyz = mu * (g[1][2] + g[2][1]);
+    const double kq = mu * a.cp_pr;
+    const double u = a.Q[0][c], v = a.Q[1][c], w = a.Q[2][c];
+    a.V[VXX][o] = txx;
+    a.V[VYY][o] = tyy;
+    a.V[VZZ][o] = tzz;
+    a.V[VXY][o] = txy;
+    a.V[VXZ][o] = txz;
+    a.V[VYZ][o] = tyz;
+    a.V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
+    a.V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
+    a.V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
+}
+
+// Viscous fluxes from stored gradients (compute_viscous_flux alone,
+// ViscousFlux.cpp:151-208), for the stepwise call surface.
+__global__ void __launch_bounds__(256) k_visc_from_grad(ViscArgs a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.b.nx || j >= a.b.ny) return;
+    const long c = a.b.gidx(i, j, k);
+    const long o = a.b.oidx(i, j, k);
+    double g[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[q][d] = a.G[3 * q + d][o];
+    const double T = a.Q[3][c];
+    const double mu = sutherland(a.mu, a.S_ref, T);
+    const double divU = g[0][0] + g[1][1] + g[2][2];
+    const double txx = 2.0 * mu * g[0][0] - (2.0 / 3.0) * mu * divU;
+    const double tyy = 2.0 * mu * g[1][1] - (2.0 / 3.0) * mu * divU;
+    const double tzz = 2.0 * mu * g[2][2] - (2.0 / 3.0) * mu * divU;
+    const double txy = mu * (g[0][1] + g[1][0]);
+    const double txz = mu * (g[0][2] + g[2][0]);
+    const double tyz = mu * (g[1][2] + g[2][1]);
+    const double kq = mu * a.cp_pr;
+    const double u = a.Q[0][c], v = a.Q[1][c], w = a.Q[2][c];
+    a.V[VXX][o] = txx;
+    a.V[VYY][o] = tyy;
+    a.V[VZZ][o] = tzz;
+    a.V[VXY][o] = txy;
+    a.V[VXZ][o] = txz;
+    a.V[VYZ][o] = tyz;
+    a.V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
+    a.V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
+    a.V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
+}
+
+// ---------------------------------------------------------------- assembly
+// Per interior cell: rhs = - d/dx F - d/dy G - d/dz H (per variable x, y, z as
+// in compute_invis_dflux, InviscousFlux.cpp:1037-1055) + viscous divergence
+// (compute_vis_flux, ViscousFlux.cpp:239-253).  Face and viscous-flux indices
+// wrap periodically, which reads the values the reference's half-node and
+// viscous-flux halo exchanges copy (field_structures.h:773-932).
+//
+// stage 0 stores the RHS; stage 1..3 applies the Shu-Osher update
+// (range_kutta.cpp:53-87) and the conserved -> primitive -> conserved round
+// trip (field_structures.h:444-488) in registers and stores U and Q.
+__global__ void __launch_bounds__(256) k_assemble(AsmArgs a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.b.nx || j >= a.b.ny) return;
+    const Box &b = a.b;
+    const long c = b.gidx(i, j, k);
+    long ox[7], oy[7], oz[7];  // compact indices of offsets -3..+3 along each axis
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+        ox[m] = b.oidx(wrap(i + m - 3, b.nx), j, k);
+        oy[m] = b.oidx(i, wrap(j + m - 3, b.ny), k);
+        oz[m] = b.oidx(i, j, wrap(k + m - 3, b.nz));
+    }
+    double r[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) r[n] = (a.mode & ASM_ACCUMULATE) ? a.RHS[n][c] : 0.0;
+    if (a.mode & ASM_INVISCID) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            const double *X = a.F[0][n], *Y = a.F[1][n], *Z = a.F[2][n];
+            // faces i-3 .. i+2 are stored at cells i-3 .. i+2
+            r[n] -= d6h(X[ox[0]], X[ox[1]], X[ox[2]], X[ox[3]], X[ox[4]], X[ox[5]], a.ih[0]);
+            r[n] -= d6h(Y[oy[0]], Y[oy[1]], Y[oy[2]], Y[oy[3]], Y[oy[4]], Y[oy[5]], a.ih[1]);
+            r[n] -= d6h(Z[oz[0]], Z[oz[1]], Z[oz[2]], Z[oz[3]], Z[oz[4]], Z[oz[5]], a.ih[2]);
+        }
+    }
+    if (a.mode & ASM_VISCOUS) {
+        // momentum n = 1..3 and energy n = 4: d/dx Fv + d/dy Hv + d/dz Gv
+        const int vx[4] = {VXX, VXY, VXZ, VEX};
+        const int vy[4] = {VXY, VYY, VYZ, VEY};
+        const int vz[4] = {VXZ, VYZ, VZZ, VEZ};
+#pragma unroll
+        for (int n = 1; n < 5; ++n) {
+            const double *X = a.V[vx[n - 1]], *Y = a.V[vy[n - 1]], *Z = a.V[vz[n - 1]];
+            r[n] += d6c(X[ox[0]], X[ox[1]], X[ox[2]], X[ox[4]], X[ox[5]], X[ox[6]], a.i60[0]);
+            r[n] += d6c(Y[oy[0]], Y[oy[1]], Y[oy[2]], Y[oy[4]], Y[oy[5]], Y[oy[6]], a.i60[1]);
+            r[n] += d6c(Z[oz[0]], Z[oz[1]], Z[oz[2]], Z[oz[4]], Z[oz[5]], Z[oz[6]], a.i60[2]);
+        }
+    }
+    if (a.stage == 0) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) a.RHS[n][c] = r[n];
+        return;
+    }
+    const double dt = a.dt ? *a.dt : a.dtv;
+    double un[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        const double u0 = a.U0[n][c];
+        if (a.stage == 1) un[n] = u0 + dt * r[n];
+        else if (a.stage == 2) un[n] = 0.75 * u0 + 0.25 * (a.U[n][c] + dt * r[n]);
+        else un[n] = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (a.U[n][c] + dt * r[n]);
+    }
+    // conservedToPrimitive then primitiveToConserved
+    double rr = un[0];
+    if (rr <= 0.0) rr = 1e-12;
+    const double ir = 1.0 / rr;
+    const double u = un[1] * ir, v = un[2] * ir, w = un[3] * ir;
+    const double kin = 0.5 * (u * u + v * v + w * w);
+    const double p = (a.gamma - 1.0) * rr * (un[4] * ir - kin);
+    const double T = p / (rr * a.Rgas);
+    a.U[0][c] = rr;
+    a.U[1][c] = rr * u;
+    a.U[2][c] = rr * v;
+    a.U[3][c] = rr * w;
+    a.U[4][c] = rr * (p / ((a.gamma - 1.0) * rr) + kin);
+    a.Q[0][c] = u;
+    a.Q[1][c] = v;
+    a.Q[2][c] = w;
+    a.Q[3][c] = p;
+    a.Q[4][c] = T;
+}
+
+// ---------------------------------------------------------------- conversions
+// Field3D::conservedToPrimitive over all cells (field_structures.h:444-465),
+// without contraction: the same operation sequence rounds to the reference's bits.
+__global__ void k_c2p(ConvArgs a) {
+#pragma clang fp contract(off)
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= a.n) return;
+    double rr = a.U[0][id];
+    if (rr <= 0.0) {
+        rr = 1e-12;
+        a.U[0][id] = rr;
+    }
+    const double u = a.U[1][id] / rr, v = a.U[2][id] / rr, w = a.U[3][id] / rr;
+    const double kin = 0.5 * (u * u + v * v + w * w);
+    const double p = (a.gamma - 1.0) * rr * (a.U[4][id] / rr - kin);
+    a.Q[0][id] = u;
+    a.Q[1][id] = v;
+    a.Q[2][id] = w;
+    a.Q[3][id] = p;
+    a.Q[4][id] = p / (rr * a.Rgas);
+}
+
+// Field3D::primitiveToConserved over all cells (field_structures.h:468-488).
+__global__ void k_p2c(ConvArgs a) {
+#pragma clang fp contract(off)
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= a.n) return;
+    double rr = a.U[0][id];
+    if (rr <= 0.0) {
+        rr = 1e-12;
+        a.U[0][id] = rr;
+    }
+    const double u = a.Q[0][id], v = a.Q[1][id], w = a.Q[2][id], p = a.Q[3][id];
+    a.U[1][id] = rr * u;
+    a.U[2][id] = rr * v;
+    a.U[3][id] = rr * w;
+    const double kin = 0.5 * (u * u + v * v + w * w);
+    a.U[4][id] = rr * (p / ((a.gamma - 1.0) * rr) + kin);
+    a.Q[4][id] = p / (rr * a.Rgas);
+}
+
+__global__ void k_fill(double *p, long n, double v) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) p[id] = v;
+}
+
+// ---------------------------------------------------------------- halo
+// Periodic copy into the NG ghost layers of the six faces, tangential range
+// interior only (exchange_halos_physical on one periodic rank,
+// field_structures.h:511-649, 936-994).  blockIdx.y selects the x, y or z
+// faces; element e = ((side*NG + layer)*B + b)*A + a with a the fastest
+// tangential index (x for the y and z faces).
+__global__ void k_halo(HaloArgs h) {
+    const Box &b = h.b;
+    const int grp = blockIdx.y;
+    const int A = grp == 0 ? b.ny : b.nx;
+    const int B = grp == 2 ? b.ny : b.nz;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)A * B;
+    if (e >= 2 * NG * per) return;
+    const int a = (int)(e % A);
+    const int bb = (int)((e / A) % B);
+    const int layer = (int)((e / per) % NG);
+    const int side = (int)(e / (per * NG));
+    const int n = grp == 0 ? b.nx : (grp == 1 ? b.ny : b.nz);
+    // ghosted coordinate along the normal: left ghosts take cell + n, right ghosts cell - n
+    const int gd = side == 0 ? layer : NG + n + layer;
+    const int gs = side == 0 ? layer + n : NG + layer;
+    long dst, src;
+    const long sx = b.sx, sxy = (long)b.sx * b.sy;
+    if (grp == 0) {
+        const long base = (long)(bb + NG) * sxy + (long)(a + NG) * sx;
+        dst = base + gd;
+        src = base + gs;
+    } else if (grp == 1) {
+        const long base = (long)(bb + NG) * sxy + (a + NG);
+        dst = base + (long)gd * sx;
+        src = base + (long)gs * sx;
+    } else {
+        const long base = (long)(bb + NG) * sx + (a + NG);
+        dst = base + (long)gd * sxy;
+        src = base + (long)gs * sxy;
+    }
+    for (int f = 0; f < h.nf; ++f) h.f[f][dst] = h.f[f][src];
+}
+
+// ---------------------------------------------------------------- CFL
+// max over the interior of |u| + a with the reference's x-velocity formula
+// (compute_timestep, timestep.cpp:12-20); NaN candidates are skipped, as
+// std::min skips them there.  Evaluated without contraction so that
+// dt = cfl * min(1e9, dx / smax) reproduces the reference's bits (correctly
+// rounded division is monotone, so min_i dx/s_i = dx / max_i s_i).
+__device__ __forceinline__ double cfl_speed(double rho, double rhou, double E, double gamma) {
+#pragma clang fp contract(off)
+    const double u = rhou / rho;
+    const double p = (gamma - 1.) * (E - 0.5 * rho * u * u);
+    const double a = sqrt(gamma * p / rho);
+    return fabs(u) + a;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double s = 0.0;
+    if (i < a.b.nx && j < a.b.ny) {
+        const long c = a.b.gidx(i, j, k);
+        const double v = cfl_speed(a.rho[c], a.rhou[c], a.E[c], a.gamma);
+        if (v == v) s = v;  // skip NaN
+    }
+    s = wave_max(s);
+    __shared__ double part[4];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    if ((t & 63) == 0) part[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+        const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        a.partial[blk] = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+    }
+}
+
+// Second pass: max of the block maxima, then dt = cfl * min(1e9, dx / smax).
+__global__ void __launch_bounds__(256) k_cfl_final(CflArgs a, long nblk) {
+    __shared__ double part[4];
+    double s = 0.0;
+    for (long q = threadIdx.x; q < nblk; q += 256) s = fmax(s, a.partial[q]);
+    s = wave_max(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma clang fp contract(off)
+        const double smax = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+        double dtl = a.dx / smax;  // +inf when every candidate was 0 or NaN
+        dtl = dtl < 1e9 ? dtl : 1e9;
+        *a.dt_out = a.cfl * dtl;
+    }
+}
+
+// ---------------------------------------------------------------- kinetic energy
+// Deterministic two-pass sum of 0.5 rho |u|^2 dV over the interior
+// (compute_total_energy, postprocess.cpp:103-128).
+__global__ void __launch_bounds__(256) k_ke_partial(KeArgs a) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double s = 0.0;
+    if (i < a.b.nx && j < a.b.ny) {
+        const long c = a.b.gidx(i, j, k);
+        const double u = a.u[c], v = a.v[c], w = a.w[c];
+        s = 0.5 * a.rho[c] * (u * u + v * v + w * w) * a.vol;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ double part[4];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    if ((t & 63) == 0) part[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+        const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        a.partial[blk] = (part[0] + part[1]) + (part[2] + part[3]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sum_final(const double *partial, long n, double *out) {
+    __shared__ double part[4];
+    double s = 0.0;
+    for (long q = threadIdx.x; q < n; q += 256) s += partial[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// ---------------------------------------------------------------- scalar batches
+template <int INTERP>
+__global__ void k_interp_batch(SchemeCoef<double> c, const double *st, int n, double flag, double *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double *s = st + 6 * (long)t;
+    double l, r;
+    interp_lr<INTERP>(c, s, l, r);
+    out[t] = flag < 0.0 ? r : l;
+}
+
+template <int RS, int DIR>
+__global__ void k_riemann_batch(double g, const double *UL, const double *UR, int n, double *F) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
+    constexpr int XS = DIR == 0 ? 1 : (DIR == 1 ? 3 : 2);
+    const double *l = UL + 5 * (long)t, *r = UR + 5 * (long)t;
+    const double a[5] = {l[0], l[P1], l[P2], l[P3], l[4]};
+    const double b[5] = {r[0], r[P1], r[P2], r[P3], r[4]};
+    double f[5];
+    riemann<RS, XS>(a, b, g, f);
+    double *o = F + 5 * (long)t;
+    o[0] = f[0];
+    o[P1] = f[1];
+    o[P2] = f[2];
+    o[P3] = f[3];
+    o[4] = f[4];
+}
+
+// ---------------------------------------------------------------- launchers
+long ke_partial_count(const Box &b) {
+    const dim3 g = cell_grid(b);
+    return (long)g.x * g.y * g.z;
+}
+void launch_visc(const ViscArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_visc, cell_grid(a.b), cell_block(), 0, s, a);
+}
+void launch_visc_from_grad(const ViscArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_visc_from_grad, cell_grid(a.b), cell_block(), 0, s, a);
+}
+void launch_assemble(const AsmArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_assemble, cell_grid(a.b), cell_block(), 0, s, a);
+}
+void launch_c2p(const ConvArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_c2p, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+}
+void launch_p2c(const ConvArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_p2c, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+}
+void launch_fill(double *p, long n, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+void launch_halo(const HaloArgs &h, hipStream_t s) {
+    const long m = (long)std::max(std::max(h.b.ny * h.b.nz, h.b.nx * h.b.nz), h.b.nx * h.b.ny) * 2 * NG;
+    hipLaunchKernelGGL(k_halo, dim3((unsigned)((m + 255) / 256), 3), dim3(256), 0, s, h);
+}
+void launch_cfl(const CflArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_cfl, cell_grid(a.b), cell_block(), 0, s, a);
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, ke_partial_count(a.b));
+}
+void launch_ke(const KeArgs &a, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_ke_partial, cell_grid(a.b), cell_block(), 0, s, a);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, (const double *)a.partial, ke_partial_count(a.b),
+                       out);
+}
+void launch_interp_batch(int interp, const SchemeCoef<double> &c, const double *st, int n, double flag, double *out,
+                         hipStream_t s) {
+    const dim3 g((n + 255) / 256), bl(256);
+    switch (interp) {
+        case I_WENO5: hipLaunchKernelGGL(k_interp_batch<I_WENO5>, g, bl, 0, s, c, st, n, flag, out); break;
+        case I_MDCD_LINEAR: hipLaunchKernelGGL(k_interp_batch<I_MDCD_LINEAR>, g, bl, 0, s, c, st, n, flag, out); break;
+        case I_MDCD_HYBRID: hipLaunchKernelGGL(k_interp_batch<I_MDCD_HYBRID>, g, bl, 0, s, c, st, n, flag, out); break;
+        default: hipLaunchKernelGGL(k_interp_batch<I_WENO5_JS>, g, bl, 0, s, c, st, n, flag, out); break;
+    }
+}
+
+template <int RS>
+static void riemann_dir(int dir, double g, const double *UL, const double *UR, int n, double *F, hipStream_t s) {
+    const dim3 gr((n + 255) / 256), bl(256);
+    if (dir == 0) hipLaunchKernelGGL((k_riemann_batch<RS, 0>), gr, bl, 0, s, g, UL, UR, n, F);
+    else if (dir == 1) hipLaunchKernelGGL((k_riemann_batch<RS, 1>), gr, bl, 0, s, g, UL, UR, n, F);
+    else hipLaunchKernelGGL((k_riemann_batch<RS, 2>), gr, bl, 0, s, g, UL, UR, n, F);
+}
+
+void launch_riemann_batch(int rs, int dir, double g, const double *UL, const double *UR, int n, double *F,
+                          hipStream_t s) {
+    switch (rs) {
+        case R_ROE: riemann_dir<R_ROE>(dir, g, UL, UR, n, F, s); break;
+        case R_RUSANOV: riemann_dir<R_RUSANOV>(dir, g, UL, UR, n, F, s); break;
+        case R_HLLC: riemann_dir<R_HLLC>(dir, g, UL, UR, n, F, s); break;
+        case R_HLL: riemann_dir<R_HLL>(dir, g, UL, UR, n, F, s); break;
+        case R_HLLC_P: riemann_dir<R_HLLC_P>(dir, g, UL, UR, n, F, s); break;
+        default: riemann_dir<R_AUSM>(dir, g, UL, UR, n, F, s); break;
+    }
+}
+
+}  // namespace ns3d

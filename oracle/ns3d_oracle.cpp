@@ -1,0 +1,1014 @@
+// ns3d_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the reference solver's right-hand-side and SSP-RK3 path
+// (wylx-2/ns3d_wcns) for one periodic rank.  It is the checker for the HIP
+// library and the timed CPU baseline; nothing in the product links it.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+//
+// Every routine follows the reference function it names (file:line cited) in
+// arithmetic order, so that, compiled with -ffp-contract=off (as the reference's
+// `-O3` build on x86-64 without -march is), it would reproduce the reference's
+// bits.  The reference itself cannot be compiled here (every hot-path file
+// includes <fftw3-mpi.h>, which this image lacks), so each function is pinned by
+// the closed forms, invariants and brute-force checks in tests/test_oracle_*.py.
+//
+// Storage is the reference's: SoA double arrays of sx*sy*sz cells (ghosts
+// included, x fastest, field_structures.h:64-67) and face arrays of
+// (sx-1)*sy*sz / sx*(sy-1)*sz / sx*sy*(sz-1) entries (field_structures.h:73-82).
+// On one periodic rank the MPI halo exchanges reduce to the index copies below
+// (field_structures.h:511-1056 with nbr_* == self).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+// Mirrors the numerical fields of SolverParams (field_structures.h:86-186).
+struct orc_params {
+    double gamma, Pr, Ma, Re, cfl;
+    double mu, S_ref, Cv, Cp, Rgas;
+    double mdcd_diss, mdcd_disp;
+    int interpolation;   // 0 WENO5(=WCNS5 weights) 2 MDCD_LINEAR 3 MDCD_HYBRID 4 WENO5-JS
+    int riemann_solver;  // 0 Roe 1 Rusanov 2 HLLC 3 HLL 4 HLLC_p 5 AUSM
+    int char_recon;
+    int pad_;
+};
+
+}  // extern "C"
+
+namespace {
+
+enum { I_WENO5 = 0, I_ZERO = 1, I_MDCD_LINEAR = 2, I_MDCD_HYBRID = 3, I_WENO5_JS = 4 };
+enum { R_ROE = 0, R_RUSANOV = 1, R_HLLC = 2, R_HLL = 3, R_HLLC_P = 4, R_AUSM = 5 };
+
+// ---------------------------------------------------------------- interpolation
+// interpolation.cpp:86-115 (weno5_interpolate: the WCNS5 weights 1/16, 5/8, 5/16)
+double weno5_interpolate(const double *s) {
+    double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+    double eps = 1e-6;
+    double b0 = (13.0 / 12.0) * (f0 - 2 * f1 + f2) * (f0 - 2 * f1 + f2) +
+                0.25 * (f0 - 4 * f1 + 3 * f2) * (f0 - 4 * f1 + 3 * f2);
+    double b1 = (13.0 / 12.0) * (f1 - 2 * f2 + f3) * (f1 - 2 * f2 + f3) + 0.25 * (f1 - f3) * (f1 - f3);
+    double b2 = (13.0 / 12.0) * (f2 - 2 * f3 + f4) * (f2 - 2 * f3 + f4) +
+                0.25 * (3 * f2 - 4 * f3 + f4) * (3 * f2 - 4 * f3 + f4);
+    double a0 = (1.0 / 16.0) / ((eps + b0) * (eps + b0));
+    double a1 = (5.0 / 8.0) / ((eps + b1) * (eps + b1));
+    double a2 = (5.0 / 16.0) / ((eps + b2) * (eps + b2));
+    double as = a0 + a1 + a2;
+    double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+    double p0 = (3 * f0 - 10 * f1 + 15 * f2) / 8.0;
+    double p1 = (-f1 + 6 * f2 + 3 * f3) / 8.0;
+    double p2 = (3 * f2 + 6 * f3 - f4) / 8.0;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
+// reconstruction.cpp:69-100 (weno5_reconstruction: Jiang-Shu 0.1/0.6/0.3)
+double weno5_js(const double *s) {
+    double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+    double eps = 1e-6;
+    double b0 = 13.0 / 12.0 * (f0 - 2 * f1 + f2) * (f0 - 2 * f1 + f2) +
+                0.25 * (f0 - 4 * f1 + 3 * f2) * (f0 - 4 * f1 + 3 * f2);
+    double b1 = 13.0 / 12.0 * (f1 - 2 * f2 + f3) * (f1 - 2 * f2 + f3) + 0.25 * (f1 - f3) * (f1 - f3);
+    double b2 = 13.0 / 12.0 * (f2 - 2 * f3 + f4) * (f2 - 2 * f3 + f4) +
+                0.25 * (3 * f2 - 4 * f3 + f4) * (3 * f2 - 4 * f3 + f4);
+    double a0 = 0.1 / ((eps + b0) * (eps + b0));
+    double a1 = 0.6 / ((eps + b1) * (eps + b1));
+    double a2 = 0.3 / ((eps + b2) * (eps + b2));
+    double as = a0 + a1 + a2;
+    double w0 = a0 / as, w1 = a1 / as, w2 = a2 / as;
+    double p0 = (2 * f0 - 7 * f1 + 11 * f2) / 6.0;
+    double p1 = (-f1 + 5 * f2 + 2 * f3) / 6.0;
+    double p2 = (2 * f2 + 5 * f3 - f4) / 6.0;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
+// interpolation.cpp:118-133
+double mdcd_linear(const double *s, const orc_params &P) {
+    double diss = P.mdcd_diss, disp = P.mdcd_disp;
+    return (3.0 * disp + 3.0 * diss) / 8.0 * s[0] + (-18.0 * disp - 30 * diss - 1.0) / 16.0 * s[1] +
+           (12.0 * disp + 60.0 * diss + 9.0) / 16.0 * s[2] + (12.0 * disp - 60.0 * diss + 9.0) / 16.0 * s[3] +
+           (-18.0 * disp + 30.0 * diss - 1.0) / 16.0 * s[4] + (3.0 * disp - 3.0 * diss) / 8.0 * s[5];
+}
+
+// interpolation.cpp:135-209.  sai > 0.4 selects the linear scheme (the code, not
+// the comment at 166-167, is followed).
+double mdcd_hybrid(const double *s, const orc_params &P) {
+    double diss = P.mdcd_diss, disp = P.mdcd_disp;
+    double eps_small = 1e-40;
+    double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4], f5 = s[5];
+    double eps = 0.9 * 0.4 / (1.0 - 0.9 * 0.4) * 1e-4;
+    double a1 = std::abs(f2 - f1) + std::abs(f2 - 2 * f1 + f0);
+    double b1 = std::abs(f2 - f3) + std::abs(f2 - 2 * f3 + f4);
+    double a2 = std::abs(f3 - f2) + std::abs(f3 - 2 * f2 + f1);
+    double b2 = std::abs(f3 - f4) + std::abs(f3 - 2 * f4 + f5);
+    double sai1 = (2 * a1 * b1 + eps) / (a1 * a1 + b1 * b1 + eps);
+    double sai2 = (2 * a2 * b2 + eps) / (a2 * a2 + b2 * b2 + eps);
+    double sai = std::min(sai1, sai2);
+    if (sai > 0.4) return mdcd_linear(s, P);
+    double q0 = (3 * f0 - 10 * f1 + 15 * f2) / 8.0;
+    double q1 = (-f1 + 6 * f2 + 3 * f3) / 8.0;
+    double q2 = (3 * f2 + 6 * f3 - f4) / 8.0;
+    double q3 = (15 * f3 - 10 * f4 + 3 * f5) / 8.0;
+    double d0 = 1.5 * (disp + diss);
+    double d1 = 0.5 - 1.5 * (disp - 3 * diss);
+    double d2 = 0.5 - 1.5 * (disp + 3 * diss);
+    double d3 = 1.5 * (disp - diss);
+    double beta0 = 13.0 / 12.0 * std::pow(f0 - 2 * f1 + f2, 2) + 0.25 * std::pow(f0 - 4 * f1 + 3 * f2, 2);
+    double beta1 = 13.0 / 12.0 * std::pow(f1 - 2 * f2 + f3, 2) + 0.25 * std::pow(f1 - f3, 2);
+    double beta2 = 13.0 / 12.0 * std::pow(f2 - 2 * f3 + f4, 2) + 0.25 * std::pow(3 * f2 - 4 * f3 + f4, 2);
+    double beta3 = (271799 * std::pow(f0, 2) +
+                    f0 * (-2380800 * f1 + 4086352 * f2 - 3462252 * f3 + 1458762 * f4 - 245620 * f5) +
+                    f1 * (5653317 * f1 - 20427884 * f2 + 17905032 * f3 - 7727988 * f4 + 1325006 * f5) +
+                    f2 * (19510972 * f2 - 35817664 * f3 + 15929912 * f4 - 2792660 * f5) +
+                    f3 * (17195652 * f3 - 15880404 * f4 + 2863984 * f5) + f4 * (3824847 * f4 - 1429976 * f5) +
+                    139633 * std::pow(f5, 2)) /
+                   120960.0;
+    double tau6 = std::abs(beta3 - (beta0 + 4 * beta1 + beta2) / 6.0);
+    double c0 = d0 * std::pow(20 + tau6 / (beta0 + eps_small), 2);
+    double c1 = d1 * std::pow(20 + tau6 / (beta1 + eps_small), 2);
+    double c2 = d2 * std::pow(20 + tau6 / (beta2 + eps_small), 2);
+    double c3 = d3 * std::pow(20 + tau6 / (beta3 + eps_small), 2);
+    return (c0 * q0 + c1 * q1 + c2 * q2 + c3 * q3) / (c0 + c1 + c2 + c3);
+}
+
+// interpolation.cpp:13-77: reverse the 6-point stencil for flag < 0, then
+// dispatch.  Interpolation id 4 routes to weno5_reconstruction
+// (reconstruction.cpp:69-100) in the same slot.
+double interpolate_select(const double *v, double flag, const orc_params &P) {
+    double a[6];
+    for (int i = 0; i < 6; ++i) a[i] = v[i];
+    if (flag < 0.0) std::reverse(a, a + 6);
+    switch (P.interpolation) {
+        case I_WENO5: return weno5_interpolate(a);
+        case I_MDCD_LINEAR: return mdcd_linear(a, P);
+        case I_MDCD_HYBRID: return mdcd_hybrid(a, P);
+        case I_WENO5_JS: return weno5_js(a);
+        default: return v[0];
+    }
+}
+
+// ---------------------------------------------------------------- Riemann solvers
+struct LR {  // conserved -> primitive unpacking shared by the solvers (InviscousFlux.cpp:276-287 etc.)
+    double rho, u, v, w, E, p, a;
+};
+inline LR unpack(const double *U, double gamma) {
+    LR s;
+    s.rho = U[0];
+    s.u = U[1] / s.rho;
+    s.v = U[2] / s.rho;
+    s.w = U[3] / s.rho;
+    s.E = U[4];
+    s.p = (U[4] - 0.5 * s.rho * (s.u * s.u + s.v * s.v + s.w * s.w)) * (gamma - 1.0);
+    s.a = std::sqrt(gamma * s.p / s.rho);
+    return s;
+}
+inline void phys_flux(const LR &s, const double *U, double nx, double ny, double nz, double *F, double &Vn) {
+    Vn = s.u * nx + s.v * ny + s.w * nz;
+    F[0] = s.rho * Vn;
+    F[1] = s.rho * s.u * Vn + s.p * nx;
+    F[2] = s.rho * s.v * Vn + s.p * ny;
+    F[3] = s.rho * s.w * Vn + s.p * nz;
+    F[4] = (U[4] + s.p) * Vn;
+}
+
+// InviscousFlux.cpp:849-884
+void roe_average(double &rho_bar, double &u_bar, double &v_bar, double &w_bar, double &h_bar, double &a_bar,
+                 const double *ul, const double *ur, double gamma) {
+    double rho_L = ul[0], u_L = ul[1], v_L = ul[2], w_L = ul[3], p_L = ul[4];
+    double E_L = rho_L * (0.5 * (u_L * u_L + v_L * v_L + w_L * w_L) + p_L / ((gamma - 1.0) * rho_L));
+    double rho_R = ur[0], u_R = ur[1], v_R = ur[2], w_R = ur[3], p_R = ur[4];
+    double E_R = rho_R * (0.5 * (u_R * u_R + v_R * v_R + w_R * w_R) + p_R / ((gamma - 1.0) * rho_R));
+    double sL = std::sqrt(rho_L), sR = std::sqrt(rho_R);
+    rho_bar = sL * sR;
+    double denom = 1.0 / (sL + sR);
+    u_bar = (sL * u_L + sR * u_R) * denom;
+    v_bar = (sL * v_L + sR * v_R) * denom;
+    w_bar = (sL * w_L + sR * w_R) * denom;
+    double H_L = (E_L + p_L) / rho_L;
+    double H_R = (E_R + p_R) / rho_R;
+    h_bar = (sL * H_L + sR * H_R) * denom;
+    double kin = 0.5 * (u_bar * u_bar + v_bar * v_bar + w_bar * w_bar);
+    a_bar = std::sqrt((gamma - 1.0) * (h_bar - kin));
+}
+
+// InviscousFlux.cpp:888-987 (Blazek-form left/right eigenvectors for normal n)
+void eigen_matrices(const double *ul, const double *ur, double nx, double ny, double nz, double gamma,
+                    double L[5][5], double R[5][5], double lam[5]) {
+    double rb, ub, vb, wb, Hb, ab;
+    roe_average(rb, ub, vb, wb, Hb, ab, ul, ur, gamma);
+    double V = nx * ub + ny * vb + nz * wb;
+    double c = ab;
+    lam[0] = V - c;
+    lam[1] = V;
+    lam[2] = V;
+    lam[3] = V;
+    lam[4] = V + c;
+    double phi = 0.5 * (gamma - 1.0) * (ub * ub + vb * vb + wb * wb);
+    double a1 = gamma - 1.0;
+    double a2 = 1.0 / (std::sqrt(2.0) * rb * c);
+    double a3 = rb / (std::sqrt(2.0) * c);
+    double a4 = (phi + c * c) / (gamma - 1.0);
+    double a5 = 1.0 - phi / (c * c);
+    double a6 = phi / (gamma - 1.0);
+    L[0][0] = a2 * (phi + c * V);
+    L[0][1] = -a2 * (a1 * ub + nx * c);
+    L[0][2] = -a2 * (a1 * vb + ny * c);
+    L[0][3] = -a2 * (a1 * wb + nz * c);
+    L[0][4] = a1 * a2;
+    L[1][0] = nx * a5 - (nz * vb - ny * wb) / rb;
+    L[1][1] = nx * a1 * ub / (c * c);
+    L[1][2] = nx * a1 * vb / (c * c) + nz / rb;
+    L[1][3] = nx * a1 * wb / (c * c) - ny / rb;
+    L[1][4] = -nx * a1 / (c * c);
+    L[2][0] = nz * a5 - (ny * ub - nx * vb) / rb;
+    L[2][1] = nz * a1 * ub / (c * c) + ny / rb;
+    L[2][2] = nz * a1 * vb / (c * c) - nx / rb;
+    L[2][3] = nz * a1 * wb / (c * c);
+    L[2][4] = -nz * a1 / (c * c);
+    L[3][0] = ny * a5 - (nx * wb - nz * ub) / rb;
+    L[3][1] = ny * a1 * ub / (c * c) - nz / rb;
+    L[3][2] = ny * a1 * vb / (c * c);
+    L[3][3] = ny * a1 * wb / (c * c) + nx / rb;
+    L[3][4] = -ny * a1 / (c * c);
+    L[4][0] = a2 * (phi - c * V);
+    L[4][1] = -a2 * (a1 * ub - nx * c);
+    L[4][2] = -a2 * (a1 * vb - ny * c);
+    L[4][3] = -a2 * (a1 * wb - nz * c);
+    L[4][4] = a1 * a2;
+    R[0][0] = a3;
+    R[1][0] = a3 * (ub - nx * c);
+    R[2][0] = a3 * (vb - ny * c);
+    R[3][0] = a3 * (wb - nz * c);
+    R[4][0] = a3 * (a4 - c * V);
+    R[0][1] = nx;
+    R[1][1] = nx * ub;
+    R[2][1] = nx * vb + nz * rb;
+    R[3][1] = nx * wb - ny * rb;
+    R[4][1] = nx * a6 + rb * (nz * vb - ny * wb);
+    R[0][2] = nz;
+    R[1][2] = nz * ub + ny * rb;
+    R[2][2] = nz * vb - nx * rb;
+    R[3][2] = nz * wb;
+    R[4][2] = nz * a6 + rb * (ny * ub - nx * vb);
+    R[0][3] = ny;
+    R[1][3] = ny * ub - nz * rb;
+    R[2][3] = ny * vb;
+    R[3][3] = ny * wb + nx * rb;
+    R[4][3] = ny * a6 + rb * (nx * wb - nz * ub);
+    R[0][4] = a3;
+    R[1][4] = a3 * (ub + nx * c);
+    R[2][4] = a3 * (vb + ny * c);
+    R[3][4] = a3 * (wb + nz * c);
+    R[4][4] = a3 * (a4 + c * V);
+}
+
+// InviscousFlux.cpp:270-311
+void rusanov(double *F, const double *UL, const double *UR, double nx, double ny, double nz, double g) {
+    LR L = unpack(UL, g), R = unpack(UR, g);
+    double FL[5], FR[5], VnL, VnR;
+    phys_flux(L, UL, nx, ny, nz, FL, VnL);
+    phys_flux(R, UR, nx, ny, nz, FR, VnR);
+    double smax = std::max(std::abs(L.u * nx + L.v * ny + L.w * nz) + L.a,
+                           std::abs(R.u * nx + R.v * ny + R.w * nz) + R.a);
+    for (int n = 0; n < 5; ++n) F[n] = 0.5 * (FL[n] + FR[n]) - 0.5 * smax * (UR[n] - UL[n]);
+}
+
+// InviscousFlux.cpp:313-382 (entropy fix delta = 0.1 max|lambda|, 366-373)
+void roe(double *F, const double *UL, const double *UR, double nx, double ny, double nz, double g) {
+    LR L = unpack(UL, g), R = unpack(UR, g);
+    double FL[5], FR[5], VnL, VnR;
+    phys_flux(L, UL, nx, ny, nz, FL, VnL);
+    phys_flux(R, UR, nx, ny, nz, FR, VnR);
+    const double ul[5] = {L.rho, L.u, L.v, L.w, L.p};
+    const double ur[5] = {R.rho, R.u, R.v, R.w, R.p};
+    double Lm[5][5], Rm[5][5], lam[5];
+    eigen_matrices(ul, ur, nx, ny, nz, g, Lm, Rm, lam);
+    double alpha[5];
+    for (int m = 0; m < 5; ++m) {
+        alpha[m] = 0.0;
+        for (int n = 0; n < 5; ++n) alpha[m] += Lm[m][n] * (UR[n] - UL[n]);
+    }
+    double mx = 0.0;
+    for (int m = 0; m < 5; ++m) mx = std::max(mx, std::abs(lam[m]));
+    double delta = 0.1 * mx;
+    for (int m = 0; m < 5; ++m)
+        if (std::abs(lam[m]) < delta) lam[m] = (lam[m] * lam[m] + delta * delta) / (2.0 * delta);
+    for (int n = 0; n < 5; ++n) {
+        F[n] = 0.5 * (FL[n] + FR[n]);
+        for (int m = 0; m < 5; ++m) F[n] -= 0.5 * std::abs(lam[m]) * alpha[m] * Rm[n][m];
+    }
+}
+
+// InviscousFlux.cpp:384-447
+void hll(double *F, const double *UL, const double *UR, double nx, double ny, double nz, double g) {
+    LR L = unpack(UL, g), R = unpack(UR, g);
+    double FL[5], FR[5], unL, unR;
+    phys_flux(L, UL, nx, ny, nz, FL, unL);
+    phys_flux(R, UR, nx, ny, nz, FR, unR);
+    double SL = std::min(unL - L.a, unR - R.a);
+    double SR = std::max(unL + L.a, unR + R.a);
+    if (SL >= 0.0) {
+        for (int i = 0; i < 5; ++i) F[i] = FL[i];
+    } else if (SR <= 0.0) {
+        for (int i = 0; i < 5; ++i) F[i] = FR[i];
+    } else {
+        double denom = SR - SL;
+        double f1 = SR / denom, f2 = SL / denom, f3 = SL * SR / denom;
+        for (int i = 0; i < 5; ++i) F[i] = (f1 * FL[i] - f2 * FR[i] + f3 * (UR[i] - UL[i]));
+    }
+}
+
+// Star-region flux shared by HLLC and HLLC_p (InviscousFlux.cpp:538-591, 663-716).
+void hllc_star(double *F, const LR &L, const LR &R, const double *UL, const double *UR, const double *FL,
+               const double *FR, double unL, double unR, double SL, double SR, double SM, double nx, double ny,
+               double nz) {
+    if (SM >= 0.0) {
+        double rs = L.rho * (SL - unL) / (SL - SM);
+        double tx = L.u - unL * nx, ty = L.v - unL * ny, tz = L.w - unL * nz;
+        double sx = SM * nx + tx, sy = SM * ny + ty, sz = SM * nz + tz;
+        double Es = L.E / L.rho + (SM - unL) * (SM + L.p / (L.rho * (SL - unL)));
+        double Us[5] = {rs, rs * sx, rs * sy, rs * sz, rs * Es};
+        for (int i = 0; i < 5; ++i) F[i] = FL[i] + SL * (Us[i] - UL[i]);
+    } else {
+        double rs = R.rho * (SR - unR) / (SR - SM);
+        double tx = R.u - unR * nx, ty = R.v - unR * ny, tz = R.w - unR * nz;
+        double sx = SM * nx + tx, sy = SM * ny + ty, sz = SM * nz + tz;
+        double Es = R.E / R.rho + (SM - unR) * (SM + R.p / (R.rho * (SR - unR)));
+        double Us[5] = {rs, rs * sx, rs * sy, rs * sz, rs * Es};
+        for (int i = 0; i < 5; ++i) F[i] = FR[i] + SR * (Us[i] - UR[i]);
+    }
+}
+
+// InviscousFlux.cpp:449-592
+void hllc(double *F, const double *UL, const double *UR, double nx, double ny, double nz, double g) {
+    LR L = unpack(UL, g), R = unpack(UR, g);
+    double FL[5], FR[5], unL, unR;
+    phys_flux(L, UL, nx, ny, nz, FL, unL);
+    phys_flux(R, UR, nx, ny, nz, FR, unR);
+    double SL = std::min(unL - L.a, unR - R.a);
+    double SR = std::max(unL + L.a, unR + R.a);
+    if (SL >= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FL[i]; return; }
+    if (SR <= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FR[i]; return; }
+    double sqL = std::sqrt(L.rho), sqR = std::sqrt(R.rho), ssum = sqL + sqR;
+    double u_roe = (sqL * L.u + sqR * R.u) / ssum;
+    double v_roe = (sqL * L.v + sqR * R.v) / ssum;
+    double w_roe = (sqL * L.w + sqR * R.w) / ssum;
+    double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+    double SM = un_roe;
+    double num = R.rho * unR * (SR - unR) - L.rho * unL * (SL - unL) + L.p - R.p;
+    double den = R.rho * (SR - unR) - L.rho * (SL - unL);
+    if (std::fabs(den) > 1e-12) SM = num / den; else SM = un_roe;
+    hllc_star(F, L, R, UL, UR, FL, FR, unL, unR, SL, SR, SM, nx, ny, nz);
+}
+
+// InviscousFlux.cpp:594-717 (pressure-based wave speeds, Toro)
+void hllc_p(double *F, const double *UL, const double *UR, double nx, double ny, double nz, double g) {
+    LR L = unpack(UL, g), R = unpack(UR, g);
+    double FL[5], FR[5], unL, unR;
+    phys_flux(L, UL, nx, ny, nz, FL, unL);
+    phys_flux(R, UR, nx, ny, nz, FR, unR);
+    double ps = 0.5 * (L.p + R.p) - 0.125 * (unR - unL) * (L.rho + R.rho) * (L.a + R.a);
+    ps = std::max(0.0, ps);
+    double SL, SR;
+    if (ps <= L.p) SL = unL - L.a;
+    else SL = unL - L.a * std::sqrt((g + 1.0) * ps / (2.0 * g * L.p) + (g - 1.0) / (2.0 * g));
+    if (ps <= R.p) SR = unR + R.a;
+    else SR = unR + R.a * std::sqrt((g + 1.0) * ps / (2.0 * g * R.p) + (g - 1.0) / (2.0 * g));
+    if (SL >= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FL[i]; return; }
+    if (SR <= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FR[i]; return; }
+    double SM = (R.p - L.p + L.rho * unL * (SL - unL) - R.rho * unR * (SR - unR)) /
+                (L.rho * (SL - unL) - R.rho * (SR - unR));
+    hllc_star(F, L, R, UL, UR, FL, FR, unL, unR, SL, SR, SM, nx, ny, nz);
+}
+
+// InviscousFlux.cpp:719-846.  Note the reference's p_half uses the x-velocity
+// difference (u_R - u_L) for every face direction (line 823); reproduced.
+void ausm(double *F, const double *UL, const double *UR, double nx, double ny, double nz, double g) {
+    const double Kp = 0.25, Ku = 0.75;
+    double rho_L = UL[0], u_L = UL[1] / rho_L, v_L = UL[2] / rho_L, w_L = UL[3] / rho_L, E_L = UL[4];
+    double p_L = (E_L - 0.5 * rho_L * (u_L * u_L + v_L * v_L + w_L * w_L)) * (g - 1.0);
+    p_L = std::max(p_L, 1e-12);
+    double a_L = std::sqrt(g * p_L / rho_L);
+    double H_L = (E_L + p_L) / rho_L;
+    double rho_R = UR[0], u_R = UR[1] / rho_R, v_R = UR[2] / rho_R, w_R = UR[3] / rho_R, E_R = UR[4];
+    double p_R = (E_R - 0.5 * rho_R * (u_R * u_R + v_R * v_R + w_R * w_R)) * (g - 1.0);
+    p_R = std::max(p_R, 1e-12);
+    double a_R = std::sqrt(g * p_R / rho_R);
+    double H_R = (E_R + p_R) / rho_R;
+    double un_L = u_L * nx + v_L * ny + w_L * nz;
+    double un_R = u_R * nx + v_R * ny + w_R * nz;
+    double a_bar = 0.5 * (a_L + a_R);
+    double rho_bar = 0.5 * (rho_L + rho_R);
+    double ML = un_L / a_bar, MR = un_R / a_bar;
+    double M_bar2 = (un_L * un_L + un_R * un_R) / (2.0 * a_bar * a_bar);
+    double M_o2 = std::min(1.0, std::max(M_bar2, 0.5));
+    double f_a = M_o2 * (2.0 - M_o2);
+    auto M4p = [](double M) { return std::abs(M) >= 1.0 ? 0.5 * (M + std::abs(M)) : 0.25 * (M + 1.0) * (M + 1.0); };
+    auto M4m = [](double M) { return std::abs(M) >= 1.0 ? 0.5 * (M - std::abs(M)) : -0.25 * (M - 1.0) * (M - 1.0); };
+    double Mp = M4p(ML), Mm = M4m(MR);
+    double M_half = Mp + Mm - Kp / f_a * std::max(1.0 - M_bar2, 0.0) * ((p_R - p_L) / (rho_bar * a_bar * a_bar));
+    double m_dot = a_bar * M_half * (M_half >= 0.0 ? rho_L : rho_R);
+    auto P5p = [](double M) {
+        return std::abs(M) >= 1.0 ? 0.5 * (1.0 + (M > 0 ? 1.0 : -1.0)) : 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M);
+    };
+    auto P5m = [](double M) {
+        return std::abs(M) >= 1.0 ? 0.5 * (1.0 - (M > 0 ? 1.0 : -1.0)) : 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M);
+    };
+    double pp = P5p(ML), pm = P5m(MR);
+    double p_half = pp * p_L + pm * p_R - Ku * pm * pp * (rho_L + rho_R) * f_a * (u_R - u_L) * a_bar;
+    double uf, vf, wf, Hf;
+    if (m_dot >= 0.0) { uf = u_L; vf = v_L; wf = w_L; Hf = H_L; }
+    else { uf = u_R; vf = v_R; wf = w_R; Hf = H_R; }
+    F[0] = m_dot;
+    F[1] = m_dot * uf + p_half * nx;
+    F[2] = m_dot * vf + p_half * ny;
+    F[3] = m_dot * wf + p_half * nz;
+    F[4] = m_dot * Hf;
+}
+
+void riemann(int which, double *F, const double *UL, const double *UR, double nx, double ny, double nz, double g) {
+    // InviscousFlux.cpp:245-267; an unknown id leaves F untouched (zero), as the reference does.
+    switch (which) {
+        case R_ROE: roe(F, UL, UR, nx, ny, nz, g); break;
+        case R_RUSANOV: rusanov(F, UL, UR, nx, ny, nz, g); break;
+        case R_HLLC: hllc(F, UL, UR, nx, ny, nz, g); break;
+        case R_HLL: hll(F, UL, UR, nx, ny, nz, g); break;
+        case R_HLLC_P: hllc_p(F, UL, UR, nx, ny, nz, g); break;
+        case R_AUSM: ausm(F, UL, UR, nx, ny, nz, g); break;
+        default: break;
+    }
+}
+
+// InviscousFlux.cpp:146-268.  Ut/ut are [5][6] conserved/primitive stencils
+// (cells i-2..i+3 of face i+1/2).
+void wcns_face_flux(double *F, const double Ut[5][6], const double ut[5][6], const orc_params &P, int dim) {
+    double nx = 0.0, ny = 0.0, nz = 0.0;
+    if (dim == 0) nx = 1.0;
+    if (dim == 1) ny = 1.0;
+    if (dim == 2) nz = 1.0;
+    double UL[5], UR[5];
+    if (P.char_recon) {
+        double Lm[5][5], Rm[5][5], lam[5], LU[5][6], Q[5];
+        for (int side = 0; side < 2; ++side) {
+            int c = side == 0 ? 2 : 3;  // cell i for the left state, cell i+1 for the right
+            const double us[5] = {ut[0][c], ut[1][c], ut[2][c], ut[3][c], ut[4][c]};
+            eigen_matrices(us, us, nx, ny, nz, P.gamma, Lm, Rm, lam);
+            for (int m = 0; m < 6; ++m)
+                for (int n = 0; n < 5; ++n) {
+                    double s = 0.0;
+                    for (int r = 0; r < 5; ++r) s += Lm[n][r] * Ut[r][m];
+                    LU[n][m] = s;
+                }
+            for (int n = 0; n < 5; ++n) Q[n] = interpolate_select(LU[n], side == 0 ? +1.0 : -1.0, P);
+            double *out = side == 0 ? UL : UR;
+            for (int n = 0; n < 5; ++n) {
+                double s = 0.0;
+                for (int r = 0; r < 5; ++r) s += Rm[n][r] * Q[r];
+                out[n] = s;
+            }
+        }
+    } else {
+        for (int n = 0; n < 5; ++n) {
+            UL[n] = interpolate_select(Ut[n], +1.0, P);
+            UR[n] = interpolate_select(Ut[n], -1.0, P);
+        }
+    }
+    for (int n = 0; n < 5; ++n) F[n] = 0.0;
+    riemann(P.riemann_solver, F, UL, UR, nx, ny, nz, P.gamma);
+}
+
+// ---------------------------------------------------------------- field container
+struct Ctx {
+    int nx, ny, nz, ng, sx, sy, sz;
+    double dx, dy, dz;
+    orc_params P;
+    // conserved, U0, primitives, RHS (field_structures.h:202-220)
+    std::vector<double> rho, rhou, rhov, rhow, E, rho0, rhou0, rhov0, rhow0, E0;
+    std::vector<double> u, v, w, p, T, rhs[5];
+    std::vector<double> fx[5], fy[5], fz[5];  // half-node fluxes (field_structures.h:240-243)
+    std::vector<double> grad[12];             // du_dx du_dy du_dz dv_.. dw_.. dT_.. (233-236)
+    std::vector<double> Fv[5], Hv[5], Gv[5];  // viscous fluxes, x / y / z (226-228)
+    int I(int i, int j, int k) const { return (k * sy + j) * sx + i; }
+    int IFX(int i, int j, int k) const { return (k * sy + j) * (sx - 1) + i; }
+    int IFY(int i, int j, int k) const { return (k * (sy - 1) + j) * sx + i; }
+    int IFZ(int i, int j, int k) const { return (k * sy + j) * sx + i; }
+    std::vector<double> *cons(int n) {
+        std::vector<double> *c[5] = {&rho, &rhou, &rhov, &rhow, &E};
+        return c[n];
+    }
+};
+
+// field_structures.h:444-465
+void conserved_to_primitive(Ctx &F) {
+    const double gamma = F.P.gamma;
+    const int N = F.sx * F.sy * F.sz;
+#pragma omp parallel for schedule(static)
+    for (int id = 0; id < N; ++id) {
+        double rr = F.rho[id];
+        if (rr <= 0.0) { rr = 1e-12; F.rho[id] = rr; }
+        F.u[id] = F.rhou[id] / rr;
+        F.v[id] = F.rhov[id] / rr;
+        F.w[id] = F.rhow[id] / rr;
+        double kin = 0.5 * (F.u[id] * F.u[id] + F.v[id] * F.v[id] + F.w[id] * F.w[id]);
+        double e = F.E[id] / rr - kin;
+        F.p[id] = (gamma - 1.0) * rr * e;
+        F.T[id] = F.p[id] / (rr * F.P.Rgas);
+    }
+}
+
+// field_structures.h:468-488
+void primitive_to_conserved(Ctx &F) {
+    const double gamma = F.P.gamma;
+    const int N = F.sx * F.sy * F.sz;
+#pragma omp parallel for schedule(static)
+    for (int id = 0; id < N; ++id) {
+        double rr = F.rho[id];
+        if (rr <= 0.0) { rr = 1e-12; F.rho[id] = rr; }
+        F.rhou[id] = rr * F.u[id];
+        F.rhov[id] = rr * F.v[id];
+        F.rhow[id] = rr * F.w[id];
+        double kin = 0.5 * (F.u[id] * F.u[id] + F.v[id] * F.v[id] + F.w[id] * F.w[id]);
+        double e = F.p[id] / ((gamma - 1.0) * rr);
+        F.E[id] = rr * (e + kin);
+        F.T[id] = F.p[id] / (rr * F.P.Rgas);
+    }
+}
+
+// Periodic one-rank image of a ghost layer: cells [0,ng) take i+n, cells
+// [ng+n, ng+2ng) take i-n (field_structures.h:517-649 pack/unpack ranges).
+template <class Fn>
+void for_face_ghosts(const Ctx &F, Fn fn) {
+    const int ng = F.ng, nx = F.nx, ny = F.ny, nz = F.nz;
+    // x faces: j,k interior (field_structures.h:514-559)
+    for (int k = ng; k < ng + nz; ++k)
+        for (int j = ng; j < ng + ny; ++j)
+            for (int g = 0; g < ng; ++g) {
+                fn(F.I(g, j, k), F.I(g + nx, j, k));
+                fn(F.I(ng + nx + g, j, k), F.I(ng + g, j, k));
+            }
+    // y faces: i,k interior (561-605)
+    for (int k = ng; k < ng + nz; ++k)
+        for (int g = 0; g < ng; ++g)
+            for (int i = ng; i < ng + nx; ++i) {
+                fn(F.I(i, g, k), F.I(i, g + ny, k));
+                fn(F.I(i, ng + ny + g, k), F.I(i, ng + g, k));
+            }
+    // z faces: i,j interior (607-649)
+    for (int g = 0; g < ng; ++g)
+        for (int j = ng; j < ng + ny; ++j)
+            for (int i = ng; i < ng + nx; ++i) {
+                fn(F.I(i, j, g), F.I(i, j, g + nz));
+                fn(F.I(i, j, ng + nz + g), F.I(i, j, ng + g));
+            }
+}
+
+// exchange_halos_physical (field_structures.h:936-994): rho,u,v,w,p only.
+void exchange_physical(Ctx &F) {
+    for_face_ghosts(F, [&](int d, int s) {
+        F.rho[d] = F.rho[s];
+        F.u[d] = F.u[s];
+        F.v[d] = F.v[s];
+        F.w[d] = F.w[s];
+        F.p[d] = F.p[s];
+    });
+}
+
+// exchange_halos_viscous_flux (field_structures.h:877-932): 12 viscous fluxes.
+void exchange_viscous(Ctx &F) {
+    for_face_ghosts(F, [&](int d, int s) {
+        for (int n = 1; n < 5; ++n) {
+            F.Fv[n][d] = F.Fv[n][s];
+            F.Hv[n][d] = F.Hv[n][s];
+            F.Gv[n][d] = F.Gv[n][s];
+        }
+    });
+}
+
+// exchange_halos_halfnode_flux (field_structures.h:773-874, 998-1056): the
+// ng-1 missing ghost faces per side.  Left ghost faces [0,ng-1) take face
+// f+n; right ghost faces [ng+n, ng+n+ng-1) take face f-n.
+void exchange_halfnode(Ctx &F) {
+    const int ng = F.ng, nx = F.nx, ny = F.ny, nz = F.nz;
+    for (int v = 0; v < 5; ++v) {
+        std::vector<double> &X = F.fx[v], &Y = F.fy[v], &Z = F.fz[v];
+        for (int k = ng; k < ng + nz; ++k)
+            for (int j = ng; j < ng + ny; ++j)
+                for (int g = 0; g < ng - 1; ++g) {
+                    X[F.IFX(g, j, k)] = X[F.IFX(g + nx, j, k)];
+                    X[F.IFX(ng + nx + g, j, k)] = X[F.IFX(ng + g, j, k)];
+                }
+        for (int k = ng; k < ng + nz; ++k)
+            for (int i = ng; i < ng + nx; ++i)
+                for (int g = 0; g < ng - 1; ++g) {
+                    Y[F.IFY(i, g, k)] = Y[F.IFY(i, g + ny, k)];
+                    Y[F.IFY(i, ng + ny + g, k)] = Y[F.IFY(i, ng + g, k)];
+                }
+        for (int i = ng; i < ng + nx; ++i)
+            for (int j = ng; j < ng + ny; ++j)
+                for (int g = 0; g < ng - 1; ++g) {
+                    Z[F.IFZ(i, j, g)] = Z[F.IFZ(i, j, g + nz)];
+                    Z[F.IFZ(i, j, ng + nz + g)] = Z[F.IFZ(i, j, ng + g)];
+                }
+    }
+}
+
+// apply_boundary (boundary.cpp:14-54) on a periodic rank: the exchange only.
+void apply_boundary(Ctx &F) { exchange_physical(F); }
+
+// compute_invis_flux (InviscousFlux.cpp:10-144): faces i in [ng-1, ng+n-1],
+// tangential indices interior, stencil cells i-2..i+3 (mid = (6-1)/2).
+void compute_invis_flux(Ctx &F) {
+    const int ng = F.ng;
+    const std::vector<double> *C[5] = {&F.rho, &F.rhou, &F.rhov, &F.rhow, &F.E};
+    const std::vector<double> *Q[5] = {&F.rho, &F.u, &F.v, &F.w, &F.p};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int n0 = dim == 0 ? F.nx : dim == 1 ? F.ny : F.nz;
+        const int na = dim == 0 ? F.ny : F.nx;  // first tangential extent
+        const int nb = dim == 2 ? F.ny : F.nz;  // second tangential extent
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int b = ng; b < ng + nb; ++b)
+            for (int a = ng; a < ng + na; ++a)
+                for (int f = ng - 1; f < ng + n0; ++f) {
+                    double Ut[5][6], ut[5][6];
+                    for (int m = 0; m < 6; ++m) {
+                        int c = f + m - 2;
+                        int id = dim == 0 ? F.I(c, a, b) : dim == 1 ? F.I(a, c, b) : F.I(a, b, c);
+                        for (int n = 0; n < 5; ++n) {
+                            Ut[n][m] = (*C[n])[id];
+                            ut[n][m] = (*Q[n])[id];
+                        }
+                    }
+                    double Fc[5];
+                    wcns_face_flux(Fc, Ut, ut, F.P, dim);
+                    int fid = dim == 0 ? F.IFX(f, a, b) : dim == 1 ? F.IFY(a, f, b) : F.IFZ(a, b, f);
+                    std::vector<double> *dst = dim == 0 ? F.fx : dim == 1 ? F.fy : F.fz;
+                    for (int n = 0; n < 5; ++n) dst[n][fid] = Fc[n];
+                }
+    }
+}
+
+// diff_6th_central_half (InviscousFlux.cpp:990-992) on faces f[0..5] = i-3..i+2
+inline double d6_half(double f0, double f1, double f2, double f3, double f4, double f5, double h) {
+    return ((f3 - f2) * (75.0 / 64.0) + (f4 - f1) * (-25.0 / 384.0) + (f5 - f0) * (3.0 / 640.0)) / h;
+}
+
+// compute_invis_dflux (InviscousFlux.cpp:1027-1056): per variable x, y, z.
+void compute_invis_dflux(Ctx &F) {
+    const int ng = F.ng;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = ng; k < ng + F.nz; ++k)
+        for (int j = ng; j < ng + F.ny; ++j)
+            for (int i = ng; i < ng + F.nx; ++i) {
+                int id = F.I(i, j, k);
+                for (int n = 0; n < 5; ++n) {
+                    const std::vector<double> &X = F.fx[n], &Y = F.fy[n], &Z = F.fz[n];
+                    F.rhs[n][id] -= d6_half(X[F.IFX(i - 3, j, k)], X[F.IFX(i - 2, j, k)], X[F.IFX(i - 1, j, k)],
+                                            X[F.IFX(i, j, k)], X[F.IFX(i + 1, j, k)], X[F.IFX(i + 2, j, k)], F.dx);
+                    F.rhs[n][id] -= d6_half(Y[F.IFY(i, j - 3, k)], Y[F.IFY(i, j - 2, k)], Y[F.IFY(i, j - 1, k)],
+                                            Y[F.IFY(i, j, k)], Y[F.IFY(i, j + 1, k)], Y[F.IFY(i, j + 2, k)], F.dy);
+                    F.rhs[n][id] -= d6_half(Z[F.IFZ(i, j, k - 3)], Z[F.IFZ(i, j, k - 2)], Z[F.IFZ(i, j, k - 1)],
+                                            Z[F.IFZ(i, j, k)], Z[F.IFZ(i, j, k + 1)], Z[F.IFZ(i, j, k + 2)], F.dz);
+                }
+            }
+}
+
+// diff_6th_central (ViscousFlux.cpp:27-29) around index 3 of a 7-point line
+inline double d6(const double *f, double h) {
+    return (f[6] - 9.0 * f[5] + 45.0 * f[4] - 45.0 * f[2] + 9.0 * f[1] - f[0]) / (60.0 * h);
+}
+inline double dline(const std::vector<double> &a, int id, int stride, double h) {
+    double f[7];
+    for (int m = 0; m < 7; ++m) f[m] = a[id + (m - 3) * stride];
+    return d6(f, h);
+}
+
+// compute_gradients (ViscousFlux.cpp:109-148); periodic => 6th order everywhere.
+void compute_gradients(Ctx &F) {
+    const int ng = F.ng, sx = F.sx, sxy = F.sx * F.sy;
+    const std::vector<double> *src[4] = {&F.u, &F.v, &F.w, &F.T};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = ng; k < ng + F.nz; ++k)
+        for (int j = ng; j < ng + F.ny; ++j)
+            for (int i = ng; i < ng + F.nx; ++i) {
+                int id = F.I(i, j, k);
+                for (int q = 0; q < 4; ++q) {
+                    F.grad[3 * q + 0][id] = dline(*src[q], id, 1, F.dx);
+                    F.grad[3 * q + 1][id] = dline(*src[q], id, sx, F.dy);
+                    F.grad[3 * q + 2][id] = dline(*src[q], id, sxy, F.dz);
+                }
+            }
+}
+
+// compute_viscous_flux + SolverParams::get_mu (ViscousFlux.cpp:151-208,
+// field_structures.h:96-106).  F = x, H = y, G = z fluxes.
+void compute_viscous_flux(Ctx &F) {
+    const int ng = F.ng;
+    const orc_params &P = F.P;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = ng; k < ng + F.nz; ++k)
+        for (int j = ng; j < ng + F.ny; ++j)
+            for (int i = ng; i < ng + F.nx; ++i) {
+                int id = F.I(i, j, k);
+                double Tc = F.T[id];
+                double mu = P.mu * std::pow(Tc, 1.5) * (1.0 + P.S_ref) / (Tc + P.S_ref);
+                const double ux = F.grad[0][id], uy = F.grad[1][id], uz = F.grad[2][id];
+                const double vx = F.grad[3][id], vy = F.grad[4][id], vz = F.grad[5][id];
+                const double wx = F.grad[6][id], wy = F.grad[7][id], wz = F.grad[8][id];
+                const double Tx = F.grad[9][id], Ty = F.grad[10][id], Tz = F.grad[11][id];
+                double divU = ux + vy + wz;
+                double txx = 2.0 * mu * ux - (2.0 / 3.0) * mu * divU;
+                double tyy = 2.0 * mu * vy - (2.0 / 3.0) * mu * divU;
+                double tzz = 2.0 * mu * wz - (2.0 / 3.0) * mu * divU;
+                double txy = mu * (uy + vx);
+                double txz = mu * (uz + wx);
+                double tyz = mu * (vz + wy);
+                double qx = mu * P.Cp / P.Pr * Tx;
+                double qy = mu * P.Cp / P.Pr * Ty;
+                double qz = mu * P.Cp / P.Pr * Tz;
+                double uu = F.u[id], vv = F.v[id], ww = F.w[id];
+                F.Fv[0][id] = 0.0;
+                F.Fv[1][id] = txx;
+                F.Fv[2][id] = txy;
+                F.Fv[3][id] = txz;
+                F.Fv[4][id] = uu * txx + vv * txy + ww * txz + qx;
+                F.Hv[0][id] = 0.0;
+                F.Hv[1][id] = txy;
+                F.Hv[2][id] = tyy;
+                F.Hv[3][id] = tyz;
+                F.Hv[4][id] = uu * txy + vv * tyy + ww * tyz + qy;
+                F.Gv[0][id] = 0.0;
+                F.Gv[1][id] = txz;
+                F.Gv[2][id] = tyz;
+                F.Gv[3][id] = tzz;
+                F.Gv[4][id] = uu * txz + vv * tyz + ww * tzz + qz;
+            }
+}
+
+// compute_vis_flux (ViscousFlux.cpp:210-255): rhs += d/dx Fv + d/dy Hv + d/dz Gv
+// for momentum and energy (the mass term is commented out in the reference).
+void compute_vis_flux(Ctx &F) {
+    const int ng = F.ng, sx = F.sx, sxy = F.sx * F.sy;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = ng; k < ng + F.nz; ++k)
+        for (int j = ng; j < ng + F.ny; ++j)
+            for (int i = ng; i < ng + F.nx; ++i) {
+                int id = F.I(i, j, k);
+                for (int n = 1; n < 5; ++n) {
+                    F.rhs[n][id] += dline(F.Fv[n], id, 1, F.dx);
+                    F.rhs[n][id] += dline(F.Hv[n], id, sx, F.dy);
+                    F.rhs[n][id] += dline(F.Gv[n], id, sxy, F.dz);
+                }
+            }
+}
+
+// compute_rhs (range_kutta.cpp:11-43)
+void compute_rhs(Ctx &F) {
+    for (int n = 0; n < 5; ++n) std::fill(F.rhs[n].begin(), F.rhs[n].end(), 0.0);
+    compute_invis_flux(F);
+    exchange_halfnode(F);
+    compute_invis_dflux(F);
+    compute_gradients(F);
+    compute_viscous_flux(F);
+    exchange_viscous(F);
+    compute_vis_flux(F);
+}
+
+// runge_kutta_3 (range_kutta.cpp:46-92): Shu-Osher stages on all cells, each
+// followed by C2P -> apply_boundary -> P2C.
+void runge_kutta_3(Ctx &F, double dt) {
+    const int N = F.sx * F.sy * F.sz;
+    std::vector<double> *U[5] = {&F.rho, &F.rhou, &F.rhov, &F.rhow, &F.E};
+    std::vector<double> *U0[5] = {&F.rho0, &F.rhou0, &F.rhov0, &F.rhow0, &F.E0};
+    for (int stage = 0; stage < 3; ++stage) {
+        compute_rhs(F);
+        for (int v = 0; v < 5; ++v) {
+            double *u = U[v]->data();
+            const double *u0 = U0[v]->data(), *r = F.rhs[v].data();
+            if (stage == 0) {
+#pragma omp parallel for schedule(static)
+                for (int n = 0; n < N; ++n) u[n] = u0[n] + dt * r[n];
+            } else if (stage == 1) {
+#pragma omp parallel for schedule(static)
+                for (int n = 0; n < N; ++n) u[n] = 0.75 * u0[n] + 0.25 * (u[n] + dt * r[n]);
+            } else {
+#pragma omp parallel for schedule(static)
+                for (int n = 0; n < N; ++n) u[n] = (1.0 / 3.0) * u0[n] + (2.0 / 3.0) * (u[n] + dt * r[n]);
+            }
+        }
+        conserved_to_primitive(F);
+        apply_boundary(F);
+        primitive_to_conserved(F);
+    }
+}
+
+// compute_timestep (timestep.cpp:5-23): x-velocity and dx only.
+double compute_timestep(Ctx &F) {
+    double dtl = 1e9;
+    const int ng = F.ng;
+    for (int k = ng; k < ng + F.nz; ++k)
+        for (int j = ng; j < ng + F.ny; ++j)
+            for (int i = ng; i < ng + F.nx; ++i) {
+                int id = F.I(i, j, k);
+                double rho = F.rho[id];
+                double u = F.rhou[id] / rho;
+                double p = (F.P.gamma - 1.) * (F.E[id] - 0.5 * rho * u * u);
+                double a = std::sqrt(F.P.gamma * p / rho);
+                dtl = std::min(dtl, F.dx / (std::fabs(u) + a));
+            }
+    return F.P.cfl * dtl;
+}
+
+// field_structures.h:491-497
+void record_conserved_to0(Ctx &F) {
+    F.rho0 = F.rho;
+    F.rhou0 = F.rhou;
+    F.rhov0 = F.rhov;
+    F.rhow0 = F.rhow;
+    F.E0 = F.E;
+}
+
+// compute_total_energy (postprocess.cpp:103-128): sum 0.5 rho |u|^2 dV over the interior.
+double total_kinetic(const Ctx &F) {
+    const double vol = F.dx * F.dy * F.dz;
+    double s = 0.0;
+    const int ng = F.ng;
+    for (int k = ng; k < ng + F.nz; ++k)
+        for (int j = ng; j < ng + F.ny; ++j)
+            for (int i = ng; i < ng + F.nx; ++i) {
+                int id = F.I(i, j, k);
+                double uu = F.u[id], vv = F.v[id], ww = F.w[id];
+                double kin = 0.5 * F.rho[id] * (uu * uu + vv * vv + ww * ww);
+                double eint = 0;
+                s += (kin + eint) * vol;
+            }
+    return s;
+}
+
+std::vector<double> *field_by_name(Ctx &F, const char *name, size_t *count) {
+    static const char *cons[5] = {"rho", "rhou", "rhov", "rhow", "E"};
+    static const char *cons0[5] = {"rho0", "rhou0", "rhov0", "rhow0", "E0"};
+    static const char *prim[5] = {"u", "v", "w", "p", "T"};
+    static const char *rhs[5] = {"rhs_rho", "rhs_rhou", "rhs_rhov", "rhs_rhow", "rhs_E"};
+    static const char *comp[5] = {"mass", "momx", "momy", "momz", "E"};
+    static const char *grad[12] = {"du_dx", "du_dy", "du_dz", "dv_dx", "dv_dy", "dv_dz",
+                                   "dw_dx", "dw_dy", "dw_dz", "dT_dx", "dT_dy", "dT_dz"};
+    std::string s(name);
+    std::vector<double> *r = nullptr;
+    std::vector<double> *c[5] = {&F.rho, &F.rhou, &F.rhov, &F.rhow, &F.E};
+    std::vector<double> *c0[5] = {&F.rho0, &F.rhou0, &F.rhov0, &F.rhow0, &F.E0};
+    std::vector<double> *pr[5] = {&F.u, &F.v, &F.w, &F.p, &F.T};
+    for (int n = 0; n < 5; ++n) {
+        if (s == cons[n]) r = c[n];
+        if (s == cons0[n]) r = c0[n];
+        if (s == prim[n]) r = pr[n];
+        if (s == rhs[n]) r = &F.rhs[n];
+        if (s == std::string("flux_fx_") + comp[n]) r = &F.fx[n];
+        if (s == std::string("flux_fy_") + comp[n]) r = &F.fy[n];
+        if (s == std::string("flux_fz_") + comp[n]) r = &F.fz[n];
+        if (s == std::string("Fvflux_") + comp[n]) r = &F.Fv[n];
+        if (s == std::string("Hvflux_") + comp[n]) r = &F.Hv[n];
+        if (s == std::string("Gvflux_") + comp[n]) r = &F.Gv[n];
+    }
+    for (int n = 0; n < 12; ++n)
+        if (s == grad[n]) r = &F.grad[n];
+    if (r && count) *count = r->size();
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Field3D::allocate (field_structures.h:250-373): cells default to rho = 1,
+// everything else 0.
+void *orc_create(int nx, int ny, int nz, double dx, double dy, double dz, const orc_params *P) {
+    Ctx *F = new Ctx();
+    F->nx = nx; F->ny = ny; F->nz = nz; F->ng = 3;
+    F->sx = nx + 6; F->sy = ny + 6; F->sz = nz + 6;
+    F->dx = dx; F->dy = dy; F->dz = dz;
+    F->P = *P;
+    const size_t tot = (size_t)F->sx * F->sy * F->sz;
+    const size_t fxc = (size_t)(F->sx - 1) * F->sy * F->sz;
+    const size_t fyc = (size_t)F->sx * (F->sy - 1) * F->sz;
+    const size_t fzc = (size_t)F->sx * F->sy * (F->sz - 1);
+    F->rho.assign(tot, 1.0);
+    for (auto *v : {&F->rhou, &F->rhov, &F->rhow, &F->E, &F->rho0, &F->rhou0, &F->rhov0, &F->rhow0, &F->E0,
+                    &F->u, &F->v, &F->w, &F->p, &F->T})
+        v->assign(tot, 0.0);
+    for (int n = 0; n < 5; ++n) {
+        F->rhs[n].assign(tot, 0.0);
+        F->fx[n].assign(fxc, 0.0);
+        F->fy[n].assign(fyc, 0.0);
+        F->fz[n].assign(fzc, 0.0);
+        F->Fv[n].assign(tot, 0.0);
+        F->Hv[n].assign(tot, 0.0);
+        F->Gv[n].assign(tot, 0.0);
+    }
+    for (int n = 0; n < 12; ++n) F->grad[n].assign(tot, 0.0);
+    return F;
+}
+
+void orc_destroy(void *h) { delete static_cast<Ctx *>(h); }
+
+void orc_set_params(void *h, const orc_params *P) { static_cast<Ctx *>(h)->P = *P; }
+
+double *orc_field(void *h, const char *name, size_t *count) {
+    std::vector<double> *v = field_by_name(*static_cast<Ctx *>(h), name, count);
+    return v ? v->data() : nullptr;
+}
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+void orc_conserved_to_primitive(void *h) { conserved_to_primitive(*static_cast<Ctx *>(h)); }
+void orc_primitive_to_conserved(void *h) { primitive_to_conserved(*static_cast<Ctx *>(h)); }
+void orc_apply_boundary(void *h) { apply_boundary(*static_cast<Ctx *>(h)); }
+void orc_record_conserved_to0(void *h) { record_conserved_to0(*static_cast<Ctx *>(h)); }
+void orc_compute_invis_flux(void *h) { compute_invis_flux(*static_cast<Ctx *>(h)); }
+void orc_apply_boundary_halfnode_flux(void *h) { exchange_halfnode(*static_cast<Ctx *>(h)); }
+void orc_compute_invis_dflux(void *h) { compute_invis_dflux(*static_cast<Ctx *>(h)); }
+void orc_compute_gradients(void *h) { compute_gradients(*static_cast<Ctx *>(h)); }
+void orc_compute_viscous_flux(void *h) { compute_viscous_flux(*static_cast<Ctx *>(h)); }
+void orc_exchange_halos_viscous_flux(void *h) { exchange_viscous(*static_cast<Ctx *>(h)); }
+void orc_compute_vis_flux(void *h) { compute_vis_flux(*static_cast<Ctx *>(h)); }
+void orc_compute_rhs(void *h) { compute_rhs(*static_cast<Ctx *>(h)); }
+void orc_runge_kutta_3(void *h, double dt) { runge_kutta_3(*static_cast<Ctx *>(h), dt); }
+double orc_compute_timestep(void *h) { return compute_timestep(*static_cast<Ctx *>(h)); }
+double orc_total_kinetic(void *h) { return total_kinetic(*static_cast<Ctx *>(h)); }
+
+// time_advance (maincycle.cpp:18-56) without monitors or output: record U0,
+// dt from the CFL (or dt_fixed), clip to output_Timefreq / TotalTime, stop on
+// dt <= 1e-12, RK3.  Writes each step's dt; returns the number of steps taken.
+int orc_time_advance(void *h, int max_steps, double dt_fixed, double total_time, double output_timefreq,
+                     double *dt_out) {
+    Ctx &F = *static_cast<Ctx *>(h);
+    double t = 0.0, out_t = output_timefreq;
+    int taken = 0;
+    for (int step = 1; step <= max_steps; ++step) {
+        double dt = dt_fixed;
+        record_conserved_to0(F);
+        const double eps = 1e-12;
+        if (std::abs(total_time - t) <= eps) break;
+        if (dt_fixed < 0.0) dt = compute_timestep(F);
+        if (t >= out_t - eps) out_t += output_timefreq;
+        if (t + dt > total_time) dt = total_time - t;
+        if (t + dt > out_t) { dt = out_t - t; out_t += output_timefreq; }
+        if (dt <= eps) break;
+        t += dt;
+        runge_kutta_3(F, dt);
+        if (dt_out) dt_out[taken] = dt;
+        ++taken;
+        if (t >= total_time) break;
+    }
+    return taken;
+}
+
+// Scalar kernels over batches, for known-answer tests.
+void orc_interpolate_batch(const orc_params *P, const double *st, int n, double flag, double *out) {
+    for (int i = 0; i < n; ++i) out[i] = interpolate_select(st + 6 * i, flag, *P);
+}
+void orc_riemann_batch(const orc_params *P, const double *UL, const double *UR, int n, int dim, double *F) {
+    double nx = dim == 0, ny = dim == 1, nz = dim == 2;
+    for (int i = 0; i < n; ++i) {
+        double f[5] = {0, 0, 0, 0, 0};
+        riemann(P->riemann_solver, f, UL + 5 * i, UR + 5 * i, nx, ny, nz, P->gamma);
+        for (int c = 0; c < 5; ++c) F[5 * i + c] = f[c];
+    }
+}
+void orc_face_flux_batch(const orc_params *P, const double *Ut, const double *ut, int n, int dim, double *F) {
+    for (int i = 0; i < n; ++i) {
+        double a[5][6], b[5][6];
+        for (int v = 0; v < 5; ++v)
+            for (int m = 0; m < 6; ++m) {
+                a[v][m] = Ut[30 * i + 6 * v + m];
+                b[v][m] = ut[30 * i + 6 * v + m];
+            }
+        wcns_face_flux(F + 5 * i, a, b, *P, dim);
+    }
+}
+void orc_eigen(const double *ul, const double *ur, int dim, double gamma, double *L, double *R, double *lam) {
+    double Lm[5][5], Rm[5][5];
+    eigen_matrices(ul, ur, dim == 0, dim == 1, dim == 2, gamma, Lm, Rm, lam);
+    for (int a = 0; a < 5; ++a)
+        for (int b = 0; b < 5; ++b) {
+            L[5 * a + b] = Lm[a][b];
+            R[5 * a + b] = Rm[a][b];
+        }
+}
+double orc_sutherland(const orc_params *P, double T) { return P->mu * std::pow(T, 1.5) * (1.0 + P->S_ref) / (T + P->S_ref); }
+
+}  // extern "C"
