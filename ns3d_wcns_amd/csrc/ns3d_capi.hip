@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,7 @@ struct ns3d_ctx {
     double *cell[NS3D_NUM_CELL_FIELDS];
     double *face[3][5];
     double *visc[NVISC];
+    double *rz[5], *ry[5];  // compact partial RHS of the z and y pencils
     double *grad_slab;
     double *grad[12];
     double *partial;
@@ -213,16 +215,55 @@ int halo(ns3d_ctx *c, bool all) {
     return NS3D_OK;
 }
 
-int rk3(ns3d_ctx *c, const double *d_dt, double dt) {
-    int rc;
-    for (int stage = 1; stage <= 3; ++stage) {
-        if ((rc = run_faces(c))) return rc;
-        launch_visc(visc_args(c, false, true), c->stream);
-        NS3D_LAUNCHED("viscous");
-        launch_assemble(asm_args(c, stage, ASM_INVISCID | ASM_VISCOUS, d_dt, dt), c->stream);
-        NS3D_LAUNCHED("assemble");
-        if ((rc = halo(c, true))) return rc;
+// One fused stage (stage 0: RHS only): viscous fluxes, then the z, y and x
+// pencils; stages 1..3 update the state in place and refill the ghost layers.
+int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false) {
+    launch_visc(visc_args(c, false, true), c->stream);
+    NS3D_LAUNCHED("viscous");
+    PencilArgs a{};
+    for (int n = 0; n < 5; ++n) {
+        a.U[n] = conserved(c)[n];
+        a.RHS[n] = rhs(c)[n];
+        a.U0[n] = recorded(c)[n];
+        a.Unew[n] = conserved(c)[n];  // updated in place by the x pencil
+        a.Q[n] = prims(c)[n];
+        a.Rz[n] = c->rz[n];
+        a.Ry[n] = c->ry[n];
     }
+    for (int n = 0; n < NVISC; ++n) a.V[n] = c->visc[n];
+    a.b = c->b;
+    a.ref = MixedRef{0.0, 0.0};
+    a.record = record ? 1 : 0;
+    a.ihd[0] = 1.0 / c->grid.dx;
+    a.ihd[1] = 1.0 / c->grid.dy;
+    a.ihd[2] = 1.0 / c->grid.dz;
+    a.gamma = c->P.gamma;
+    a.Rgas = c->P.Rgas;
+    a.dt = d_dt;
+    a.dtv = dt;
+    a.stage = stage;
+    const SchemeCoef<double> c64 = coef64(c->P);
+    const SchemeCoef<float> c32 = coef32(c->P);
+    const int interp = c->P.interpolation;
+    const bool chr = c->P.char_recon != 0;
+    switch (c->P.riemann_solver) {
+        case R_ROE: launch_pencils<R_ROE>(interp, chr, c->precision, a, c64, c32, c->stream); break;
+        case R_RUSANOV: launch_pencils<R_RUSANOV>(interp, chr, c->precision, a, c64, c32, c->stream); break;
+        case R_HLLC: launch_pencils<R_HLLC>(interp, chr, c->precision, a, c64, c32, c->stream); break;
+        case R_HLL: launch_pencils<R_HLL>(interp, chr, c->precision, a, c64, c32, c->stream); break;
+        case R_HLLC_P: launch_pencils<R_HLLC_P>(interp, chr, c->precision, a, c64, c32, c->stream); break;
+        default: launch_pencils<R_AUSM>(interp, chr, c->precision, a, c64, c32, c->stream); break;
+    }
+    NS3D_LAUNCHED("pencils");
+    if (stage == 0) return NS3D_OK;
+    return halo(c, true);
+}
+
+// record: stage 1 also stores U0 (interior cells) from the state it reads.
+int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false) {
+    int rc;
+    for (int stage = 1; stage <= 3; ++stage)
+        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1))) return rc;
     return NS3D_OK;
 }
 
@@ -233,11 +274,13 @@ int record0(ns3d_ctx *c) {
     return NS3D_OK;
 }
 
-int cfl(ns3d_ctx *c, double *d_out) {
+int cfl(ns3d_ctx *c, double *d_out, bool record) {
     CflArgs a{};
-    a.rho = c->cell[NS3D_RHO];
-    a.rhou = c->cell[NS3D_RHOU];
-    a.E = c->cell[NS3D_E];
+    for (int n = 0; n < 5; ++n) {
+        a.U[n] = conserved(c)[n];
+        a.U0[n] = recorded(c)[n];
+    }
+    a.record = record ? 1 : 0;
     a.partial = c->partial;
     a.dt_out = d_out;
     a.b = c->b;
@@ -286,15 +329,18 @@ int fetch_compact(ns3d_ctx *c, const double *dptr, std::vector<double> &out) {
     return NS3D_OK;
 }
 
+size_t field_stride(size_t n) { return ((n + 63) / 64) * 64 + 320; }
+
 int ensure_grad(ns3d_ctx *c) {
     if (c->grad_slab) return NS3D_OK;
-    if (hipMalloc(&c->grad_slab, 12 * c->nint * sizeof(double)) != hipSuccess) {
+    const size_t istride = field_stride(c->nint);
+    if (hipMalloc(&c->grad_slab, 12 * istride * sizeof(double)) != hipSuccess) {
         c->grad_slab = nullptr;
         (void)hipGetLastError();
         return fail(NS3D_E_NOMEM, "gradient arrays: out of device memory");
     }
-    for (int q = 0; q < 12; ++q) c->grad[q] = c->grad_slab + q * c->nint;
-    NS3D_HIP(hipMemsetAsync(c->grad_slab, 0, 12 * c->nint * sizeof(double), c->stream));
+    for (int q = 0; q < 12; ++q) c->grad[q] = c->grad_slab + q * istride;
+    NS3D_HIP(hipMemsetAsync(c->grad_slab, 0, 12 * istride * sizeof(double), c->stream));
     return NS3D_OK;
 }
 
@@ -330,17 +376,23 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     c->ncell = (size_t)b.sx * b.sy * b.sz;
     c->nint = (size_t)b.nx * b.ny * b.nz;
     const size_t npart = (size_t)ke_partial_count(b);
-    const size_t total = NS3D_NUM_CELL_FIELDS * c->ncell + (15 + NVISC) * c->nint + npart + 16;
+    // Field strides: rounded to 512 B and staggered by 2.5 KB, so that arrays
+    // whose sizes are powers of two (128 MiB at 256^3) do not all start on
+    // the same HBM channel when a kernel touches the same index of many.
+    const size_t cstride = field_stride(c->ncell), istride = field_stride(c->nint);
+    const size_t total = NS3D_NUM_CELL_FIELDS * cstride + (25 + NVISC) * istride + npart + 16;
     if (hipMalloc(&c->slab, total * sizeof(double)) != hipSuccess) {
         (void)hipGetLastError();
         delete c;
         return fail(NS3D_E_NOMEM, "out of device memory for %zu doubles", total);
     }
     double *p = c->slab;
-    for (int f = 0; f < NS3D_NUM_CELL_FIELDS; ++f, p += c->ncell) c->cell[f] = p;
+    for (int f = 0; f < NS3D_NUM_CELL_FIELDS; ++f, p += cstride) c->cell[f] = p;
     for (int d = 0; d < 3; ++d)
-        for (int n = 0; n < 5; ++n, p += c->nint) c->face[d][n] = p;
-    for (int n = 0; n < NVISC; ++n, p += c->nint) c->visc[n] = p;
+        for (int n = 0; n < 5; ++n, p += istride) c->face[d][n] = p;
+    for (int n = 0; n < NVISC; ++n, p += istride) c->visc[n] = p;
+    for (int n = 0; n < 5; ++n, p += istride) c->rz[n] = p;
+    for (int n = 0; n < 5; ++n, p += istride) c->ry[n] = p;
     c->partial = p;
     p += npart;
     c->scal = p;
@@ -549,14 +601,9 @@ int ns3d_download_viscous_flux(ns3d_ctx *c, int dir, int comp, double *host) {
 
 int ns3d_compute_rhs(ns3d_ctx *c) {
     NS3D_CTX(c);
+    // the RHS of ghost cells is zero, as after the reference's std::fill
     for (int n = 0; n < 5; ++n) NS3D_HIP(hipMemsetAsync(rhs(c)[n], 0, c->ncell * sizeof(double), c->stream));
-    int rc = run_faces(c);
-    if (rc) return rc;
-    launch_visc(visc_args(c, false, true), c->stream);
-    NS3D_LAUNCHED("viscous");
-    launch_assemble(asm_args(c, 0, ASM_INVISCID | ASM_VISCOUS, nullptr, 0.0), c->stream);
-    NS3D_LAUNCHED("assemble");
-    return NS3D_OK;
+    return fused_stage(c, 0, nullptr, 0.0);
 }
 
 int ns3d_runge_kutta_3(ns3d_ctx *c, double dt) {
@@ -573,7 +620,7 @@ int ns3d_runge_kutta_3_dev(ns3d_ctx *c, const double *d_dt) {
 int ns3d_compute_timestep(ns3d_ctx *c, double *dt_out) {
     NS3D_CTX(c);
     if (!dt_out) return fail(NS3D_E_ARG, "null dt_out");
-    int rc = cfl(c, c->scal);
+    int rc = cfl(c, c->scal, false);
     if (rc) return rc;
     NS3D_HIP(hipMemcpyAsync(dt_out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     NS3D_HIP(hipStreamSynchronize(c->stream));
@@ -583,7 +630,7 @@ int ns3d_compute_timestep(ns3d_ctx *c, double *dt_out) {
 int ns3d_compute_timestep_dev(ns3d_ctx *c, double *d_dt) {
     NS3D_CTX(c);
     if (!d_dt) return fail(NS3D_E_ARG, "null device dt");
-    return cfl(c, d_dt);
+    return cfl(c, d_dt, false);
 }
 
 int ns3d_time_advance(ns3d_ctx *c, int max_steps, double dt_fixed, double total_time, double output_timefreq,
@@ -623,9 +670,9 @@ int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
     int rc;
     for (int s = 0; s < nsteps; ++s) {
         double *d_dt = d_dt_hist ? d_dt_hist + s : c->scal;
-        if ((rc = record0(c))) return rc;
-        if ((rc = cfl(c, d_dt))) return rc;
-        if ((rc = rk3(c, d_dt, 0.0))) return rc;
+        // recordConservedTo0 fused into the first stage
+        if ((rc = cfl(c, d_dt, false))) return rc;
+        if ((rc = rk3(c, d_dt, 0.0, true))) return rc;
     }
     return NS3D_OK;
 }

@@ -43,12 +43,80 @@ struct FaceArgs {
     MixedRef ref;
 };
 
+// Compact slots of the nine distinct viscous-flux components.  The reference
+// stores twelve (Fvflux_* = x, Hvflux_* = y, Gvflux_* = z for momx, momy, momz,
+// E; ViscousFlux.cpp:187-206); the off-diagonal stresses appear twice there.
+enum { VXX = 0, VYY, VZZ, VXY, VXZ, VYZ, VEX, VEY, VEZ, NVISC };
+
+// Cells per marching segment of the y / z pencils.
+constexpr int PENCIL_SEG = 64;
+
+// Arguments of the fused pencil kernels of one stage.
+struct PencilArgs {
+    const double *U[5];       // ghosted current state
+    double *RHS[5];           // ghosted RHS (stage 0 output)
+    double *Rz[5], *Ry[5];    // compact partial RHS of the z and y pencils
+    const double *V[NVISC];   // compact viscous fluxes
+    double *U0[5];            // ghosted recorded state (written when record is set)
+    double *Unew[5];          // ghosted state written by the stage update (stage 1..3)
+    double *Q[5];             // ghosted u, v, w, p, T (written at stage 1..3)
+    Box b;
+    MixedRef ref;
+    double ihd[3];            // 1/dx, 1/dy, 1/dz
+    double ih, i60;           // this direction: 1/h, 1/(60 h)
+    double gamma, Rgas;
+    const double *dt;         // device scalar, or nullptr to use dtv
+    double dtv;
+    int stage;                // 0: store the RHS; 1..3: RK3 stage
+    int record;               // stage 1: also store U0 = U (recordConservedTo0) for the interior
+    int seg;                  // y / z segment length
+    int line_threads;         // x pencil: threads per line
+};
+
+template <int RS>
+void launch_pencils(int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,
+                    const SchemeCoef<float> &c32, hipStream_t s);
+
 template <int RS>
 void launch_face(int dir, int interp, bool chr, int precision, const FaceArgs<double> &a64,
                  const FaceArgs<float> &a32, hipStream_t s);
 
-// 256-thread blocks of 64 x 4 interior cells, one z plane per block row.
+// 256-thread blocks of 64 x 4 interior cells of one z plane.
 inline dim3 cell_block() { return dim3(64, 4, 1); }
 inline dim3 cell_grid(const Box &b) { return dim3((b.nx + 63) / 64, (b.ny + 3) / 4, b.nz); }
+
+// XCD-aware tile order.  Workgroup ids are dispatched round-robin over the
+// eight XCDs, each with its own L2, so neighbouring ids land on different L2s.
+// The remap gives XCD x the contiguous tile range [x*per, (x+1)*per) and walks
+// the tiles with axis FAST (1 = y, 2 = z) varying fastest: one XCD then sweeps
+// neighbouring tiles along the stencil axis while their data are in its L2.
+enum { TILE_Y_FAST = 1, TILE_Z_FAST = 2 };
+
+inline long tile_count(const Box &b) { return (long)((b.nx + 63) / 64) * ((b.ny + 3) / 4) * b.nz; }
+inline dim3 tile_grid(const Box &b) { return dim3((unsigned)(((tile_count(b) + 7) / 8) * 8)); }
+
+template <int FAST>
+__device__ __forceinline__ bool tile_cell(const Box &b, int &i, int &j, int &k) {
+    const int ntx = (b.nx + 63) >> 6, nty = (b.ny + 3) >> 2;
+    const long nblk = (long)ntx * nty * b.nz;
+    const long per = (nblk + 7) >> 3;
+    const long t = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= nblk) return false;
+    int tx, ty;
+    if constexpr (FAST == TILE_Z_FAST) {
+        k = (int)(t % b.nz);
+        const long r = t / b.nz;
+        tx = (int)(r % ntx);
+        ty = (int)(r / ntx);
+    } else {
+        ty = (int)(t % nty);
+        const long r = t / nty;
+        k = (int)(r % b.nz);
+        tx = (int)(r / b.nz);
+    }
+    i = tx * 64 + threadIdx.x;
+    j = ty * 4 + threadIdx.y;
+    return true;
+}
 
 }  // namespace ns3d

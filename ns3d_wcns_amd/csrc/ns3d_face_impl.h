@@ -12,9 +12,8 @@ namespace ns3d {
 
 template <int DIR, int INTERP, int RS, bool CHAR, class R>
 __global__ void __launch_bounds__(256) k_face(FaceArgs<R> a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    if (!tile_cell<DIR == 1 ? TILE_Y_FAST : TILE_Z_FAST>(a.b, i, j, k)) return;
     if (i >= a.b.nx || j >= a.b.ny) return;
     // rotated slots: (n, t1, t2) = (u,v,w) / (v,w,u) / (w,u,v)
     constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
@@ -64,7 +63,7 @@ template <int DIR, int INTERP, int RS, bool CHAR>
 inline void launch_face_t(int precision, const FaceArgs<double> &a64, const FaceArgs<float> &a32, hipStream_t s) {
     (void)precision;
     (void)a32;
-    hipLaunchKernelGGL((k_face<DIR, INTERP, RS, CHAR, double>), cell_grid(a64.b), cell_block(), 0, s, a64);
+    hipLaunchKernelGGL((k_face<DIR, INTERP, RS, CHAR, double>), tile_grid(a64.b), cell_block(), 0, s, a64);
 }
 
 template <int RS, int INTERP, bool CHAR>

@@ -6,11 +6,6 @@
 
 namespace ns3d {
 
-// Compact slots of the nine distinct viscous-flux components.  The reference
-// stores twelve (Fvflux_* = x, Hvflux_* = y, Gvflux_* = z for momx, momy, momz,
-// E; ViscousFlux.cpp:187-206); the off-diagonal stresses appear twice there.
-enum { VXX = 0, VYY, VZZ, VXY, VXZ, VYZ, VEX, VEY, VEZ, NVISC };
-
 struct ViscArgs {
     const double *Q[4];  // ghosted u, v, w, T
     double *G[12];       // compact gradients (optional: nullptr for the fused path)
@@ -51,8 +46,13 @@ struct HaloArgs {
     Box b;
 };
 
+// Fixed block count of the grid-stride reductions (partials per reduction).
+constexpr long REDUCE_BLOCKS = 8192;
+
 struct CflArgs {
-    const double *rho, *rhou, *E;
+    const double *U[5];
+    double *U0[5];  // written when record != 0
+    int record;
     double *partial;
     double *dt_out;
     Box b;

@@ -29,9 +29,8 @@ __device__ __forceinline__ double sutherland(double mu, double S, double T) {
 // (compute_gradients + compute_viscous_flux, ViscousFlux.cpp:109-208) in one
 // pass; writes the nine distinct viscous flux components in compact layout.
 __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    if (!tile_cell<TILE_Z_FAST>(a.b, i, j, k)) return;
     if (i >= a.b.nx || j >= a.b.ny) return;
     const long c = a.b.gidx(i, j, k);
     const long sx = a.b.sx, sxy = (long)a.b.sx * a.b.sy;
@@ -76,9 +75,8 @@ __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
 // Viscous fluxes from stored gradients (compute_viscous_flux alone,
 // ViscousFlux.cpp:151-208), for the stepwise call surface.
 __global__ void __launch_bounds__(256) k_visc_from_grad(ViscArgs a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    if (!tile_cell<TILE_Z_FAST>(a.b, i, j, k)) return;
     if (i >= a.b.nx || j >= a.b.ny) return;
     const long c = a.b.gidx(i, j, k);
     const long o = a.b.oidx(i, j, k);
@@ -120,9 +118,8 @@ __global__ void __launch_bounds__(256) k_visc_from_grad(ViscArgs a) {
 // (range_kutta.cpp:53-87) and the conserved -> primitive -> conserved round
 // trip (field_structures.h:444-488) in registers and stores U and Q.
 __global__ void __launch_bounds__(256) k_assemble(AsmArgs a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    if (!tile_cell<TILE_Z_FAST>(a.b, i, j, k)) return;
     if (i >= a.b.nx || j >= a.b.ny) return;
     const Box &b = a.b;
     const long c = b.gidx(i, j, k);
@@ -299,25 +296,38 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// One pass over the rows of the ghosted box: U0 <- U on every cell when
+// a.record is set (Field3D::recordConservedTo0, field_structures.h:491-497),
+// and the block's maximum CFL speed over the interior cells.
 __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const Box &b = a.b;
+    const long rows = (long)b.sy * b.sz;
     double s = 0.0;
-    if (i < a.b.nx && j < a.b.ny) {
-        const long c = a.b.gidx(i, j, k);
-        const double v = cfl_speed(a.rho[c], a.rhou[c], a.E[c], a.gamma);
-        if (v == v) s = v;  // skip NaN
+    for (long r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int j = (int)(r % b.sy), k = (int)(r / b.sy);
+        const bool row_in = j >= NG && j < NG + b.ny && k >= NG && k < NG + b.nz;
+        const long base = r * b.sx;
+        for (int i = threadIdx.x; i < b.sx; i += 256) {
+            const long c = base + i;
+            const double rho = a.U[0][c], rhou = a.U[1][c], E = a.U[4][c];
+            if (a.record) {
+                a.U0[0][c] = rho;
+                a.U0[1][c] = rhou;
+                a.U0[2][c] = a.U[2][c];
+                a.U0[3][c] = a.U[3][c];
+                a.U0[4][c] = E;
+            }
+            if (row_in && i >= NG && i < NG + b.nx) {
+                const double v = cfl_speed(rho, rhou, E, a.gamma);
+                if (v == v) s = fmax(s, v);  // skip NaN
+            }
+        }
     }
     s = wave_max(s);
     __shared__ double part[4];
-    const int t = threadIdx.y * 64 + threadIdx.x;
-    if ((t & 63) == 0) part[t >> 6] = s;
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (t == 0) {
-        const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        a.partial[blk] = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
-    }
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
 }
 
 // Second pass: max of the block maxima, then dt = cfl * min(1e9, dx / smax).
@@ -339,27 +349,26 @@ __global__ void __launch_bounds__(256) k_cfl_final(CflArgs a, long nblk) {
 
 // ---------------------------------------------------------------- kinetic energy
 // Deterministic two-pass sum of 0.5 rho |u|^2 dV over the interior
-// (compute_total_energy, postprocess.cpp:103-128).
+// (compute_total_energy, postprocess.cpp:103-128): fixed block count, fixed
+// row assignment, fixed reduction tree.
 __global__ void __launch_bounds__(256) k_ke_partial(KeArgs a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const Box &b = a.b;
+    const long rows = (long)b.ny * b.nz;
     double s = 0.0;
-    if (i < a.b.nx && j < a.b.ny) {
-        const long c = a.b.gidx(i, j, k);
-        const double u = a.u[c], v = a.v[c], w = a.w[c];
-        s = 0.5 * a.rho[c] * (u * u + v * v + w * w) * a.vol;
+    for (long r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int j = (int)(r % b.ny), k = (int)(r / b.ny);
+        for (int i = threadIdx.x; i < b.nx; i += 256) {
+            const long c = b.gidx(i, j, k);
+            const double u = a.u[c], v = a.v[c], w = a.w[c];
+            s += 0.5 * a.rho[c] * (u * u + v * v + w * w) * a.vol;
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     __shared__ double part[4];
-    const int t = threadIdx.y * 64 + threadIdx.x;
-    if ((t & 63) == 0) part[t >> 6] = s;
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (t == 0) {
-        const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        a.partial[blk] = (part[0] + part[1]) + (part[2] + part[3]);
-    }
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 __global__ void __launch_bounds__(256) k_sum_final(const double *partial, long n, double *out) {
@@ -404,18 +413,16 @@ __global__ void k_riemann_batch(double g, const double *UL, const double *UR, in
 }
 
 // ---------------------------------------------------------------- launchers
-long ke_partial_count(const Box &b) {
-    const dim3 g = cell_grid(b);
-    return (long)g.x * g.y * g.z;
-}
+long ke_partial_count(const Box &) { return REDUCE_BLOCKS; }
+static unsigned reduce_grid(long rows) { return (unsigned)(rows < REDUCE_BLOCKS ? rows : REDUCE_BLOCKS); }
 void launch_visc(const ViscArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_visc, cell_grid(a.b), cell_block(), 0, s, a);
+    hipLaunchKernelGGL(k_visc, tile_grid(a.b), cell_block(), 0, s, a);
 }
 void launch_visc_from_grad(const ViscArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_visc_from_grad, cell_grid(a.b), cell_block(), 0, s, a);
+    hipLaunchKernelGGL(k_visc_from_grad, tile_grid(a.b), cell_block(), 0, s, a);
 }
 void launch_assemble(const AsmArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble, cell_grid(a.b), cell_block(), 0, s, a);
+    hipLaunchKernelGGL(k_assemble, tile_grid(a.b), cell_block(), 0, s, a);
 }
 void launch_c2p(const ConvArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_c2p, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
@@ -431,13 +438,14 @@ void launch_halo(const HaloArgs &h, hipStream_t s) {
     hipLaunchKernelGGL(k_halo, dim3((unsigned)((m + 255) / 256), 3), dim3(256), 0, s, h);
 }
 void launch_cfl(const CflArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_cfl, cell_grid(a.b), cell_block(), 0, s, a);
-    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, ke_partial_count(a.b));
+    const unsigned g = reduce_grid((long)a.b.sy * a.b.sz);
+    hipLaunchKernelGGL(k_cfl, dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, (long)g);
 }
 void launch_ke(const KeArgs &a, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_ke_partial, cell_grid(a.b), cell_block(), 0, s, a);
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, (const double *)a.partial, ke_partial_count(a.b),
-                       out);
+    const unsigned g = reduce_grid((long)a.b.ny * a.b.nz);
+    hipLaunchKernelGGL(k_ke_partial, dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, (const double *)a.partial, (long)g, out);
 }
 void launch_interp_batch(int interp, const SchemeCoef<double> &c, const double *st, int n, double flag, double *out,
                          hipStream_t s) {

@@ -1,0 +1,382 @@
+// ns3d_pencil_impl.h -- fused per-direction pencil kernels of the RHS + RK3
+// stage: each computes the WCNS face fluxes of its direction
+// (WCNS_Riemann_InviscidFlux, InviscousFlux.cpp:146-268), their 6th-order
+// half-node difference (compute_invis_dflux, InviscousFlux.cpp:990-1056) and
+// the viscous-flux divergence of that direction (compute_vis_flux,
+// ViscousFlux.cpp:210-255) without writing face fluxes to memory.
+//
+//   x pencil: one block per (group of) periodic x line(s); the ghosted line is
+//             staged in LDS, one thread per face, exactly N faces per line.
+//   y / z pencils: one thread per (x, other, segment) column marching along
+//             the line with a register window of the six stencil cells and six
+//             running cell sums; lanes run along x, so every load is one
+//             coalesced row.  Segments of L cells compute L + 5 faces.
+//
+// Accumulation: the z and y pencils store their parts (-dH/dz + dGv/dz and
+// -dG/dy + dHv/dy) in compact partial arrays; the x pencil sums them with its
+// own part and then either stores the RHS
+// (stage 0) or applies the Shu-Osher stage update and the C2P/P2C round trip
+// (range_kutta.cpp:53-91, field_structures.h:444-488) in place: its block
+// owns the whole x line, which no other block of that kernel reads.
+#pragma once
+#include "ns3d_common.h"
+
+namespace ns3d {
+
+// Cell-sum coefficient of face f for cell c = f - 2 + q, q = 0..5 (the
+// half-node difference of cell c reads faces c-3..c+2 with weights
+// -3/640, 25/384, -75/64, 75/64, -25/384, 3/640).
+__device__ constexpr double kHalfW[6] = {3.0 / 640.0, -25.0 / 384.0, 75.0 / 64.0, -75.0 / 64.0, 25.0 / 384.0,
+                                         -3.0 / 640.0};
+
+// Viscous-flux slots differentiated along each direction, per momentum/energy
+// component (Fv for x, Hv for y, Gv for z; ViscousFlux.cpp:239-253).
+__device__ constexpr int kViscSlot[3][4] = {{VXX, VXY, VXZ, VEX}, {VXY, VYY, VYZ, VEY}, {VXZ, VYZ, VZZ, VEZ}};
+
+// Primitives (rho, un, u1, u2, p) of a rotated conserved state, for the
+// characteristic projection (the reference projects with the cell's own
+// primitive state, InviscousFlux.cpp:174-176, 204-206).
+template <class R>
+__device__ __forceinline__ void prim_of(const R *U, R g, R *pc) {
+    const R ir = R(1) / U[0];
+    pc[0] = U[0];
+    pc[1] = U[1] * ir;
+    pc[2] = U[2] * ir;
+    pc[3] = U[3] * ir;
+    pc[4] = (g - R(1)) * (U[4] - R(0.5) * U[0] * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3]));
+}
+
+// Shu-Osher stage update of one cell from the complete RHS r, the recorded
+// state u0v and the cell's current state ucur, then the conserved -> primitive -> conserved round
+// trip; writes the new state and the primitives (range_kutta.cpp:53-91,
+// field_structures.h:444-488).
+__device__ __forceinline__ void stage_update(const PencilArgs &a, long g, const double *u0v, const double *ucur,
+                                             const double *r) {
+    const double dt = a.dt ? *a.dt : a.dtv;
+    double un[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        const double u0 = u0v[n];
+        if (a.stage == 1) un[n] = u0 + dt * r[n];
+        else if (a.stage == 2) un[n] = 0.75 * u0 + 0.25 * (ucur[n] + dt * r[n]);
+        else un[n] = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (ucur[n] + dt * r[n]);
+    }
+    double rr = un[0];
+    if (rr <= 0.0) rr = 1e-12;
+    const double ir = 1.0 / rr;
+    const double u = un[1] * ir, v = un[2] * ir, w = un[3] * ir;
+    const double kin = 0.5 * (u * u + v * v + w * w);
+    const double p = (a.gamma - 1.0) * rr * (un[4] * ir - kin);
+    a.Unew[0][g] = rr;
+    a.Unew[1][g] = rr * u;
+    a.Unew[2][g] = rr * v;
+    a.Unew[3][g] = rr * w;
+    a.Unew[4][g] = rr * (p / ((a.gamma - 1.0) * rr) + kin);
+    a.Q[0][g] = u;
+    a.Q[1][g] = v;
+    a.Q[2][g] = w;
+    a.Q[3][g] = p;
+    a.Q[4][g] = p / (rr * a.Rgas);
+}
+
+// ------------------------------------------------------------------ x pencil
+// Last of the three: adds the x part to the partial RHS of the z and y
+// pencils and stores the RHS (stage 0) or updates the line in place (stage
+// 1..3; the block holds its whole line in LDS and no other block reads it).
+// With a.record at stage 1 the current line is also stored as U0.
+// ONE: every thread owns exactly one face and one cell (N == threads per
+// line); its partial RHS and U0 are then loaded into registers before the
+// face work, so that their latency is hidden by it.
+template <int INTERP, int RS, bool CHAR, bool ONE, class R>
+__global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf) {
+    extern __shared__ double lds[];
+    const Box &b = a.b;
+    const int N = b.nx, W = N + 2 * NG;
+    const int P = a.line_threads, Lb = blockDim.x / P;
+    const int ll = threadIdx.x / P, t = threadIdx.x % P;
+    const long line = (long)blockIdx.x * Lb + ll;
+    const bool valid = line < (long)b.ny * b.nz;
+    const int j = valid ? (int)(line % b.ny) : 0, k = valid ? (int)(line / b.ny) : 0;
+    double *Ul = lds + (long)ll * (5 * W + 9 * N);  // U[5][W], F[5][N], V[4][N]
+    double *Fl = Ul + 5 * W;
+    double *Vl = Fl + 5 * N;
+    const long row = ((long)(k + NG) * b.sy + (j + NG)) * b.sx;
+    const long orow = b.oidx(0, j, k);
+    const bool need_u0 = a.stage > 1 || (a.stage == 1 && !a.record);
+    double sp[5], zp[5];
+    if constexpr (ONE) {
+        if (valid) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                sp[n] = a.Rz[n][orow + t] + a.Ry[n][orow + t];
+                zp[n] = need_u0 ? a.U0[n][row + NG + t] : 0.0;
+            }
+        }
+    }
+    if (valid) {
+        if constexpr (ONE) {
+            // all loads of the thread in flight at once, then the LDS stores:
+            // x = t and x = t + P (< W) of the five fields, x = t of the four
+            // viscous fluxes
+            double uv[5][2], vv[4];
+            const bool tail = t + P < W;
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                uv[n][0] = a.U[n][row + t];
+                uv[n][1] = tail ? a.U[n][row + t + P] : 0.0;
+            }
+#pragma unroll
+            for (int n = 0; n < 4; ++n) vv[n] = a.V[kViscSlot[0][n]][orow + t];
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                Ul[n * W + t] = uv[n][0];
+                if (tail) Ul[n * W + t + P] = uv[n][1];
+            }
+#pragma unroll
+            for (int n = 0; n < 4; ++n) Vl[n * N + t] = vv[n];
+        } else {
+            for (int n = 0; n < 5; ++n)
+                for (int x = t; x < W; x += P) Ul[n * W + x] = a.U[n][row + x];
+            for (int n = 0; n < 4; ++n)
+                for (int x = t; x < N; x += P) Vl[n * N + x] = a.V[kViscSlot[0][n]][orow + x];
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        for (int f = t; f < N; f += P) {
+            R s[5][6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m)
+#pragma unroll
+                for (int n = 0; n < 5; ++n) s[n][m] = (R)Ul[n * W + f + 1 + m];
+            R pc[2][5];
+            if constexpr (CHAR) {
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const R u[5] = {s[0][2 + side], s[1][2 + side], s[2][2 + side], s[3][2 + side], s[4][2 + side]};
+                    prim_of(u, cf.gamma, pc[side]);
+                }
+            }
+            R F[5];
+            face_flux<INTERP, RS, CHAR, 1>(cf, s, pc, F);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) Fl[n * N + f] = (double)F[n];
+            if constexpr (ONE) break;
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    for (int c = t; c < N; c += P) {
+        int fi[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) fi[m] = wrap(c + m - 3, N);
+        const long g = row + c + NG;
+        double r[5], u0[5], u[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            u[n] = Ul[n * W + c + NG];
+            double part;
+            if constexpr (ONE) {
+                part = sp[n];
+                u0[n] = need_u0 ? zp[n] : u[n];
+            } else {
+                part = a.Rz[n][orow + c] + a.Ry[n][orow + c];
+                u0[n] = need_u0 ? a.U0[n][g] : u[n];
+            }
+            const double *Fn = Fl + n * N;
+            r[n] = part - ((Fn[fi[3]] - Fn[fi[2]]) * (75.0 / 64.0) + (Fn[fi[4]] - Fn[fi[1]]) * (-25.0 / 384.0) +
+                           (Fn[fi[5]] - Fn[fi[0]]) * (3.0 / 640.0)) *
+                              a.ih;
+        }
+#pragma unroll
+        for (int n = 1; n < 5; ++n) {
+            const double *V = Vl + (n - 1) * N;
+            r[n] += (V[fi[6]] - 9.0 * V[fi[5]] + 45.0 * V[fi[4]] - 45.0 * V[fi[2]] + 9.0 * V[fi[1]] - V[fi[0]]) * a.i60;
+        }
+        if (a.stage == 0) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) a.RHS[n][g] = r[n];
+        } else {
+            if (a.stage == 1 && a.record) {
+#pragma unroll
+                for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
+            }
+            stage_update(a, g, u0, u, r);
+        }
+        if constexpr (ONE) break;
+    }
+}
+
+// ------------------------------------------------------------ y / z pencils
+// Thread per (x, other, segment) column marching along DIR.  Iteration f
+// computes face f (cells f-2..f+3 in the register window) and loads the
+// viscous fluxes of cell f; seven running sums hold cells f-3..f+3, and cell
+// f-3 is complete (faces f-6..f-1, viscous fluxes f-6..f) and is emitted
+// into the pencil's compact partial-RHS array.
+template <int DIR, int INTERP, int RS, bool CHAR, class R>
+__global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf) {
+    const Box &b = a.b;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int other = blockIdx.y * 4 + threadIdx.y;  // z for the y pencil, y for the z pencil
+    const int nother = DIR == 1 ? b.nz : b.ny;
+    if (i >= b.nx || other >= nother) return;
+    constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
+    constexpr int XS = DIR == 1 ? 3 : 2;
+    const int N = DIR == 1 ? b.ny : b.nz;
+    const int c0 = blockIdx.z * a.seg, c1 = min(c0 + a.seg, N);
+    const long sxy = (long)b.sx * b.sy;
+    const long gst = DIR == 1 ? (long)b.sx : sxy;               // ghosted stride along DIR
+    const long ost = DIR == 1 ? (long)b.nx : (long)b.nx * b.ny;  // compact stride along DIR
+    const long gbase = DIR == 1 ? b.gidx(i, 0, other) : b.gidx(i, other, 0);
+    const long obase = DIR == 1 ? b.oidx(i, 0, other) : b.oidx(i, other, 0);
+    const bool mixed = sizeof(R) == 4;
+    auto load = [&](int c, R *u) {
+        const long g = gbase + (long)wrap(c, N) * gst;
+        u[0] = (R)a.U[0][g];
+        u[1] = (R)a.U[P1][g];
+        u[2] = (R)a.U[P2][g];
+        u[3] = (R)a.U[P3][g];
+        u[4] = mixed ? (R)(a.U[4][g] - a.ref.E_ref) : (R)a.U[4][g];
+    };
+    // pre-scaled weights: face f -> cells f-2..f+3 (slots 1..6), viscous flux
+    // of cell f -> cells f-3..f+3 (slots 0..6)
+    double wf[7], wv[7];
+    wf[0] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) wf[q + 1] = -a.ih * kHalfW[q];
+    {
+        const double d[7] = {1.0, -9.0, 45.0, 0.0, -45.0, 9.0, -1.0};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) wv[q] = d[q] * a.i60;
+    }
+    R w[5][6];
+    {
+        R u[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            load(c0 - 5 + m, u);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) w[n][m] = u[n];
+        }
+    }
+    R nxt[5];
+    load(c0, nxt);
+    double acc[7][5];
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) acc[q][n] = 0.0;
+    for (int f = c0 - 3; f <= c1 + 2; ++f) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) w[n][5] = nxt[n];
+        if (f < c1 + 2) load(f + 4, nxt);  // prefetch the next window cell
+        double vv[4];
+        {
+            const long o = obase + (long)wrap(f, N) * ost;
+#pragma unroll
+            for (int n = 0; n < 4; ++n) vv[n] = a.V[kViscSlot[DIR][n]][o];
+        }
+        if (f <= c1 + 1) {  // the last iteration only completes cell c1 - 1
+            R pc[2][5];
+            if constexpr (CHAR) {
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const R u[5] = {w[0][2 + side], w[1][2 + side], w[2][2 + side], w[3][2 + side], w[4][2 + side]};
+                    prim_of(u, cf.gamma, pc[side]);
+                }
+            }
+            R Fr[5];
+            face_flux<INTERP, RS, CHAR, XS>(cf, w, pc, Fr);
+            double F[5];
+            F[0] = (double)Fr[0];
+            F[P1] = (double)Fr[1];
+            F[P2] = (double)Fr[2];
+            F[P3] = (double)Fr[3];
+            F[4] = (double)Fr[4];
+#pragma unroll
+            for (int q = 1; q < 7; ++q)
+#pragma unroll
+                for (int n = 0; n < 5; ++n) acc[q][n] += wf[q] * F[n];
+        }
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+#pragma unroll
+            for (int n = 1; n < 5; ++n) acc[q][n] += wv[q] * vv[n - 1];
+        const int c = f - 3;
+        if (c >= c0) {
+            const long o = obase + (long)c * ost;
+            double *const *dst = DIR == 2 ? a.Rz : a.Ry;
+#pragma unroll
+            for (int n = 0; n < 5; ++n) dst[n][o] = acc[0][n];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) acc[q][n] = acc[q + 1][n];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) acc[6][n] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) w[n][m] = w[n][m + 1];
+    }
+}
+
+// ------------------------------------------------------------------ launch
+template <int INTERP, int RS, bool CHAR, class R>
+inline void launch_pencils_t(const PencilArgs &a, const SchemeCoef<R> &cf, hipStream_t s) {
+    const Box &b = a.b;
+    PencilArgs az = a;
+    az.seg = b.nz < PENCIL_SEG ? b.nz : PENCIL_SEG;
+    az.ih = a.ihd[2];
+    az.i60 = a.ihd[2] / 60.0;
+    hipLaunchKernelGGL((k_pencil_m<2, INTERP, RS, CHAR, R>),
+                       dim3((b.nx + 63) / 64, (b.ny + 3) / 4, (b.nz + az.seg - 1) / az.seg), dim3(64, 4), 0, s, az, cf);
+    PencilArgs ay = a;
+    ay.seg = b.ny < PENCIL_SEG ? b.ny : PENCIL_SEG;
+    ay.ih = a.ihd[1];
+    ay.i60 = a.ihd[1] / 60.0;
+    hipLaunchKernelGGL((k_pencil_m<1, INTERP, RS, CHAR, R>),
+                       dim3((b.nx + 63) / 64, (b.nz + 3) / 4, (b.ny + ay.seg - 1) / ay.seg), dim3(64, 4), 0, s, ay, cf);
+    // x: one thread per face of a line (N rounded up to 64 lanes, up to 512);
+    // lines shorter than 256 share a 256-thread block; longer lines loop
+    PencilArgs ax = a;
+    const int npad = ((b.nx + 63) / 64) * 64;
+    const bool one = npad == b.nx && b.nx <= 512;
+    ax.line_threads = npad <= 512 ? npad : 256;
+    const int bs = ax.line_threads >= 256 ? ax.line_threads : 256;
+    const int lb = bs / ax.line_threads;
+    const long lines = (long)b.ny * b.nz;
+    const size_t shm = (size_t)lb * (5 * (b.nx + 2 * NG) + 9 * b.nx) * sizeof(double);
+    ax.ih = a.ihd[0];
+    ax.i60 = a.ihd[0] / 60.0;
+    const dim3 gx((unsigned)((lines + lb - 1) / lb));
+    if (one) hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, true, R>), gx, dim3(bs), shm, s, ax, cf);
+    else hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, false, R>), gx, dim3(bs), shm, s, ax, cf);
+}
+
+template <int RS, bool CHAR>
+inline void launch_pencils_i(int interp, const PencilArgs &a, const SchemeCoef<double> &c64, hipStream_t s) {
+    switch (interp) {
+        case I_WENO5: launch_pencils_t<I_WENO5, RS, CHAR>(a, c64, s); break;
+        case I_MDCD_LINEAR: launch_pencils_t<I_MDCD_LINEAR, RS, CHAR>(a, c64, s); break;
+        case I_MDCD_HYBRID: launch_pencils_t<I_MDCD_HYBRID, RS, CHAR>(a, c64, s); break;
+        default: launch_pencils_t<I_WENO5_JS, RS, CHAR>(a, c64, s); break;
+    }
+}
+
+template <int RS>
+void launch_pencils(int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,
+                    const SchemeCoef<float> &c32, hipStream_t s) {
+    (void)precision;
+    (void)c32;
+    if (chr) launch_pencils_i<RS, true>(interp, a, c64, s);
+    else launch_pencils_i<RS, false>(interp, a, c64, s);
+}
+
+}  // namespace ns3d
+
+#define NS3D_INSTANTIATE_PENCILS(RS)                                                                            \
+    template void ns3d::launch_pencils<RS>(int, bool, int, const ns3d::PencilArgs &,                   \
+                                           const ns3d::SchemeCoef<double> &, const ns3d::SchemeCoef<float> &,  \
+                                           hipStream_t);

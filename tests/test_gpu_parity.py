@@ -193,3 +193,26 @@ def test_time_advance_clipping(gpu):
     assert rel_err(dts_dev, dts_ref) < 1e-12
     assert abs(np.sum(dts_dev) - 0.05) < 1e-12
     dev.close()
+
+
+def test_advance_steps_matches_time_advance(gpu):
+    """The device-only loop (CFL fused with recordConservedTo0, dt kept on the
+    device) takes the same steps as the host-scheduled time_advance."""
+    import torch
+
+    prim, ma = state(16, 0.3, seed=23)
+    a, _ = pair(16, params(ma), prim)
+    b, _ = pair(16, params(ma), prim)
+    dts = a.time_advance(5)
+    d_hist = torch.zeros(5, dtype=torch.float64, device="cuda")
+    b.advance_steps(5, d_hist.data_ptr())
+    b.synchronize()
+    assert np.array_equal(d_hist.cpu().numpy(), dts)
+    for name in CONS + ("u", "p", "T"):
+        assert np.array_equal(a.download(name), b.download(name)), name
+    # the device loop records U0 for the interior cells only (ghost cells of
+    # U0 are never read)
+    for name in ("rho0", "rhou0", "E0"):
+        assert np.array_equal(a.interior(a.download(name)), b.interior(b.download(name))), name
+    a.close()
+    b.close()
