@@ -218,7 +218,7 @@ int halo(ns3d_ctx *c, bool all) {
 // One fused stage (stage 0: RHS only): viscous fluxes, then the z, y and x
 // pencils; stages 1..3 update the state in place and refill the ghost layers.
 int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false) {
-    launch_visc(visc_args(c, false, true), c->stream);
+    launch_visc_march(visc_args(c, false, true), c->stream);
     NS3D_LAUNCHED("viscous");
     PencilArgs a{};
     for (int n = 0; n < 5; ++n) {

@@ -68,6 +68,7 @@ struct KeArgs {
 
 long ke_partial_count(const Box &b);
 void launch_visc(const ViscArgs &a, hipStream_t s);
+void launch_visc_march(const ViscArgs &a, hipStream_t s);
 void launch_visc_from_grad(const ViscArgs &a, hipStream_t s);
 void launch_assemble(const AsmArgs &a, hipStream_t s);
 void launch_c2p(const ConvArgs &a, hipStream_t s);

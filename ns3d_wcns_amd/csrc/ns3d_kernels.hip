@@ -72,6 +72,113 @@ __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
     a.V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
 }
 
+// The same viscous fluxes from a z-marching 64 x 4 column tile: each thread
+// keeps u, v, w, T of planes k-3..k+3 of its own column in registers (z
+// derivatives); each plane's tile with a 3-cell x/y halo is staged in LDS
+// (x and y derivatives).  Reads the ghosted primitives (ghost layers filled).
+constexpr int VZ_SEG = 16;
+
+__global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
+    constexpr int TX = 64, TY = 4, HX = TX + 6, HY = TY + 6;
+    __shared__ double tile[4][HY][HX];
+    const Box &b = a.b;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own = i < b.nx && j < b.ny;
+    // threads of a partial tile whose column is a ghost column (i < nx + 3,
+    // j < ny + 3) still stage its value: their tile slots are neighbours of
+    // the owned cells; columns beyond the ghost layers are never read
+    const bool inbox = i < b.nx + 3 && j < b.ny + 3;
+    const int k0 = blockIdx.z * VZ_SEG, k1 = min(k0 + VZ_SEG, b.nz);
+    const long sxy = (long)b.sx * b.sy;
+    // ghosted index of (i, j, 0); clamped to cell (0, 0, 0) outside the ghosted box
+    const long col = b.gidx(inbox ? i : 0, inbox ? j : 0, 0);
+    double wq[4][7];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wq[q][m] = a.Q[q][col + (long)(k0 - 3 + m) * sxy];
+    // halo cells of a plane: 6 rows x 64 above/below, then 4 rows x 6 left/right
+    int hi[2], hj[2];
+    bool hv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e = tid + 256 * h;
+        int x, y;
+        if (e < 6 * TX) {
+            const int r = e / TX;
+            y = r < 3 ? r - 3 : r + 1;
+            x = e % TX;
+        } else {
+            const int e2 = e - 6 * TX, r = e2 / 6, c6 = e2 % 6;
+            y = r;
+            x = c6 < 3 ? c6 - 3 : TX + c6 - 3;
+        }
+        hi[h] = x;
+        hj[h] = y;
+        // inside the ghosted box: x in [-3, nx + 3), y in [-3, ny + 3)
+        hv[h] = e < 6 * TX + 6 * TY && i0 + x < b.nx + 3 && j0 + y < b.ny + 3;
+    }
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wq[q][6] = a.Q[q][col + (long)(k + 3) * sxy];
+        double hvv[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const long g = b.gidx(i0 + hi[h], j0 + hj[h], k);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) hvv[h][q] = hv[h] ? a.Q[q][g] : 0.0;
+        }
+        __syncthreads();  // the previous plane's tile has been read
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tile[q][ty + 3][tx + 3] = wq[q][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (hv[h])
+#pragma unroll
+                for (int q = 0; q < 4; ++q) tile[q][hj[h] + 3][hi[h] + 3] = hvv[h][q];
+        __syncthreads();
+        if (own) {
+            double g[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double(*tq)[HX] = tile[q];
+                g[q][0] = d6c(tq[ty + 3][tx], tq[ty + 3][tx + 1], tq[ty + 3][tx + 2], tq[ty + 3][tx + 4],
+                              tq[ty + 3][tx + 5], tq[ty + 3][tx + 6], a.i60[0]);
+                g[q][1] = d6c(tq[ty][tx + 3], tq[ty + 1][tx + 3], tq[ty + 2][tx + 3], tq[ty + 4][tx + 3],
+                              tq[ty + 5][tx + 3], tq[ty + 6][tx + 3], a.i60[1]);
+                g[q][2] = d6c(wq[q][0], wq[q][1], wq[q][2], wq[q][4], wq[q][5], wq[q][6], a.i60[2]);
+            }
+            const double T = wq[3][3];
+            const double mu = sutherland(a.mu, a.S_ref, T);
+            const double divU = g[0][0] + g[1][1] + g[2][2];
+            const double txx = 2.0 * mu * g[0][0] - (2.0 / 3.0) * mu * divU;
+            const double tyy = 2.0 * mu * g[1][1] - (2.0 / 3.0) * mu * divU;
+            const double tzz = 2.0 * mu * g[2][2] - (2.0 / 3.0) * mu * divU;
+            const double txy = mu * (g[0][1] + g[1][0]);
+            const double txz = mu * (g[0][2] + g[2][0]);
+            const double tyz = mu * (g[1][2] + g[2][1]);
+            const double kq = mu * a.cp_pr;
+            const double u = wq[0][3], v = wq[1][3], w = wq[2][3];
+            const long o = b.oidx(i, j, k);
+            a.V[VXX][o] = txx;
+            a.V[VYY][o] = tyy;
+            a.V[VZZ][o] = tzz;
+            a.V[VXY][o] = txy;
+            a.V[VXZ][o] = txz;
+            a.V[VYZ][o] = tyz;
+            a.V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
+            a.V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
+            a.V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wq[q][m] = wq[q][m + 1];
+    }
+}
+
 // Viscous fluxes from stored gradients (compute_viscous_flux alone,
 // ViscousFlux.cpp:151-208), for the stepwise call surface.
 __global__ void __launch_bounds__(256) k_visc_from_grad(ViscArgs a) {
@@ -417,6 +524,10 @@ long ke_partial_count(const Box &) { return REDUCE_BLOCKS; }
 static unsigned reduce_grid(long rows) { return (unsigned)(rows < REDUCE_BLOCKS ? rows : REDUCE_BLOCKS); }
 void launch_visc(const ViscArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_visc, tile_grid(a.b), cell_block(), 0, s, a);
+}
+void launch_visc_march(const ViscArgs &a, hipStream_t s) {
+    const dim3 g((a.b.nx + 63) / 64, (a.b.ny + 3) / 4, (a.b.nz + VZ_SEG - 1) / VZ_SEG);
+    hipLaunchKernelGGL(k_visc_march, g, dim3(64, 4), 0, s, a);
 }
 void launch_visc_from_grad(const ViscArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_visc_from_grad, tile_grid(a.b), cell_block(), 0, s, a);
