@@ -22,6 +22,36 @@ sys.path.insert(0, ROOT)
 METRIC = "Mcell-RK3-steps/s @256^3: FP64 plain vs mixed-FP32 at stated err; adjoint/fwd"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# compulsory FP64 state traffic per cell and RK3 step: stage 1 reads U^n and
+# writes U^(1) (80 B); stages 2 and 3 read U^(s), U^n and write U^(s+1) (120 B each)
+B_ALG_STEP = 320.0
+PROFILE_DIR = os.path.join(ROOT, "profiles")
+
+
+def kernel_flops_per_cell(f, cls):
+    """Algorithmic FP operations per cell of one launch of a kernel class."""
+    if cls == "pencil_x":
+        return f["face"] + f["diff"] / 3 + f["visc_div"] / 3 + f["update"]
+    if cls in ("pencil_y", "pencil_z"):
+        return f["face"] + f["diff"] / 3 + f["visc_div"] / 3
+    if cls == "viscous":
+        return f["visc_flux"]
+    if cls == "cfl":
+        return f["cfl"]
+    return 0.0
+
+
+def profiled_traffic(cls, n):
+    """HBM bytes per launch of a kernel class at n^3 from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, calibrated on a plain
+    copy kernel), or None."""
+    path = os.path.join(PROFILE_DIR, f"{cls}_wcns5_hllc_{n}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def parse():
@@ -66,6 +96,7 @@ def main():
     import torch
 
     from ns3d_wcns_amd import DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
+    from ns3d_wcns_amd._lib import algorithmic_flops
     from ns3d_wcns_amd.isotropic import isotropic_state
 
     torch.cuda.set_device(0)
@@ -83,6 +114,10 @@ def main():
     for s in range(args.warmup):
         dev.advance_steps(1, base + 8 * s)
     torch.cuda.synchronize()
+    # HIP events bracket every kernel launch of the timed steps (on the
+    # library's stream, which is this stream)
+    dev.kernel_times()
+    dev.set_kernel_timing(True)
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record(stream)
@@ -90,11 +125,34 @@ def main():
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    dev.set_kernel_timing(False)
+    ktimes = dev.kernel_times()
     ms = start.elapsed_time(stop) / args.steps
     cells = n ** 3
     value = cells / (ms * 1e-3) / 1e6
     rho = dev.interior(dev.download("rho"))
     finite = bool(np.isfinite(rho).all())
+    flops = algorithmic_flops(P)
+    per_launch = {k: (t / c if c else 0.0) for k, (t, c) in ktimes.items()}
+    dom = max(("pencil_x", "pencil_y", "pencil_z", "viscous"), key=lambda k: ktimes[k][0])
+    fl = kernel_flops_per_cell(flops, dom) * cells
+    achieved = fl / (per_launch[dom] * 1e-3) / 1e12
+    traffic = profiled_traffic(dom, n)
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+        "traffic": traffic,
+        "kernel": dom,
+        "kernel_ms": round(per_launch[dom], 4),
+        "algorithmic_flops_per_launch": fl,
+        "compute": "FP64 vector FMA/add/mul (v_fma_f64); no MFMA instruction is issued; the MI355X FP64 "
+                   "vector and FP64 matrix peaks are both 78.6 TFLOP/s",
+        "hbm_GBps_measured_traffic": round(traffic / (per_launch[dom] * 1e-3) / 1e9, 1) if traffic else None,
+    }
+    step_tflops = flops["per_step"] * cells / (ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -111,6 +169,15 @@ def main():
         "config": {"workload": f"isotropic turbulence {n}^3 periodic, Ma_t={args.mach_t}, Re=100, "
                                f"{args.scheme.upper()}+{args.riemann}, CFL 0.5, plain FP64",
                    "cells": cells, "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "step_roofline": {
+            "algorithmic_flops_per_cell_step": flops["per_step"],
+            "fp64_TFLOPs": round(step_tflops, 3),
+            "fp64_frac": round(step_tflops / FP64_PEAK_TFLOPS, 4),
+            "algorithmic_bytes_per_cell_step": B_ALG_STEP,
+            "hbm_frac_algorithmic": round(B_ALG_STEP * cells / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+        },
+        "kernel_ms_per_launch": {k: round(v, 4) for k, v in per_launch.items()},
         "wall_s": round(wall, 3),
         "state_finite": finite,
     }

@@ -177,6 +177,31 @@ int ns3d_advance_steps(ns3d_ctx *ctx, int nsteps, double *d_dt_hist);
  * from the primitive arrays.  Synchronises. */
 int ns3d_total_kinetic_energy(ns3d_ctx *ctx, double *out);
 
+/* ---- measurement (no counterpart in the reference) ---- */
+/* Algorithmic floating-point operations per cell of the configured scheme:
+ * one face flux (interpolation + Riemann solver, counted exactly on the device
+ * numerics), the half-node differences of three directions, the viscous
+ * fluxes (gradients, Sutherland mu, stresses, heat flux) and their
+ * divergence, the stage update with the C2P/P2C round trip and the CFL
+ * reduction; per_stage = 3 face + diff + visc_flux + visc_div + update and
+ * per_step = 3 per_stage + cfl.  +,-,*,/,sqrt,fabs,fmin,fmax count 1 each. */
+typedef struct ns3d_flops {
+    double face, diff, visc_flux, visc_div, update, cfl, per_stage, per_step;
+} ns3d_flops;
+int ns3d_algorithmic_flops(const ns3d_params *params, ns3d_flops *out);
+
+/* Kernel classes of one fused stage, for ns3d_kernel_times. */
+enum ns3d_kernel_class {
+    NS3D_K_VISCOUS = 0, NS3D_K_PENCIL_Z, NS3D_K_PENCIL_Y, NS3D_K_PENCIL_X, NS3D_K_HALO, NS3D_K_CFL,
+    NS3D_NUM_KERNEL_CLASSES
+};
+/* enable != 0: record HIP events on the context's stream around every launch
+ * of the classes above until disabled; ns3d_kernel_times then synchronises and
+ * returns, per class, the summed event time (ms) and the launch count since
+ * the last call, and releases the events. */
+int ns3d_set_kernel_timing(ns3d_ctx *ctx, int enable);
+int ns3d_kernel_times(ns3d_ctx *ctx, double *ms, int *launches);
+
 /* ---- scalar kernels over batches (device pointers), for known-answer tests ---- */
 /* interpolate_select (interpolation.cpp:13-77): stencils n x 6, flag +1 / -1. */
 int ns3d_interpolate_batch(const ns3d_params *params, const double *d_stencils, int n, double flag,

@@ -28,6 +28,16 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
 
+KERNEL_CLASSES = ("viscous", "pencil_z", "pencil_y", "pencil_x", "halo", "cfl")
+
+
+class CFlops(ctypes.Structure):
+    """``ns3d_flops`` of include/ns3d_hip.h."""
+
+    _fields_ = [(n, ctypes.c_double) for n in ("face", "diff", "visc_flux", "visc_div", "update", "cfl", "per_stage",
+                                                "per_step")]
+
+
 class NS3DError(RuntimeError):
     """A nonzero status from the device library, with its message."""
 
@@ -72,6 +82,8 @@ _SIGS = {
                           ctypes.POINTER(ctypes.c_int)],
     "ns3d_advance_steps": [_VP, ctypes.c_int, _VP],
     "ns3d_total_kinetic_energy": [_VP, _DP],
+    "ns3d_set_kernel_timing": [_VP, ctypes.c_int],
+    "ns3d_kernel_times": [_VP, _DP, ctypes.POINTER(ctypes.c_int)],
     "ns3d_interpolate_batch": [ctypes.POINTER(CParams), _VP, ctypes.c_int, ctypes.c_double, _VP, _VP],
     "ns3d_riemann_batch": [ctypes.POINTER(CParams), _VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP],
 }
@@ -90,6 +102,8 @@ def load_library(path: str | None = None):
         fn = getattr(lib, name)
         fn.argtypes = argtypes
         fn.restype = ctypes.c_int
+    lib.ns3d_algorithmic_flops.restype = ctypes.c_int
+    lib.ns3d_algorithmic_flops.argtypes = [ctypes.POINTER(CParams), ctypes.POINTER(CFlops)]
     lib.ns3d_last_error.restype = ctypes.c_char_p
     lib.ns3d_last_error.argtypes = []
     lib.ns3d_cell_count.restype = ctypes.c_size_t
@@ -108,4 +122,13 @@ def check(status: int):
 
 def exported_symbols() -> list[str]:
     """Names of the entry points declared in include/ns3d_hip.h."""
-    return sorted(list(_SIGS) + ["ns3d_last_error", "ns3d_cell_count", "ns3d_face_count"])
+    return sorted(list(_SIGS) + ["ns3d_last_error", "ns3d_cell_count", "ns3d_face_count", "ns3d_algorithmic_flops"])
+
+
+def algorithmic_flops(params) -> dict:
+    """Algorithmic FP operations per cell of the configured scheme (host-side
+    count on the device numerics; needs no GPU)."""
+    out = CFlops()
+    cp = params.to_c()
+    check(load_library().ns3d_algorithmic_flops(ctypes.byref(cp), ctypes.byref(out)))
+    return {n: getattr(out, n) for n, _ in CFlops._fields_}

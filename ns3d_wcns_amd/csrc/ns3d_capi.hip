@@ -38,11 +38,34 @@ struct ns3d_ctx {
     double *grad[12];
     double *partial;
     double *scal;  // [0] dt, [1] reduction result
+    bool timing;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;  // (class, (start, stop))
 };
 
 namespace {
 
 thread_local std::string g_err;
+
+// Event bracket around one launch of a timed kernel class (no-op unless the
+// context's kernel timing is enabled).
+struct Timed {
+    ns3d_ctx *c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(ns3d_ctx *ctx, int k) : c(ctx), cls(k) {
+        if (!c->timing) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            a = b = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a, c->stream);
+    }
+    ~Timed() {
+        if (!a) return;
+        (void)hipEventRecord(b, c->stream);
+        c->events.push_back({cls, {a, b}});
+    }
+};
 
 int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -210,7 +233,10 @@ int halo(ns3d_ctx *c, bool all) {
         for (int n = 0; n < 4; ++n) h.f[1 + n] = prims(c)[n];
         h.nf = 5;
     }
-    launch_halo(h, c->stream);
+    {
+        Timed t(c, NS3D_K_HALO);
+        launch_halo(h, c->stream);
+    }
     NS3D_LAUNCHED("halo");
     return NS3D_OK;
 }
@@ -218,7 +244,10 @@ int halo(ns3d_ctx *c, bool all) {
 // One fused stage (stage 0: RHS only): viscous fluxes, then the z, y and x
 // pencils; stages 1..3 update the state in place and refill the ghost layers.
 int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false) {
-    launch_visc_march(visc_args(c, false, true), c->stream);
+    {
+        Timed t(c, NS3D_K_VISCOUS);
+        launch_visc_march(visc_args(c, false, true), c->stream);
+    }
     NS3D_LAUNCHED("viscous");
     PencilArgs a{};
     for (int n = 0; n < 5; ++n) {
@@ -246,13 +275,16 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     const SchemeCoef<float> c32 = coef32(c->P);
     const int interp = c->P.interpolation;
     const bool chr = c->P.char_recon != 0;
-    switch (c->P.riemann_solver) {
-        case R_ROE: launch_pencils<R_ROE>(interp, chr, c->precision, a, c64, c32, c->stream); break;
-        case R_RUSANOV: launch_pencils<R_RUSANOV>(interp, chr, c->precision, a, c64, c32, c->stream); break;
-        case R_HLLC: launch_pencils<R_HLLC>(interp, chr, c->precision, a, c64, c32, c->stream); break;
-        case R_HLL: launch_pencils<R_HLL>(interp, chr, c->precision, a, c64, c32, c->stream); break;
-        case R_HLLC_P: launch_pencils<R_HLLC_P>(interp, chr, c->precision, a, c64, c32, c->stream); break;
-        default: launch_pencils<R_AUSM>(interp, chr, c->precision, a, c64, c32, c->stream); break;
+    for (int dir = 2; dir >= 0; --dir) {  // z, y, x
+        Timed t(c, NS3D_K_PENCIL_X - dir);
+        switch (c->P.riemann_solver) {
+            case R_ROE: launch_pencils<R_ROE>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
+            case R_RUSANOV: launch_pencils<R_RUSANOV>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
+            case R_HLLC: launch_pencils<R_HLLC>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
+            case R_HLL: launch_pencils<R_HLL>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
+            case R_HLLC_P: launch_pencils<R_HLLC_P>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
+            default: launch_pencils<R_AUSM>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
+        }
     }
     NS3D_LAUNCHED("pencils");
     if (stage == 0) return NS3D_OK;
@@ -287,6 +319,7 @@ int cfl(ns3d_ctx *c, double *d_out, bool record) {
     a.gamma = c->P.gamma;
     a.dx = c->grid.dx;
     a.cfl = c->P.cfl;
+    Timed t(c, NS3D_K_CFL);
     launch_cfl(a, c->stream);
     NS3D_LAUNCHED("cfl");
     return NS3D_OK;
@@ -397,6 +430,7 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     p += npart;
     c->scal = p;
     c->grad_slab = nullptr;
+    c->timing = false;
     if (hipMemset(c->slab, 0, total * sizeof(double)) != hipSuccess) {
         (void)hipFree(c->slab);
         delete c;
@@ -412,9 +446,40 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     return NS3D_OK;
 }
 
+int ns3d_set_kernel_timing(ns3d_ctx *c, int enable) {
+    NS3D_CTX(c);
+    c->timing = enable != 0;
+    return NS3D_OK;
+}
+
+int ns3d_kernel_times(ns3d_ctx *c, double *ms, int *launches) {
+    NS3D_CTX(c);
+    if (!ms || !launches) return fail(NS3D_E_ARG, "null output");
+    for (int k = 0; k < NS3D_NUM_KERNEL_CLASSES; ++k) {
+        ms[k] = 0.0;
+        launches[k] = 0;
+    }
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    for (auto &e : c->events) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, e.second.first, e.second.second) == hipSuccess) {
+            ms[e.first] += t;
+            launches[e.first] += 1;
+        }
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+    c->events.clear();
+    return NS3D_OK;
+}
+
 int ns3d_destroy(ns3d_ctx *c) {
     if (!c) return NS3D_OK;
     (void)hipStreamSynchronize(c->stream);
+    for (auto &e : c->events) {
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
     if (c->grad_slab) (void)hipFree(c->grad_slab);
     (void)hipFree(c->slab);
     delete c;

@@ -74,7 +74,7 @@ struct PencilArgs {
 };
 
 template <int RS>
-void launch_pencils(int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,
+void launch_pencils(int dir, int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,
                     const SchemeCoef<float> &c32, hipStream_t s);
 
 template <int RS>

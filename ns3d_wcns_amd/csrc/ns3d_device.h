@@ -15,6 +15,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// The numerics are host-callable too, so that the library can count their
+// floating-point operations on a counting scalar type (ns3d_flops.cpp).
+#define NS3D_HD __host__ __device__ __forceinline__
+
 namespace ns3d {
 
 enum { I_WENO5 = 0, I_ZERO = 1, I_MDCD_LINEAR = 2, I_MDCD_HYBRID = 3, I_WENO5_JS = 4 };
@@ -37,7 +41,7 @@ struct SchemeCoef {
 // q_k = (eps + beta_k)^2; the weighted sum is formed over the common
 // denominator q0*q1*q2, one division instead of six.
 template <class R>
-__device__ __forceinline__ R wcns5(R f0, R f1, R f2, R f3, R f4) {
+NS3D_HD R wcns5(R f0, R f1, R f2, R f3, R f4) {
     const R eps = R(1e-6);
     const R t0 = f0 - R(2) * f1 + f2, s0 = f0 - R(4) * f1 + R(3) * f2;
     const R t1 = f1 - R(2) * f2 + f3, s1 = f1 - f3;
@@ -60,7 +64,7 @@ __device__ __forceinline__ R wcns5(R f0, R f1, R f2, R f3, R f4) {
 // Jiang-Shu WENO5 weights 0.1, 0.6, 0.3 (weno5_reconstruction,
 // reconstruction.cpp:69-100), same common-denominator form.
 template <class R>
-__device__ __forceinline__ R weno5js(R f0, R f1, R f2, R f3, R f4) {
+NS3D_HD R weno5js(R f0, R f1, R f2, R f3, R f4) {
     const R eps = R(1e-6);
     const R t0 = f0 - R(2) * f1 + f2, s0 = f0 - R(4) * f1 + R(3) * f2;
     const R t1 = f1 - R(2) * f2 + f3, s1 = f1 - f3;
@@ -81,7 +85,7 @@ __device__ __forceinline__ R weno5js(R f0, R f1, R f2, R f3, R f4) {
 }
 
 template <class R>
-__device__ __forceinline__ R mdcd_linear(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5) {
+NS3D_HD R mdcd_linear(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5) {
     return c.mdcd_lin[0] * f0 + c.mdcd_lin[1] * f1 + c.mdcd_lin[2] * f2 + c.mdcd_lin[3] * f3 +
            c.mdcd_lin[4] * f4 + c.mdcd_lin[5] * f5;
 }
@@ -90,7 +94,7 @@ __device__ __forceinline__ R mdcd_linear(const SchemeCoef<R> &c, R f0, R f1, R f
 // smoothness sensor picks the linear MDCD scheme when sai > 0.4, otherwise a
 // four-stencil WENO-Z-type combination with the 6-point indicator beta3.
 template <class R>
-__device__ __forceinline__ R mdcd_hybrid(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5) {
+NS3D_HD R mdcd_hybrid(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5) {
     const R eps = R(0.9 * 0.4 / (1.0 - 0.9 * 0.4) * 1e-4);
     const R a1 = fabs(f2 - f1) + fabs(f2 - R(2) * f1 + f0);
     const R b1 = fabs(f2 - f3) + fabs(f2 - R(2) * f3 + f4);
@@ -130,7 +134,7 @@ __device__ __forceinline__ R mdcd_hybrid(const SchemeCoef<R> &c, R f0, R f1, R f
 }
 
 template <int INTERP, class R>
-__device__ __forceinline__ R interp6(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5) {
+NS3D_HD R interp6(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5) {
     if constexpr (INTERP == I_WENO5) return wcns5(f0, f1, f2, f3, f4);
     else if constexpr (INTERP == I_WENO5_JS) return weno5js(f0, f1, f2, f3, f4);
     else if constexpr (INTERP == I_MDCD_LINEAR) return mdcd_linear(c, f0, f1, f2, f3, f4, f5);
@@ -141,7 +145,7 @@ __device__ __forceinline__ R interp6(const SchemeCoef<R> &c, R f0, R f1, R f2, R
 // (interpolate_select, interpolation.cpp:13-77: the right state reverses the
 // stencil).
 template <int INTERP, class R>
-__device__ __forceinline__ void interp_lr(const SchemeCoef<R> &c, const R *s, R &left, R &right) {
+NS3D_HD void interp_lr(const SchemeCoef<R> &c, const R *s, R &left, R &right) {
     left = interp6<INTERP>(c, s[0], s[1], s[2], s[3], s[4], s[5]);
     right = interp6<INTERP>(c, s[5], s[4], s[3], s[2], s[1], s[0]);
 }
@@ -154,7 +158,7 @@ struct Prim {
 };
 
 template <class R>
-__device__ __forceinline__ Prim<R> unpack(const R *U, R g) {
+NS3D_HD Prim<R> unpack(const R *U, R g) {
     Prim<R> s;
     s.rho = U[0];
     s.ir = R(1) / s.rho;
@@ -169,7 +173,7 @@ __device__ __forceinline__ Prim<R> unpack(const R *U, R g) {
 
 // Physical x-normal flux of a rotated state.
 template <class R>
-__device__ __forceinline__ void phys_flux(const Prim<R> &s, R *F) {
+NS3D_HD void phys_flux(const Prim<R> &s, R *F) {
     const R m = s.rho * s.un;
     F[0] = m;
     F[1] = m * s.un + s.p;
@@ -180,7 +184,7 @@ __device__ __forceinline__ void phys_flux(const Prim<R> &s, R *F) {
 
 // Rusanov (InviscousFlux.cpp:270-311)
 template <class R>
-__device__ __forceinline__ void rusanov(const R *UL, const R *UR, R g, R *F) {
+NS3D_HD void rusanov(const R *UL, const R *UR, R g, R *F) {
     const Prim<R> L = unpack(UL, g), Rr = unpack(UR, g);
     R FL[5], FR[5];
     phys_flux(L, FL);
@@ -193,7 +197,7 @@ __device__ __forceinline__ void rusanov(const R *UL, const R *UR, R g, R *F) {
 
 // HLL (InviscousFlux.cpp:384-447), Davis wave speeds.
 template <class R>
-__device__ __forceinline__ void hll(const R *UL, const R *UR, R g, R *F) {
+NS3D_HD void hll(const R *UL, const R *UR, R g, R *F) {
     const Prim<R> L = unpack(UL, g), Rr = unpack(UR, g);
     R FL[5], FR[5];
     phys_flux(L, FL);
@@ -218,7 +222,7 @@ __device__ __forceinline__ void hll(const R *UL, const R *UR, R g, R *F) {
 // Tangential velocities are unchanged across the contact; the star energy is
 // E/rho + (SM - un)(SM + p / (rho (S - un))).
 template <class R>
-__device__ __forceinline__ void hllc_star(const Prim<R> &L, const Prim<R> &Rr, const R *UL, const R *UR,
+NS3D_HD void hllc_star(const Prim<R> &L, const Prim<R> &Rr, const R *UL, const R *UR,
                                           const R *FL, const R *FR, R SL, R SR, R SM, R *F) {
     const bool left = SM >= R(0);
     const Prim<R> &s = left ? L : Rr;
@@ -237,7 +241,7 @@ __device__ __forceinline__ void hllc_star(const Prim<R> &L, const Prim<R> &Rr, c
 // pressure-based contact speed with the Roe-averaged normal velocity as the
 // fallback when |den| <= 1e-12.
 template <class R>
-__device__ __forceinline__ void hllc(const R *UL, const R *UR, R g, R *F) {
+NS3D_HD void hllc(const R *UL, const R *UR, R g, R *F) {
     const Prim<R> L = unpack(UL, g), Rr = unpack(UR, g);
     R FL[5], FR[5];
     phys_flux(L, FL);
@@ -269,7 +273,7 @@ __device__ __forceinline__ void hllc(const R *UL, const R *UR, R g, R *F) {
 // HLLC with pressure-based wave speeds (HLLC_p_Riemann_solver,
 // InviscousFlux.cpp:594-717).
 template <class R>
-__device__ __forceinline__ void hllc_p(const R *UL, const R *UR, R g, R *F) {
+NS3D_HD void hllc_p(const R *UL, const R *UR, R g, R *F) {
     const Prim<R> L = unpack(UL, g), Rr = unpack(UR, g);
     R FL[5], FR[5];
     phys_flux(L, FL);
@@ -304,7 +308,7 @@ struct Eig {
 };
 
 template <class R>
-__device__ __forceinline__ void eigen_x(R rho, R u, R v, R w, R c, R g, Eig<R> &e) {
+NS3D_HD void eigen_x(R rho, R u, R v, R w, R c, R g, Eig<R> &e) {
     const R a1 = g - R(1);
     const R phi = R(0.5) * a1 * (u * u + v * v + w * w);
     const R ic = R(1) / c;
@@ -350,7 +354,7 @@ __device__ __forceinline__ void eigen_x(R rho, R u, R v, R w, R c, R g, Eig<R> &
 
 // L * x for the sparse x-normal rows.
 template <class R>
-__device__ __forceinline__ void proj_L(const Eig<R> &e, const R *x, R *y) {
+NS3D_HD void proj_L(const Eig<R> &e, const R *x, R *y) {
     y[0] = e.L[0][0] * x[0] + e.L[0][1] * x[1] + e.L[0][2] * x[2] + e.L[0][3] * x[3] + e.L[0][4] * x[4];
     y[1] = e.L[1][0] * x[0] + e.L[1][1] * x[1] + e.L[1][2] * x[2] + e.L[1][3] * x[3] + e.L[1][4] * x[4];
     y[2] = e.L[2][0] * x[0] + e.L[2][2] * x[2];
@@ -360,7 +364,7 @@ __device__ __forceinline__ void proj_L(const Eig<R> &e, const R *x, R *y) {
 
 // R * q for the sparse x-normal columns.
 template <class R>
-__device__ __forceinline__ void proj_R(const Eig<R> &e, const R *q, R *y) {
+NS3D_HD void proj_R(const Eig<R> &e, const R *q, R *y) {
     y[0] = e.Rm[0][0] * q[0] + q[1] + e.Rm[0][4] * q[4];
     y[1] = e.Rm[1][0] * q[0] + e.Rm[1][1] * q[1] + e.Rm[1][4] * q[4];
     y[2] = e.Rm[2][0] * q[0] + e.Rm[2][1] * q[1] + e.Rm[2][2] * q[2] + e.Rm[2][4] * q[4];
@@ -371,7 +375,7 @@ __device__ __forceinline__ void proj_R(const Eig<R> &e, const R *q, R *y) {
 // Roe flux difference splitting with the entropy fix delta = 0.1 max|lambda|
 // (Roe_Riemann_solver, InviscousFlux.cpp:313-382; Roe average 849-884).
 template <class R>
-__device__ __forceinline__ void roe(const R *UL, const R *UR, R g, R *F) {
+NS3D_HD void roe(const R *UL, const R *UR, R g, R *F) {
     const Prim<R> L = unpack(UL, g), Rr = unpack(UR, g);
     R FL[5], FR[5];
     phys_flux(L, FL);
@@ -412,7 +416,7 @@ __device__ __forceinline__ void roe(const R *UL, const R *UR, R g, R *F) {
 // rotated slot of the x velocity: the reference's pressure-diffusion term uses
 // (u_R - u_L) of the x velocity for every face direction (line 823).
 template <int XS, class R>
-__device__ __forceinline__ void ausm(const R *UL, const R *UR, R g, R *F) {
+NS3D_HD void ausm(const R *UL, const R *UR, R g, R *F) {
     const R Kp = R(0.25), Ku = R(0.75);
     const R rL = UL[0], irL = R(1) / rL;
     const R vL[3] = {UL[1] * irL, UL[2] * irL, UL[3] * irL};
@@ -452,7 +456,7 @@ __device__ __forceinline__ void ausm(const R *UL, const R *UR, R g, R *F) {
 }
 
 template <int RS, int XS, class R>
-__device__ __forceinline__ void riemann(const R *UL, const R *UR, R g, R *F) {
+NS3D_HD void riemann(const R *UL, const R *UR, R g, R *F) {
     if constexpr (RS == R_ROE) roe(UL, UR, g, F);
     else if constexpr (RS == R_RUSANOV) rusanov(UL, UR, g, F);
     else if constexpr (RS == R_HLLC) hllc(UL, UR, g, F);
@@ -466,7 +470,7 @@ __device__ __forceinline__ void riemann(const R *UL, const R *UR, R g, R *F) {
 // primitives (rho, un, u1, u2, p) of cells i and i+1 (WCNS_Riemann_InviscidFlux,
 // InviscousFlux.cpp:146-268).
 template <int INTERP, int RS, bool CHAR, int XS, class R>
-__device__ __forceinline__ void face_flux(const SchemeCoef<R> &c, const R (&st)[5][6], const R (&pc)[2][5],
+NS3D_HD void face_flux(const SchemeCoef<R> &c, const R (&st)[5][6], const R (&pc)[2][5],
                                           R *F) {
     R UL[5], UR[5];
     if constexpr (!CHAR) {

@@ -324,59 +324,65 @@ __global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf
 
 // ------------------------------------------------------------------ launch
 template <int INTERP, int RS, bool CHAR, class R>
-inline void launch_pencils_t(const PencilArgs &a, const SchemeCoef<R> &cf, hipStream_t s) {
+inline void launch_pencils_t(int dir, const PencilArgs &a, const SchemeCoef<R> &cf, hipStream_t s) {
     const Box &b = a.b;
-    PencilArgs az = a;
-    az.seg = b.nz < PENCIL_SEG ? b.nz : PENCIL_SEG;
-    az.ih = a.ihd[2];
-    az.i60 = a.ihd[2] / 60.0;
-    hipLaunchKernelGGL((k_pencil_m<2, INTERP, RS, CHAR, R>),
-                       dim3((b.nx + 63) / 64, (b.ny + 3) / 4, (b.nz + az.seg - 1) / az.seg), dim3(64, 4), 0, s, az, cf);
-    PencilArgs ay = a;
-    ay.seg = b.ny < PENCIL_SEG ? b.ny : PENCIL_SEG;
-    ay.ih = a.ihd[1];
-    ay.i60 = a.ihd[1] / 60.0;
-    hipLaunchKernelGGL((k_pencil_m<1, INTERP, RS, CHAR, R>),
-                       dim3((b.nx + 63) / 64, (b.nz + 3) / 4, (b.ny + ay.seg - 1) / ay.seg), dim3(64, 4), 0, s, ay, cf);
-    // x: one thread per face of a line (N rounded up to 64 lanes, up to 512);
-    // lines shorter than 256 share a 256-thread block; longer lines loop
-    PencilArgs ax = a;
-    const int npad = ((b.nx + 63) / 64) * 64;
-    const bool one = npad == b.nx && b.nx <= 512;
-    ax.line_threads = npad <= 512 ? npad : 256;
-    const int bs = ax.line_threads >= 256 ? ax.line_threads : 256;
-    const int lb = bs / ax.line_threads;
-    const long lines = (long)b.ny * b.nz;
-    const size_t shm = (size_t)lb * (5 * (b.nx + 2 * NG) + 9 * b.nx) * sizeof(double);
-    ax.ih = a.ihd[0];
-    ax.i60 = a.ihd[0] / 60.0;
-    const dim3 gx((unsigned)((lines + lb - 1) / lb));
-    if (one) hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, true, R>), gx, dim3(bs), shm, s, ax, cf);
-    else hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, false, R>), gx, dim3(bs), shm, s, ax, cf);
+    if (dir == 2) {
+        PencilArgs az = a;
+        az.seg = b.nz < PENCIL_SEG ? b.nz : PENCIL_SEG;
+        az.ih = a.ihd[2];
+        az.i60 = a.ihd[2] / 60.0;
+        hipLaunchKernelGGL((k_pencil_m<2, INTERP, RS, CHAR, R>),
+                           dim3((b.nx + 63) / 64, (b.ny + 3) / 4, (b.nz + az.seg - 1) / az.seg), dim3(64, 4), 0, s,
+                           az, cf);
+    } else if (dir == 1) {
+        PencilArgs ay = a;
+        ay.seg = b.ny < PENCIL_SEG ? b.ny : PENCIL_SEG;
+        ay.ih = a.ihd[1];
+        ay.i60 = a.ihd[1] / 60.0;
+        hipLaunchKernelGGL((k_pencil_m<1, INTERP, RS, CHAR, R>),
+                           dim3((b.nx + 63) / 64, (b.nz + 3) / 4, (b.ny + ay.seg - 1) / ay.seg), dim3(64, 4), 0, s,
+                           ay, cf);
+    } else {
+        // one thread per face of a line (N rounded up to 64 lanes, up to
+        // 512); lines shorter than 256 share a 256-thread block; longer lines loop
+        PencilArgs ax = a;
+        const int npad = ((b.nx + 63) / 64) * 64;
+        const bool one = npad == b.nx && b.nx <= 512;
+        ax.line_threads = npad <= 512 ? npad : 256;
+        const int bs = ax.line_threads >= 256 ? ax.line_threads : 256;
+        const int lb = bs / ax.line_threads;
+        const long lines = (long)b.ny * b.nz;
+        const size_t shm = (size_t)lb * (5 * (b.nx + 2 * NG) + 9 * b.nx) * sizeof(double);
+        ax.ih = a.ihd[0];
+        ax.i60 = a.ihd[0] / 60.0;
+        const dim3 gx((unsigned)((lines + lb - 1) / lb));
+        if (one) hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, true, R>), gx, dim3(bs), shm, s, ax, cf);
+        else hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, false, R>), gx, dim3(bs), shm, s, ax, cf);
+    }
 }
 
 template <int RS, bool CHAR>
-inline void launch_pencils_i(int interp, const PencilArgs &a, const SchemeCoef<double> &c64, hipStream_t s) {
+inline void launch_pencils_i(int dir, int interp, const PencilArgs &a, const SchemeCoef<double> &c64, hipStream_t s) {
     switch (interp) {
-        case I_WENO5: launch_pencils_t<I_WENO5, RS, CHAR>(a, c64, s); break;
-        case I_MDCD_LINEAR: launch_pencils_t<I_MDCD_LINEAR, RS, CHAR>(a, c64, s); break;
-        case I_MDCD_HYBRID: launch_pencils_t<I_MDCD_HYBRID, RS, CHAR>(a, c64, s); break;
-        default: launch_pencils_t<I_WENO5_JS, RS, CHAR>(a, c64, s); break;
+        case I_WENO5: launch_pencils_t<I_WENO5, RS, CHAR>(dir, a, c64, s); break;
+        case I_MDCD_LINEAR: launch_pencils_t<I_MDCD_LINEAR, RS, CHAR>(dir, a, c64, s); break;
+        case I_MDCD_HYBRID: launch_pencils_t<I_MDCD_HYBRID, RS, CHAR>(dir, a, c64, s); break;
+        default: launch_pencils_t<I_WENO5_JS, RS, CHAR>(dir, a, c64, s); break;
     }
 }
 
 template <int RS>
-void launch_pencils(int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,
+void launch_pencils(int dir, int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,
                     const SchemeCoef<float> &c32, hipStream_t s) {
     (void)precision;
     (void)c32;
-    if (chr) launch_pencils_i<RS, true>(interp, a, c64, s);
-    else launch_pencils_i<RS, false>(interp, a, c64, s);
+    if (chr) launch_pencils_i<RS, true>(dir, interp, a, c64, s);
+    else launch_pencils_i<RS, false>(dir, interp, a, c64, s);
 }
 
 }  // namespace ns3d
 
 #define NS3D_INSTANTIATE_PENCILS(RS)                                                                            \
-    template void ns3d::launch_pencils<RS>(int, bool, int, const ns3d::PencilArgs &,                   \
+    template void ns3d::launch_pencils<RS>(int, int, bool, int, const ns3d::PencilArgs &,                   \
                                            const ns3d::SchemeCoef<double> &, const ns3d::SchemeCoef<float> &,  \
                                            hipStream_t);

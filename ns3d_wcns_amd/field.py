@@ -181,6 +181,18 @@ class DeviceField:
     def advance_steps(self, nsteps: int, d_dt_hist: int = 0):
         check(self._lib.ns3d_advance_steps(self._h, nsteps, ctypes.c_void_p(d_dt_hist or None)))
 
+    def set_kernel_timing(self, enable: bool):
+        check(self._lib.ns3d_set_kernel_timing(self._h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        """Summed HIP-event time (ms) and launch count per kernel class since
+        the last call (see ns3d_set_kernel_timing)."""
+        n = len(_lib.KERNEL_CLASSES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        check(self._lib.ns3d_kernel_times(self._h, ms, cnt))
+        return {k: (ms[q], cnt[q]) for q, k in enumerate(_lib.KERNEL_CLASSES)}
+
     def total_kinetic_energy(self) -> float:
         out = ctypes.c_double()
         check(self._lib.ns3d_total_kinetic_energy(self._h, ctypes.byref(out)))
