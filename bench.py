@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--scheme", default="wcns5", help="wcns5 | weno5_js | mdcd | mdcd_linear")
     ap.add_argument("--riemann", default="HLLC")
     ap.add_argument("--mach-t", type=float, default=0.6)
+    ap.add_argument("--mixed", type=int, default=1, help="also time the mixed method and report its error")
+    ap.add_argument("--err-steps", type=int, default=20,
+                    help="steps of the mixed-vs-FP64 error comparison from the same state")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU checker on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
@@ -87,6 +90,43 @@ def cpu_baseline(prim, grid, params, threads):
             "sample": f"1 RK3 step (CFL + 3 stages) of the {grid.global_nx}^3 workload, {el:.2f} s wall"}
 
 
+def mixed_report(args, prim, G, P, ms_plain, timed_run):
+    """Throughput of the mixed method on the same workload, and its error
+    against the plain FP64 method after --err-steps steps from the same
+    state (per conserved field, kinetic energy, TKE, dissipation, E(k) per
+    shell)."""
+    from ns3d_wcns_amd import PRECISION_FP64, PRECISION_MIXED, DeviceField
+    from ns3d_wcns_amd.observables import compare_runs
+
+    ms, kt, _, finite = timed_run(PRECISION_MIXED)
+    cells = G.global_nx * G.global_ny * G.global_nz
+    snaps = []
+    for prec in (PRECISION_MIXED, PRECISION_FP64):
+        d = DeviceField(G, P, precision=prec)
+        d.initialize_primitive(*prim)
+        d.advance_steps(args.err_steps)
+        snaps.append(d.snapshot())
+        d.close()
+    rep = compare_runs(snaps[0], snaps[1], P, G.dx * G.dy * G.dz)
+    return {
+        "value": round(cells / (ms * 1e-3) / 1e6, 2),
+        "ms_per_step": round(ms, 4),
+        "speedup_vs_fp64": round(ms_plain / ms, 3),
+        "dtype": "f32 face/viscous arithmetic, f64 state and accumulation",
+        "kernel_ms_per_launch": {k: round(t / c, 4) if c else 0.0 for k, (t, c) in kt.items()},
+        "state_finite": finite,
+        "error_vs_fp64": {
+            "steps": args.err_steps,
+            "fields": {k: float(f"{v:.3e}") for k, v in rep["fields"].items()},
+            "kinetic_energy": float(f"{rep['kinetic_energy']:.3e}"),
+            "tke": float(f"{rep['tke']:.3e}"),
+            "dissipation": float(f"{rep['dissipation']:.3e}"),
+            "spectrum_max_shell": float(f"{rep['spectrum_max']:.3e}"),
+            "spectrum_per_shell": [float(f"{e:.2e}") for e in rep["spectrum_per_shell"]],
+        },
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -95,7 +135,7 @@ def main():
     import numpy as np
     import torch
 
-    from ns3d_wcns_amd import DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
+    from ns3d_wcns_amd import PRECISION_FP64, DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
     from ns3d_wcns_amd._lib import algorithmic_flops
     from ns3d_wcns_amd.isotropic import isotropic_state
 
@@ -106,32 +146,38 @@ def main():
                      riemann_solver=RiemannSolver[args.riemann]).derive()
     G = periodic_box(n)
     stream = torch.cuda.current_stream()
-    dev = DeviceField(G, P)
-    dev.set_stream(stream.cuda_stream)
-    dev.initialize_primitive(*prim)
-    dts = torch.zeros(args.warmup + args.steps, dtype=torch.float64, device="cuda")
-    base = dts.data_ptr()
-    for s in range(args.warmup):
-        dev.advance_steps(1, base + 8 * s)
-    torch.cuda.synchronize()
-    # HIP events bracket every kernel launch of the timed steps (on the
-    # library's stream, which is this stream)
-    dev.kernel_times()
-    dev.set_kernel_timing(True)
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    start.record(stream)
-    dev.advance_steps(args.steps, base + 8 * args.warmup)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    dev.set_kernel_timing(False)
-    ktimes = dev.kernel_times()
-    ms = start.elapsed_time(stop) / args.steps
     cells = n ** 3
+
+    def timed_run(precision):
+        """W untimed + K timed steps of one method from the initial state;
+        returns (ms per step, per-class kernel times, wall s, state finite)."""
+        dev = DeviceField(G, P, precision=precision)
+        dev.set_stream(stream.cuda_stream)
+        dev.initialize_primitive(*prim)
+        dts = torch.zeros(args.warmup + args.steps, dtype=torch.float64, device="cuda")
+        base = dts.data_ptr()
+        for s in range(args.warmup):
+            dev.advance_steps(1, base + 8 * s)
+        torch.cuda.synchronize()
+        # HIP events bracket every kernel launch of the timed steps (on the
+        # library's stream, which is this stream)
+        dev.kernel_times()
+        dev.set_kernel_timing(True)
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(stream)
+        dev.advance_steps(args.steps, base + 8 * args.warmup)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        dev.set_kernel_timing(False)
+        kt = dev.kernel_times()
+        finite = bool(np.isfinite(dev.interior(dev.download("rho"))).all())
+        dev.close()
+        return start.elapsed_time(stop) / args.steps, kt, wall, finite
+
+    ms, ktimes, wall, finite = timed_run(PRECISION_FP64)
     value = cells / (ms * 1e-3) / 1e6
-    rho = dev.interior(dev.download("rho"))
-    finite = bool(np.isfinite(rho).all())
     flops = algorithmic_flops(P)
     per_launch = {k: (t / c if c else 0.0) for k, (t, c) in ktimes.items()}
     dom = max(("pencil_x", "pencil_y", "pencil_z", "viscous"), key=lambda k: ktimes[k][0])
@@ -181,9 +227,10 @@ def main():
         "wall_s": round(wall, 3),
         "state_finite": finite,
     }
+    if args.mixed:
+        out["mixed"] = mixed_report(args, prim, G, P, ms, timed_run)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prim, G, P, args.cpu_threads)
-    dev.close()
     print(json.dumps(out))
 
 

@@ -87,7 +87,15 @@ enum ns3d_field {
 };
 
 /* ---- lifetime ---- */
-/* Field3D::allocate (field_structures.h:250-373): rho = 1, all else 0. */
+/* Field3D::allocate (field_structures.h:250-373): rho = 1, all else 0.
+ * precision NS3D_PRECISION_MIXED selects the mixed method for the fused RHS
+ * and RK3 path (compute_rhs, runge_kutta_3, time_advance, advance_steps):
+ * face interpolation, characteristic projection, Riemann flux and viscous
+ * stencils in FP32 on E - E_ref and p - p_ref (E_ref = Rgas/(gamma-1),
+ * p_ref = Rgas: the nondimensional reference state rho = T = 1), with the
+ * state, the flux-difference sums, the RK3 accumulation, the pressure
+ * recovery and the CFL reduction in FP64.  The stepwise entry points
+ * (compute_invis_flux, compute_gradients, ...) always run FP64. */
 int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params, int precision);
 int ns3d_destroy(ns3d_ctx *ctx);
 /* stream: a hipStream_t passed as void* (NULL = the null stream). */

@@ -39,6 +39,7 @@ struct ns3d_ctx {
     double *partial;
     double *scal;  // [0] dt, [1] reduction result
     bool timing;
+    bool visc_fp32;  // visc[] holds the mixed method's FP32 fluxes (last writer: a mixed fused stage)
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;  // (class, (start, stop))
 };
 
@@ -113,6 +114,9 @@ SchemeCoef<double> coef64(const ns3d_params &P) {
     SchemeCoef<double> c;
     const double diss = P.mdcd_diss, disp = P.mdcd_disp;
     c.gamma = P.gamma;
+    c.p_ref = 0.0;
+    c.H_ref = 0.0;
+    c.E_ref = 0.0;
     // linear MDCD weights, interpolation.cpp:128-132
     c.mdcd_lin[0] = (3.0 * disp + 3.0 * diss) / 8.0;
     c.mdcd_lin[1] = (-18.0 * disp - 30 * diss - 1.0) / 16.0;
@@ -132,9 +136,21 @@ SchemeCoef<float> coef32(const ns3d_params &P) {
     const SchemeCoef<double> d = coef64(P);
     SchemeCoef<float> c;
     c.gamma = (float)d.gamma;
+    // perturbation form: p_ref = rho_ref T_ref Rgas with rho_ref = T_ref = 1
+    // (the nondimensional reference state), E_ref = p_ref / (gamma - 1)
+    const double pr = P.Rgas, er = pr / (P.gamma - 1.0);
+    c.p_ref = (float)pr;
+    c.H_ref = (float)(er + pr);
+    c.E_ref = (float)er;
     for (int q = 0; q < 6; ++q) c.mdcd_lin[q] = (float)d.mdcd_lin[q];
     for (int q = 0; q < 4; ++q) c.mdcd_d[q] = (float)d.mdcd_d[q];
     return c;
+}
+
+// Reference state of the mixed method's perturbation form (zero when plain).
+MixedRef mixed_ref(const ns3d_ctx *c) {
+    if (c->precision != NS3D_PRECISION_MIXED) return MixedRef{0.0, 0.0};
+    return MixedRef{c->P.Rgas / (c->P.gamma - 1.0), c->P.Rgas};
 }
 
 double **conserved(ns3d_ctx *c) { return &c->cell[NS3D_RHO]; }
@@ -246,9 +262,10 @@ int halo(ns3d_ctx *c, bool all) {
 int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false) {
     {
         Timed t(c, NS3D_K_VISCOUS);
-        launch_visc_march(visc_args(c, false, true), c->stream);
+        launch_visc_march(visc_args(c, false, true), c->precision, c->stream);
     }
     NS3D_LAUNCHED("viscous");
+    c->visc_fp32 = c->precision == NS3D_PRECISION_MIXED;
     PencilArgs a{};
     for (int n = 0; n < 5; ++n) {
         a.U[n] = conserved(c)[n];
@@ -261,7 +278,7 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     }
     for (int n = 0; n < NVISC; ++n) a.V[n] = c->visc[n];
     a.b = c->b;
-    a.ref = MixedRef{0.0, 0.0};
+    a.ref = mixed_ref(c);
     a.record = record ? 1 : 0;
     a.ihd[0] = 1.0 / c->grid.dx;
     a.ihd[1] = 1.0 / c->grid.dy;
@@ -396,8 +413,8 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     if (ghosted >= 2147483647.0)
         return fail(NS3D_E_ARG, "box of %g cells exceeds the reference's 32-bit cell index", ghosted);
     if (!(grid->dx > 0 && grid->dy > 0 && grid->dz > 0)) return fail(NS3D_E_ARG, "grid spacing must be positive");
-    if (precision != NS3D_PRECISION_FP64)
-        return fail(NS3D_E_UNSUPPORTED, "precision %d is not available in this build", precision);
+    if (precision != NS3D_PRECISION_FP64 && precision != NS3D_PRECISION_MIXED)
+        return fail(NS3D_E_ARG, "unknown precision %d", precision);
     ns3d_ctx *c = new ns3d_ctx();
     c->grid = *grid;
     c->P = *params;
@@ -431,6 +448,7 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     c->scal = p;
     c->grad_slab = nullptr;
     c->timing = false;
+    c->visc_fp32 = false;
     if (hipMemset(c->slab, 0, total * sizeof(double)) != hipSuccess) {
         (void)hipFree(c->slab);
         delete c;
@@ -628,11 +646,15 @@ int ns3d_compute_viscous_flux(ns3d_ctx *c) {
     if (rc) return rc;
     launch_visc_from_grad(visc_args(c, true, true), c->stream);
     NS3D_LAUNCHED("viscous flux");
+    c->visc_fp32 = false;
     return NS3D_OK;
 }
 
 int ns3d_compute_vis_flux(ns3d_ctx *c) {
     NS3D_CTX(c);
+    if (c->visc_fp32)
+        return fail(NS3D_E_ARG, "the viscous fluxes were last written by a mixed-precision stage: call "
+                                "compute_viscous_flux first");
     launch_assemble(asm_args(c, 0, ASM_VISCOUS | ASM_ACCUMULATE, nullptr, 0.0), c->stream);
     NS3D_LAUNCHED("vis_flux");
     return NS3D_OK;
@@ -656,6 +678,9 @@ int ns3d_download_viscous_flux(ns3d_ctx *c, int dir, int comp, double *host) {
         std::memset(host, 0, c->ncell * sizeof(double));
         return NS3D_OK;
     }
+    if (c->visc_fp32)
+        return fail(NS3D_E_ARG, "the viscous fluxes were last written by a mixed-precision stage: call "
+                                "compute_viscous_flux first");
     static const int slot[3][4] = {{VXX, VXY, VXZ, VEX}, {VXY, VYY, VYZ, VEY}, {VXZ, VYZ, VZZ, VEZ}};
     std::vector<double> f;
     int rc = fetch_compact(c, c->visc[slot[dir][comp - 1]], f);

@@ -111,6 +111,9 @@ extern "C" int ns3d_algorithmic_flops(const ns3d_params *p, ns3d_flops *out) {
         return NS3D_E_ARG;
     ns3d::SchemeCoef<CF> c;
     c.gamma = p->gamma;
+    c.p_ref = 0.0;
+    c.H_ref = 0.0;
+    c.E_ref = 0.0;
     const double diss = p->mdcd_diss, disp = p->mdcd_disp;
     const double lin[6] = {(3.0 * disp + 3.0 * diss) / 8.0, (-18.0 * disp - 30 * diss - 1.0) / 16.0,
                            (12.0 * disp + 60.0 * diss + 9.0) / 16.0, (12.0 * disp - 60.0 * diss + 9.0) / 16.0,

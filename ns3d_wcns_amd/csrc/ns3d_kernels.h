@@ -68,7 +68,9 @@ struct KeArgs {
 
 long ke_partial_count(const Box &b);
 void launch_visc(const ViscArgs &a, hipStream_t s);
-void launch_visc_march(const ViscArgs &a, hipStream_t s);
+// precision 0: FP64 viscous fluxes; 1: FP32 stencils and FP32 fluxes (the
+// mixed method's layout of the same buffers).
+void launch_visc_march(const ViscArgs &a, int precision, hipStream_t s);
 void launch_visc_from_grad(const ViscArgs &a, hipStream_t s);
 void launch_assemble(const AsmArgs &a, hipStream_t s);
 void launch_c2p(const ConvArgs &a, hipStream_t s);

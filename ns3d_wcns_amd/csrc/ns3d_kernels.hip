@@ -7,9 +7,9 @@ namespace ns3d {
 
 // 6th-order central first derivative of a 7-point line without its centre,
 // times 1/(60 h) (diff_6th_central, ViscousFlux.cpp:27-29).
-__device__ __forceinline__ double d6c(double fm3, double fm2, double fm1, double fp1, double fp2, double fp3,
-                                      double i60h) {
-    return (fp3 - 9.0 * fp2 + 45.0 * fp1 - 45.0 * fm1 + 9.0 * fm2 - fm3) * i60h;
+template <class R>
+__device__ __forceinline__ R d6c(R fm3, R fm2, R fm1, R fp1, R fp2, R fp3, R i60h) {
+    return (fp3 - R(9) * fp2 + R(45) * fp1 - R(45) * fm1 + R(9) * fm2 - fm3) * i60h;
 }
 
 // 6th-order midpoint-to-node difference of faces i-5/2..i+5/2 times 1/h
@@ -20,8 +20,9 @@ __device__ __forceinline__ double d6h(double f0, double f1, double f2, double f3
 
 // Sutherland viscosity mu(T) = mu_ref T^1.5 (1 + S) / (T + S)
 // (SolverParams::get_mu, field_structures.h:96-106).
-__device__ __forceinline__ double sutherland(double mu, double S, double T) {
-    return mu * (T * sqrt(T)) * (1.0 + S) / (T + S);
+template <class R>
+__device__ __forceinline__ R sutherland(R mu, R S, R T) {
+    return fdiv(mu * (T * fsqrt(T)) * (R(1) + S), T + S);
 }
 
 // ---------------------------------------------------------------- viscous
@@ -78,9 +79,14 @@ __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
 // (x and y derivatives).  Reads the ghosted primitives (ghost layers filled).
 constexpr int VZ_SEG = 16;
 
+// R = double: the plain method.  R = float: the mixed method's FP32 viscous
+// stencils; the temperature enters as T - T_ref (subtracted in FP64, T_ref =
+// 1, the nondimensional reference temperature) and the nine fluxes are
+// stored as FP32 in the first half of each FP64 buffer.
+template <class R>
 __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
     constexpr int TX = 64, TY = 4, HX = TX + 6, HY = TY + 6;
-    __shared__ double tile[4][HY][HX];
+    __shared__ R tile[4][HY][HX];
     const Box &b = a.b;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
@@ -92,13 +98,17 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
     const bool inbox = i < b.nx + 3 && j < b.ny + 3;
     const int k0 = blockIdx.z * VZ_SEG, k1 = min(k0 + VZ_SEG, b.nz);
     const long sxy = (long)b.sx * b.sy;
+    const double tref = sizeof(R) == 4 ? 1.0 : 0.0;
+    auto ld = [&](int q, long g) -> R { return q == 3 ? (R)(a.Q[3][g] - tref) : (R)a.Q[q][g]; };
+    const R i60[3] = {(R)a.i60[0], (R)a.i60[1], (R)a.i60[2]};
+    const R mu_ref = (R)a.mu, S_ref = (R)a.S_ref, cp_pr = (R)a.cp_pr;
     // ghosted index of (i, j, 0); clamped to cell (0, 0, 0) outside the ghosted box
     const long col = b.gidx(inbox ? i : 0, inbox ? j : 0, 0);
-    double wq[4][7];
+    R wq[4][7];
 #pragma unroll
     for (int m = 0; m < 6; ++m)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) wq[q][m] = a.Q[q][col + (long)(k0 - 3 + m) * sxy];
+        for (int q = 0; q < 4; ++q) wq[q][m] = ld(q, col + (long)(k0 - 3 + m) * sxy);
     // halo cells of a plane: 6 rows x 64 above/below, then 4 rows x 6 left/right
     int hi[2], hj[2];
     bool hv[2];
@@ -122,13 +132,13 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
     }
     for (int k = k0; k < k1; ++k) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) wq[q][6] = a.Q[q][col + (long)(k + 3) * sxy];
-        double hvv[2][4];
+        for (int q = 0; q < 4; ++q) wq[q][6] = ld(q, col + (long)(k + 3) * sxy);
+        R hvv[2][4];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const long g = b.gidx(i0 + hi[h], j0 + hj[h], k);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) hvv[h][q] = hv[h] ? a.Q[q][g] : 0.0;
+            for (int q = 0; q < 4; ++q) hvv[h][q] = hv[h] ? ld(q, g) : R(0);
         }
         __syncthreads();  // the previous plane's tile has been read
 #pragma unroll
@@ -140,37 +150,38 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
                 for (int q = 0; q < 4; ++q) tile[q][hj[h] + 3][hi[h] + 3] = hvv[h][q];
         __syncthreads();
         if (own) {
-            double g[4][3];
+            R g[4][3];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double(*tq)[HX] = tile[q];
+                const R(*tq)[HX] = tile[q];
                 g[q][0] = d6c(tq[ty + 3][tx], tq[ty + 3][tx + 1], tq[ty + 3][tx + 2], tq[ty + 3][tx + 4],
-                              tq[ty + 3][tx + 5], tq[ty + 3][tx + 6], a.i60[0]);
+                              tq[ty + 3][tx + 5], tq[ty + 3][tx + 6], i60[0]);
                 g[q][1] = d6c(tq[ty][tx + 3], tq[ty + 1][tx + 3], tq[ty + 2][tx + 3], tq[ty + 4][tx + 3],
-                              tq[ty + 5][tx + 3], tq[ty + 6][tx + 3], a.i60[1]);
-                g[q][2] = d6c(wq[q][0], wq[q][1], wq[q][2], wq[q][4], wq[q][5], wq[q][6], a.i60[2]);
+                              tq[ty + 5][tx + 3], tq[ty + 6][tx + 3], i60[1]);
+                g[q][2] = d6c(wq[q][0], wq[q][1], wq[q][2], wq[q][4], wq[q][5], wq[q][6], i60[2]);
             }
-            const double T = wq[3][3];
-            const double mu = sutherland(a.mu, a.S_ref, T);
-            const double divU = g[0][0] + g[1][1] + g[2][2];
-            const double txx = 2.0 * mu * g[0][0] - (2.0 / 3.0) * mu * divU;
-            const double tyy = 2.0 * mu * g[1][1] - (2.0 / 3.0) * mu * divU;
-            const double tzz = 2.0 * mu * g[2][2] - (2.0 / 3.0) * mu * divU;
-            const double txy = mu * (g[0][1] + g[1][0]);
-            const double txz = mu * (g[0][2] + g[2][0]);
-            const double tyz = mu * (g[1][2] + g[2][1]);
-            const double kq = mu * a.cp_pr;
-            const double u = wq[0][3], v = wq[1][3], w = wq[2][3];
+            const R T = sizeof(R) == 4 ? wq[3][3] + R(1) : wq[3][3];
+            const R mu = sutherland(mu_ref, S_ref, T);
+            const R divU = g[0][0] + g[1][1] + g[2][2];
+            const R txx = R(2) * mu * g[0][0] - R(2.0 / 3.0) * mu * divU;
+            const R tyy = R(2) * mu * g[1][1] - R(2.0 / 3.0) * mu * divU;
+            const R tzz = R(2) * mu * g[2][2] - R(2.0 / 3.0) * mu * divU;
+            const R txy = mu * (g[0][1] + g[1][0]);
+            const R txz = mu * (g[0][2] + g[2][0]);
+            const R tyz = mu * (g[1][2] + g[2][1]);
+            const R kq = mu * cp_pr;
+            const R u = wq[0][3], v = wq[1][3], w = wq[2][3];
             const long o = b.oidx(i, j, k);
-            a.V[VXX][o] = txx;
-            a.V[VYY][o] = tyy;
-            a.V[VZZ][o] = tzz;
-            a.V[VXY][o] = txy;
-            a.V[VXZ][o] = txz;
-            a.V[VYZ][o] = tyz;
-            a.V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
-            a.V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
-            a.V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
+            R *const *V = reinterpret_cast<R *const *>(a.V);
+            V[VXX][o] = txx;
+            V[VYY][o] = tyy;
+            V[VZZ][o] = tzz;
+            V[VXY][o] = txy;
+            V[VXZ][o] = txz;
+            V[VYZ][o] = tyz;
+            V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
+            V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
+            V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
         }
 #pragma unroll
         for (int m = 0; m < 6; ++m)
@@ -501,7 +512,7 @@ __global__ void k_interp_batch(SchemeCoef<double> c, const double *st, int n, do
 }
 
 template <int RS, int DIR>
-__global__ void k_riemann_batch(double g, const double *UL, const double *UR, int n, double *F) {
+__global__ void k_riemann_batch(SchemeCoef<double> c, const double *UL, const double *UR, int n, double *F) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
@@ -510,7 +521,7 @@ __global__ void k_riemann_batch(double g, const double *UL, const double *UR, in
     const double a[5] = {l[0], l[P1], l[P2], l[P3], l[4]};
     const double b[5] = {r[0], r[P1], r[P2], r[P3], r[4]};
     double f[5];
-    riemann<RS, XS>(a, b, g, f);
+    riemann<RS, XS>(c, a, b, f);
     double *o = F + 5 * (long)t;
     o[0] = f[0];
     o[P1] = f[1];
@@ -525,9 +536,10 @@ static unsigned reduce_grid(long rows) { return (unsigned)(rows < REDUCE_BLOCKS 
 void launch_visc(const ViscArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_visc, tile_grid(a.b), cell_block(), 0, s, a);
 }
-void launch_visc_march(const ViscArgs &a, hipStream_t s) {
+void launch_visc_march(const ViscArgs &a, int precision, hipStream_t s) {
     const dim3 g((a.b.nx + 63) / 64, (a.b.ny + 3) / 4, (a.b.nz + VZ_SEG - 1) / VZ_SEG);
-    hipLaunchKernelGGL(k_visc_march, g, dim3(64, 4), 0, s, a);
+    if (precision == 1) hipLaunchKernelGGL(k_visc_march<float>, g, dim3(64, 4), 0, s, a);
+    else hipLaunchKernelGGL(k_visc_march<double>, g, dim3(64, 4), 0, s, a);
 }
 void launch_visc_from_grad(const ViscArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_visc_from_grad, tile_grid(a.b), cell_block(), 0, s, a);
@@ -572,9 +584,11 @@ void launch_interp_batch(int interp, const SchemeCoef<double> &c, const double *
 template <int RS>
 static void riemann_dir(int dir, double g, const double *UL, const double *UR, int n, double *F, hipStream_t s) {
     const dim3 gr((n + 255) / 256), bl(256);
-    if (dir == 0) hipLaunchKernelGGL((k_riemann_batch<RS, 0>), gr, bl, 0, s, g, UL, UR, n, F);
-    else if (dir == 1) hipLaunchKernelGGL((k_riemann_batch<RS, 1>), gr, bl, 0, s, g, UL, UR, n, F);
-    else hipLaunchKernelGGL((k_riemann_batch<RS, 2>), gr, bl, 0, s, g, UL, UR, n, F);
+    SchemeCoef<double> c{};
+    c.gamma = g;
+    if (dir == 0) hipLaunchKernelGGL((k_riemann_batch<RS, 0>), gr, bl, 0, s, c, UL, UR, n, F);
+    else if (dir == 1) hipLaunchKernelGGL((k_riemann_batch<RS, 1>), gr, bl, 0, s, c, UL, UR, n, F);
+    else hipLaunchKernelGGL((k_riemann_batch<RS, 2>), gr, bl, 0, s, c, UL, UR, n, F);
 }
 
 void launch_riemann_batch(int rs, int dir, double g, const double *UL, const double *UR, int n, double *F,

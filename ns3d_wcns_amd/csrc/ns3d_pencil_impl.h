@@ -38,12 +38,21 @@ __device__ constexpr int kViscSlot[3][4] = {{VXX, VXY, VXZ, VEX}, {VXY, VYY, VYZ
 // primitive state, InviscousFlux.cpp:174-176, 204-206).
 template <class R>
 __device__ __forceinline__ void prim_of(const R *U, R g, R *pc) {
-    const R ir = R(1) / U[0];
+    const R ir = frcp(U[0]);
     pc[0] = U[0];
     pc[1] = U[1] * ir;
     pc[2] = U[2] * ir;
     pc[3] = U[3] * ir;
-    pc[4] = (g - R(1)) * (U[4] - R(0.5) * U[0] * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3]));
+    if constexpr (kPert<R>) pc[4] = pressure_of(U[0], pc[1], pc[2], pc[3], U[4], g);  // p' in FP64
+    else pc[4] = (g - R(1)) * (U[4] - R(0.5) * U[0] * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3]));
+}
+
+// Viscous flux component of compact cell o: FP64 for the plain method, FP32
+// (the mixed method's viscous kernel output, same buffer) otherwise.
+template <class R>
+__device__ __forceinline__ double visc_at(const double *V, long o) {
+    if constexpr (kPert<R>) return (double)reinterpret_cast<const float *>(V)[o];
+    else return V[o];
 }
 
 // Shu-Osher stage update of one cell from the complete RHS r, the recorded
@@ -126,7 +135,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
                 uv[n][1] = tail ? a.U[n][row + t + P] : 0.0;
             }
 #pragma unroll
-            for (int n = 0; n < 4; ++n) vv[n] = a.V[kViscSlot[0][n]][orow + t];
+            for (int n = 0; n < 4; ++n) vv[n] = visc_at<R>(a.V[kViscSlot[0][n]], orow + t);
 #pragma unroll
             for (int n = 0; n < 5; ++n) {
                 Ul[n * W + t] = uv[n][0];
@@ -138,7 +147,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
             for (int n = 0; n < 5; ++n)
                 for (int x = t; x < W; x += P) Ul[n * W + x] = a.U[n][row + x];
             for (int n = 0; n < 4; ++n)
-                for (int x = t; x < N; x += P) Vl[n * N + x] = a.V[kViscSlot[0][n]][orow + x];
+                for (int x = t; x < N; x += P) Vl[n * N + x] = visc_at<R>(a.V[kViscSlot[0][n]], orow + x);
         }
     }
     __syncthreads();
@@ -148,7 +157,11 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
 #pragma unroll
             for (int m = 0; m < 6; ++m)
 #pragma unroll
-                for (int n = 0; n < 5; ++n) s[n][m] = (R)Ul[n * W + f + 1 + m];
+                for (int n = 0; n < 4; ++n) s[n][m] = (R)Ul[n * W + f + 1 + m];
+            // E' = E - E_ref in FP64 before the FP32 cast (E_ref = 0 when plain)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) s[4][m] = kPert<R> ? (R)(Ul[4 * W + f + 1 + m] - a.ref.E_ref)
+                                                         : (R)Ul[4 * W + f + 1 + m];
             R pc[2][5];
             if constexpr (CHAR) {
 #pragma unroll
@@ -229,7 +242,7 @@ __global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf
     const long ost = DIR == 1 ? (long)b.nx : (long)b.nx * b.ny;  // compact stride along DIR
     const long gbase = DIR == 1 ? b.gidx(i, 0, other) : b.gidx(i, other, 0);
     const long obase = DIR == 1 ? b.oidx(i, 0, other) : b.oidx(i, other, 0);
-    const bool mixed = sizeof(R) == 4;
+    constexpr bool mixed = kPert<R>;
     auto load = [&](int c, R *u) {
         const long g = gbase + (long)wrap(c, N) * gst;
         u[0] = (R)a.U[0][g];
@@ -274,7 +287,7 @@ __global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf
         {
             const long o = obase + (long)wrap(f, N) * ost;
 #pragma unroll
-            for (int n = 0; n < 4; ++n) vv[n] = a.V[kViscSlot[DIR][n]][o];
+            for (int n = 0; n < 4; ++n) vv[n] = visc_at<R>(a.V[kViscSlot[DIR][n]], o);
         }
         if (f <= c1 + 1) {  // the last iteration only completes cell c1 - 1
             R pc[2][5];
@@ -361,23 +374,28 @@ inline void launch_pencils_t(int dir, const PencilArgs &a, const SchemeCoef<R> &
     }
 }
 
-template <int RS, bool CHAR>
-inline void launch_pencils_i(int dir, int interp, const PencilArgs &a, const SchemeCoef<double> &c64, hipStream_t s) {
+template <int RS, bool CHAR, class R>
+inline void launch_pencils_i(int dir, int interp, const PencilArgs &a, const SchemeCoef<R> &c, hipStream_t s) {
     switch (interp) {
-        case I_WENO5: launch_pencils_t<I_WENO5, RS, CHAR>(dir, a, c64, s); break;
-        case I_MDCD_LINEAR: launch_pencils_t<I_MDCD_LINEAR, RS, CHAR>(dir, a, c64, s); break;
-        case I_MDCD_HYBRID: launch_pencils_t<I_MDCD_HYBRID, RS, CHAR>(dir, a, c64, s); break;
-        default: launch_pencils_t<I_WENO5_JS, RS, CHAR>(dir, a, c64, s); break;
+        case I_WENO5: launch_pencils_t<I_WENO5, RS, CHAR>(dir, a, c, s); break;
+        case I_MDCD_LINEAR: launch_pencils_t<I_MDCD_LINEAR, RS, CHAR>(dir, a, c, s); break;
+        case I_MDCD_HYBRID: launch_pencils_t<I_MDCD_HYBRID, RS, CHAR>(dir, a, c, s); break;
+        default: launch_pencils_t<I_WENO5_JS, RS, CHAR>(dir, a, c, s); break;
     }
 }
 
+// precision 0: FP64 arithmetic; 1: the mixed method (FP32 face arithmetic in
+// the perturbation form, a.ref set, FP32 viscous fluxes in a.V).
 template <int RS>
 void launch_pencils(int dir, int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,
                     const SchemeCoef<float> &c32, hipStream_t s) {
-    (void)precision;
-    (void)c32;
-    if (chr) launch_pencils_i<RS, true>(dir, interp, a, c64, s);
-    else launch_pencils_i<RS, false>(dir, interp, a, c64, s);
+    if (precision == 1) {
+        if (chr) launch_pencils_i<RS, true>(dir, interp, a, c32, s);
+        else launch_pencils_i<RS, false>(dir, interp, a, c32, s);
+    } else {
+        if (chr) launch_pencils_i<RS, true>(dir, interp, a, c64, s);
+        else launch_pencils_i<RS, false>(dir, interp, a, c64, s);
+    }
 }
 
 }  // namespace ns3d

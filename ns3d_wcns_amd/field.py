@@ -198,6 +198,10 @@ class DeviceField:
         check(self._lib.ns3d_total_kinetic_energy(self._h, ctypes.byref(out)))
         return out.value
 
+    def snapshot(self) -> dict:
+        """Interior copies of the conserved fields and the primitives."""
+        return {name: self.interior(self.download(name)).copy() for name in CONSERVED + PRIMITIVE}
+
     # ---------------------------------------------------------------- set-up
     def initialize_primitive(self, rho, u, v, w, p):
         """Start from interior primitives (nz, ny, nx): fill the ghost layers

@@ -51,3 +51,33 @@ def turbulence_statistics(rho, u, v, w, T, params):
     k = np.arange(ek.size, dtype=np.float64)
     eps = float(np.sum(2.0 * nu * k * k * ek))
     return {"kinetic_energy": ke, "tke": tke, "u_rms": urms, "nu": nu, "dissipation": eps, "spectrum": ek}
+
+
+def compare_runs(test: dict, ref: dict, params, dV: float) -> dict:
+    """Errors of one run against another started from the same state (e.g.
+    the mixed method against the plain FP64 one).  test/ref: interior fields
+    by name (rho, rhou, rhov, rhow, E, u, v, w, p, T).
+
+    - fields: norm-wise relative error max|a - b| / max|b| per conserved field;
+    - kinetic energy (sum 0.5 rho |u|^2 dV), TKE and dissipation: relative error;
+    - spectrum: |E_a(k) - E_b(k)| / E_b(k) per shell k = 1..N/2, and the largest
+      over shells holding at least 1e-6 of the peak shell energy.
+    """
+    out = {"fields": {}}
+    for name in ("rho", "rhou", "rhov", "rhow", "E"):
+        b = ref[name]
+        out["fields"][name] = float(np.max(np.abs(test[name] - b)) / np.max(np.abs(b)))
+    ka = total_kinetic_energy(test["rho"], test["u"], test["v"], test["w"], dV)
+    kb = total_kinetic_energy(ref["rho"], ref["u"], ref["v"], ref["w"], dV)
+    out["kinetic_energy"] = abs(ka - kb) / abs(kb)
+    sa = turbulence_statistics(test["rho"], test["u"], test["v"], test["w"], test["T"], params)
+    sb = turbulence_statistics(ref["rho"], ref["u"], ref["v"], ref["w"], ref["T"], params)
+    out["tke"] = abs(sa["tke"] - sb["tke"]) / abs(sb["tke"])
+    out["dissipation"] = abs(sa["dissipation"] - sb["dissipation"]) / abs(sb["dissipation"])
+    ea, eb = sa["spectrum"], sb["spectrum"]
+    kmax = ref["u"].shape[2] // 2
+    shell = [float(abs(ea[k] - eb[k]) / eb[k]) if eb[k] > 0 else 0.0 for k in range(1, kmax + 1)]
+    out["spectrum_per_shell"] = shell
+    big = [e for k, e in enumerate(shell, start=1) if eb[k] >= 1e-6 * eb[1:].max()]
+    out["spectrum_max"] = max(big) if big else 0.0
+    return out

@@ -1,0 +1,124 @@
+"""The mixed method (FP32 interpolation, projection, Riemann flux and viscous
+stencils; FP64 state, flux-difference sums, RK3 accumulation and pressure
+recovery) against the plain FP64 method and the CPU checker, on the device
+through the C-ABI.
+
+Tolerances:
+- one RHS evaluation: 2e-5 norm-wise relative per field (the FP32 face flux
+  carries ~6e-8 of |F|; the half-node difference divides by h);
+- after N steps from the same state: TKE and E(k) of every shell holding at
+  least 1e-6 of the peak within 1e-5 relative, the conserved fields within
+  1e-5 norm-wise relative.
+"""
+import numpy as np
+import pytest
+
+from cases import ALL_INTERP, ALL_RIEMANN, SMALL, box, params, rel_err, state
+from ns3d_wcns_amd import PRECISION_FP64, PRECISION_MIXED, DeviceField, Interpolation, NS3DError, RiemannSolver
+from ns3d_wcns_amd.observables import compare_runs
+from oracle.ns3d_oracle import OracleField
+
+pytestmark = pytest.mark.gpu
+
+RHS = ("rhs_rho", "rhs_rhou", "rhs_rhov", "rhs_rhow", "rhs_E")
+
+
+def mixed_and_plain(n, P, prim):
+    G = box(n)
+    out = []
+    for prec in (PRECISION_MIXED, PRECISION_FP64):
+        d = DeviceField(G, P, precision=prec)
+        d.initialize_primitive(*prim)
+        out.append(d)
+    return out
+
+
+@pytest.mark.parametrize("interp", ALL_INTERP, ids=lambda x: x.name)
+@pytest.mark.parametrize("rs", ALL_RIEMANN, ids=lambda x: x.name)
+def test_mixed_rhs_close_to_checker(gpu, interp, rs):
+    prim, ma = state(SMALL, 0.3, noise=0.01, seed=21)
+    P = params(ma, interp, rs)
+    G = box(SMALL)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    ref.compute_rhs()
+    dev = DeviceField(G, P, precision=PRECISION_MIXED)
+    dev.initialize_primitive(*prim)
+    dev.compute_rhs()
+    for name in RHS:
+        e = rel_err(dev.interior(dev.download(name)), ref.interior(ref.view(name)))
+        assert e < 2e-5, (name, e)
+    dev.close()
+    ref.close()
+
+
+def test_mixed_rhs_char_projection(gpu):
+    prim, ma = state(SMALL, 0.3, noise=0.01, seed=22)
+    P = params(ma, Interpolation.WENO5, RiemannSolver.HLLC, char=True)
+    G = box(SMALL)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    ref.compute_rhs()
+    dev = DeviceField(G, P, precision=PRECISION_MIXED)
+    dev.initialize_primitive(*prim)
+    dev.compute_rhs()
+    for name in RHS:
+        e = rel_err(dev.interior(dev.download(name)), ref.interior(ref.view(name)))
+        assert e < 2e-5, (name, e)
+    dev.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("mach_t,steps", [(0.3, 40), (0.6, 20)])
+def test_mixed_observables_after_steps(gpu, mach_t, steps):
+    n = 32
+    prim, ma = state(n, mach_t, seed=5)
+    P = params(ma)
+    mixed, plain = mixed_and_plain(n, P, prim)
+    mixed.advance_steps(steps)
+    plain.advance_steps(steps)
+    G = box(n)
+    rep = compare_runs(mixed.snapshot(), plain.snapshot(), P, G.dx * G.dy * G.dz)
+    print(f"Ma_t={mach_t} steps={steps}: {rep}")
+    assert rep["tke"] < 1e-5, rep
+    assert rep["kinetic_energy"] < 1e-5, rep
+    assert rep["spectrum_max"] < 1e-5, rep
+    for name, e in rep["fields"].items():
+        assert e < 1e-5, (name, e)
+    mixed.close()
+    plain.close()
+
+
+def test_mixed_viscous_buffer_guard(gpu):
+    prim, ma = state(SMALL, 0.3, seed=2)
+    P = params(ma)
+    dev = DeviceField(box(SMALL), P, precision=PRECISION_MIXED)
+    dev.initialize_primitive(*prim)
+    dev.compute_rhs()
+    # the fused mixed stage leaves FP32 viscous fluxes: the FP64 readers refuse them
+    with pytest.raises(NS3DError):
+        dev.download_viscous_flux(0, 1)
+    with pytest.raises(NS3DError):
+        dev.compute_vis_flux()
+    # the stepwise FP64 path rewrites them
+    dev.compute_gradients()
+    dev.compute_viscous_flux()
+    dev.download_viscous_flux(0, 1)
+    dev.close()
+
+
+def test_mixed_time_advance_matches_advance_steps(gpu):
+    n = 16
+    prim, ma = state(n, 0.3, seed=8)
+    P = params(ma)
+    G = box(n)
+    a = DeviceField(G, P, precision=PRECISION_MIXED)
+    b = DeviceField(G, P, precision=PRECISION_MIXED)
+    a.initialize_primitive(*prim)
+    b.initialize_primitive(*prim)
+    a.time_advance(5)
+    b.advance_steps(5)
+    for name in ("rho", "rhou", "E"):
+        assert np.array_equal(a.interior(a.download(name)), b.interior(b.download(name))), name
+    a.close()
+    b.close()
