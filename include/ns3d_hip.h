@@ -219,6 +219,52 @@ int ns3d_interpolate_batch(const ns3d_params *params, const double *d_stencils, 
 int ns3d_riemann_batch(const ns3d_params *params, const double *d_UL, const double *d_UR, int n, int dim,
                        double *d_F, void *stream);
 
+/* ---- device-to-device field copies (no counterpart in the reference) ----
+ * For harnesses that keep states in their own device buffers (sx*sy*sz
+ * doubles, the ghosted layout), e.g. checkpoints of the adjoint. */
+int ns3d_copy_field_to_device(ns3d_ctx *ctx, int field, double *d_dst);
+int ns3d_copy_field_from_device(ns3d_ctx *ctx, int field, const double *d_src);
+/* set_state: the conserved state from five device fields (interior read),
+ * then the primitives (conservedToPrimitive) and the periodic ghost layers
+ * of both, as before a step; get_state: the conserved state, ghosts included. */
+int ns3d_set_state(ns3d_ctx *ctx, const double *const *d_U);
+int ns3d_get_state(ns3d_ctx *ctx, double *const *d_U);
+
+/* ---- tangent-linear and adjoint model (no counterpart in the reference) ----
+ * Derivatives of compute_rhs (range_kutta.cpp:11-43) and of one
+ * runge_kutta_3 step with a fixed dt (range_kutta.cpp:46-92; the C2P/P2C
+ * round trip is taken as the identity) at the context's current conserved
+ * state.  Arrays are five device pointers (rho, rho u, rho v, rho w, E) in
+ * the ghosted layout; only interior cells are read, outputs have zero
+ * ghosts.  The non-smooth parts (min/max wave speeds, |.|, branches) use a
+ * fixed subgradient, the same in the tangent and the adjoint.
+ *   tangent_rhs:  out = (dRHS/dU) dU           any scheme combination
+ *   adjoint_rhs:  out = (dRHS/dU)^T lam        component-wise WCNS5 or
+ *                                              WENO5-JS with HLLC or Rusanov
+ *   tangent_step: advances the state one step and dU to its tangent
+ *   adjoint_step: adj = (dU^{n+1}/dU^n)^T adj; the state stays U^n
+ * The adjoint entry points return NS3D_E_UNSUPPORTED for other schemes. */
+int ns3d_tangent_rhs(ns3d_ctx *ctx, const double *const *d_dU, double *const *d_out);
+int ns3d_adjoint_rhs(ns3d_ctx *ctx, const double *const *d_lam, double *const *d_out);
+int ns3d_tangent_step(ns3d_ctx *ctx, double dt, double *const *d_dU);
+int ns3d_adjoint_step(ns3d_ctx *ctx, double dt, double *const *d_adj);
+
+/* ---- host-side derivative checks (no device work) ----
+ * The device derivative numerics on one stencil, evaluated on the host:
+ * face: x-normal 6-cell conserved stencil st[var*6 + m], cells i-2..i+3,
+ * giving F (5) and its tangent dF along dst, or the reverse stb = J^T Fb;
+ * viscous: (u, v, w, T) q, gradients g[3*q + d], the nine flux components
+ * (slots xx, yy, zz, xy, xz, yz, energy x, y, z) and their tangent, or the
+ * reverse (gb, qb); c2p: Ub = (d(u, v, w, T)/dU)^T qb. */
+int ns3d_check_face_tangent(const ns3d_params *params, const double *st, const double *dst, double *F,
+                            double *dF);
+int ns3d_check_face_adjoint(const ns3d_params *params, const double *st, const double *Fb, double *stb);
+int ns3d_check_visc_tangent(const ns3d_params *params, const double *q, const double *g, const double *dq,
+                            const double *dg, double *V, double *dV);
+int ns3d_check_visc_adjoint(const ns3d_params *params, const double *q, const double *g, const double *Vb,
+                            double *gb, double *qb);
+int ns3d_check_c2p_adjoint(const ns3d_params *params, const double *U, const double *qb, double *Ub);
+
 #ifdef __cplusplus
 }
 #endif

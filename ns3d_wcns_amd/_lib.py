@@ -31,6 +31,14 @@ PRECISION_MIXED = 1
 KERNEL_CLASSES = ("viscous", "pencil_z", "pencil_y", "pencil_x", "halo", "cfl")
 
 
+def ptr5(ptrs) -> "ctypes.Array":
+    """Five device addresses (ints) as the C array ``double *[5]``."""
+    ptrs = list(ptrs)
+    if len(ptrs) != 5:
+        raise ValueError("five field pointers expected")
+    return _VP5(*[ctypes.c_void_p(int(p)) for p in ptrs])
+
+
 class CFlops(ctypes.Structure):
     """``ns3d_flops`` of include/ns3d_hip.h."""
 
@@ -50,6 +58,7 @@ _lib = None
 
 _VP = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
+_VP5 = ctypes.c_void_p * 5
 _SIGS = {
     "ns3d_create": [ctypes.POINTER(_VP), ctypes.POINTER(CGrid), ctypes.POINTER(CParams), ctypes.c_int],
     "ns3d_destroy": [_VP],
@@ -84,6 +93,19 @@ _SIGS = {
     "ns3d_total_kinetic_energy": [_VP, _DP],
     "ns3d_set_kernel_timing": [_VP, ctypes.c_int],
     "ns3d_kernel_times": [_VP, _DP, ctypes.POINTER(ctypes.c_int)],
+    "ns3d_copy_field_to_device": [_VP, ctypes.c_int, _VP],
+    "ns3d_copy_field_from_device": [_VP, ctypes.c_int, _VP],
+    "ns3d_set_state": [_VP, _VP5],
+    "ns3d_get_state": [_VP, _VP5],
+    "ns3d_tangent_rhs": [_VP, _VP5, _VP5],
+    "ns3d_adjoint_rhs": [_VP, _VP5, _VP5],
+    "ns3d_tangent_step": [_VP, ctypes.c_double, _VP5],
+    "ns3d_adjoint_step": [_VP, ctypes.c_double, _VP5],
+    "ns3d_check_face_tangent": [ctypes.POINTER(CParams), _DP, _DP, _DP, _DP],
+    "ns3d_check_face_adjoint": [ctypes.POINTER(CParams), _DP, _DP, _DP],
+    "ns3d_check_visc_tangent": [ctypes.POINTER(CParams), _DP, _DP, _DP, _DP, _DP, _DP],
+    "ns3d_check_visc_adjoint": [ctypes.POINTER(CParams), _DP, _DP, _DP, _DP, _DP],
+    "ns3d_check_c2p_adjoint": [ctypes.POINTER(CParams), _DP, _DP, _DP],
     "ns3d_interpolate_batch": [ctypes.POINTER(CParams), _VP, ctypes.c_int, ctypes.c_double, _VP, _VP],
     "ns3d_riemann_batch": [ctypes.POINTER(CParams), _VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP],
 }

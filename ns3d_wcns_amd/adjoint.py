@@ -1,0 +1,196 @@
+"""Discrete adjoint of the RK3 trajectory on one periodic box, behind a
+``torch.autograd.Function``.
+
+``rollout(U0, field, dts, checkpoint=...)`` advances the conserved state
+``U0`` (a float64 CUDA tensor of shape (5, sz, sy, sx), the ghosted layout)
+by ``len(dts)`` RK3 steps with the given fixed time steps and returns the
+final state; ``backward`` runs the hand-written discrete adjoint of each step
+(``DeviceField.adjoint_step``) in reverse order, recomputing the states it
+needs from checkpoints:
+
+- ``"all"``: every state kept;
+- ``"uniform:K"``: every K-th state kept; each segment is recomputed once,
+  its K states kept during its reverse sweep;
+- ``"binomial:S"``: S snapshots placed by the binomial (revolve) schedule,
+  for a fixed memory budget.
+
+The time steps are inputs, not functions of the state: the gradient is that
+of the map U0 -> U_N at the given dt sequence (for a CFL-controlled run,
+record the dt sequence of the forward run and pass it).  Gradients reach
+interior cells only; ghost entries of the input are ignored by the forward
+(periodic ghosts are refilled) and get zero gradient.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import torch
+
+from .field import NG, DeviceField
+
+CONSERVED = ("rho", "rhou", "rhov", "rhow", "E")
+
+
+def _ptrs(t: torch.Tensor):
+    return [t[n].data_ptr() for n in range(5)]
+
+
+def _check_state(U: torch.Tensor, field: DeviceField):
+    if U.dtype != torch.float64 or not U.is_cuda or not U.is_contiguous():
+        raise ValueError("states are contiguous float64 CUDA tensors")
+    if tuple(U.shape) != (5,) + field.shape:
+        raise ValueError(f"state shape {tuple(U.shape)} != {(5,) + field.shape}")
+
+
+def parse_checkpoint(policy: str, nsteps: int):
+    """('all', n) | ('uniform', K) | ('binomial', S) from a policy string."""
+    if policy == "all":
+        return "all", nsteps
+    kind, _, arg = policy.partition(":")
+    if kind in ("uniform", "binomial") and arg.isdigit() and int(arg) >= (1 if kind == "uniform" else 0):
+        return kind, int(arg)
+    raise ValueError(f"checkpoint policy {policy!r}: 'all', 'uniform:K' (K >= 1) or 'binomial:S' (S >= 0)")
+
+
+def binomial_beta(s: int, t: int) -> int:
+    """Largest number of steps reversible with s snapshots and t forward
+    sweeps: C(s + t, s) (Griewank's binomial checkpointing bound)."""
+    return math.comb(s + t, s)
+
+
+class _Sweep:
+    """State handling of one reverse sweep on a DeviceField."""
+
+    def __init__(self, field: DeviceField, dts, stats: dict):
+        self.field = field
+        self.dts = [float(d) for d in dts]
+        self.stats = stats
+
+    def load(self, U: torch.Tensor):
+        self.field.set_state(_ptrs(U))
+
+    def save(self) -> torch.Tensor:
+        U = torch.empty((5,) + self.field.shape, dtype=torch.float64, device="cuda")
+        self.field.get_state(_ptrs(U))
+        return U
+
+    def advance(self, i: int, m: int):
+        """Steps i..i+m-1 from the loaded state."""
+        for j in range(i, i + m):
+            self.field.step(self.dts[j])
+        self.stats["recomputed_steps"] += m
+
+    def adjoint(self, i: int, U: torch.Tensor, adj: torch.Tensor):
+        self.load(U)
+        self.field.adjoint_step(self.dts[i], _ptrs(adj))
+        self.stats["adjoint_steps"] += 1
+
+    def reverse_binomial(self, i: int, n: int, s: int, Ui: torch.Tensor, adj: torch.Tensor):
+        """Reverse steps i..i+n-1 from the state Ui at step i with s free
+        snapshots (recursive binomial schedule)."""
+        if n == 0:
+            return
+        if n == 1:
+            self.adjoint(i, Ui, adj)
+            return
+        if s == 0:
+            for j in range(n - 1, -1, -1):
+                self.load(Ui)
+                self.advance(i, j)
+                Uj = self.save() if j else Ui
+                self.adjoint(i + j, Uj, adj)
+            return
+        t = 1
+        while binomial_beta(s, t) < n:
+            t += 1
+        m = max(n - binomial_beta(s - 1, t), 1)
+        self.load(Ui)
+        self.advance(i, m)
+        Um = self.save()
+        self.stats["peak_snapshots"] = max(self.stats["peak_snapshots"], self.stats["live"] + 1)
+        self.stats["live"] += 1
+        self.reverse_binomial(i + m, n - m, s - 1, Um, adj)
+        del Um
+        self.stats["live"] -= 1
+        self.reverse_binomial(i, m, s, Ui, adj)
+
+
+class RK3Rollout(torch.autograd.Function):
+    """U0 -> U_N over fixed time steps, with the discrete adjoint as backward."""
+
+    @staticmethod
+    def forward(ctx, U0, field: DeviceField, dts, checkpoint: str, stats: dict | None):
+        _check_state(U0, field)
+        n = len(dts)
+        kind, arg = parse_checkpoint(checkpoint, n)
+        field.set_stream(torch.cuda.current_stream().cuda_stream)
+        sweep = _Sweep(field, dts, stats if stats is not None else {})
+        for k in ("recomputed_steps", "adjoint_steps", "peak_snapshots", "live"):
+            sweep.stats.setdefault(k, 0)
+        t0 = time.perf_counter()
+        saved = {0: U0.detach().clone()}
+        sweep.load(saved[0])
+        for i in range(n):
+            field.step(sweep.dts[i])
+            if kind == "all" or (kind == "uniform" and (i + 1) % arg == 0 and i + 1 < n):
+                saved[i + 1] = sweep.save()
+        out = sweep.save()
+        torch.cuda.synchronize()
+        sweep.stats["forward_s"] = time.perf_counter() - t0
+        sweep.stats["stored_states"] = len(saved)
+        ctx.sweep, ctx.kind, ctx.arg, ctx.saved, ctx.n = sweep, kind, arg, saved, n
+        return out
+
+    @staticmethod
+    def backward(ctx, gU):
+        sweep, kind, arg, saved, n = ctx.sweep, ctx.kind, ctx.arg, ctx.saved, ctx.n
+        sweep.field.set_stream(torch.cuda.current_stream().cuda_stream)
+        t0 = time.perf_counter()
+        adj = gU.detach().contiguous().clone()
+        # only interior cells carry gradient
+        ghost = torch.ones_like(adj, dtype=torch.bool)
+        ghost[:, NG:-NG, NG:-NG, NG:-NG] = False
+        adj[ghost] = 0.0
+        if kind == "all":
+            for i in range(n - 1, -1, -1):
+                sweep.adjoint(i, saved[i], adj)
+        elif kind == "uniform":
+            starts = list(range(0, n, arg))
+            for s0 in reversed(starts):
+                s1 = min(s0 + arg, n)
+                states = [saved[s0]]
+                sweep.load(saved[s0])
+                for i in range(s0, s1 - 1):
+                    sweep.advance(i, 1)
+                    states.append(sweep.save())
+                for i in range(s1 - 1, s0 - 1, -1):
+                    sweep.adjoint(i, states[i - s0], adj)
+        else:
+            sweep.reverse_binomial(0, n, arg, saved[0], adj)
+        torch.cuda.synchronize()
+        sweep.stats["backward_s"] = time.perf_counter() - t0
+        return adj, None, None, None, None
+
+
+def rollout(U0: torch.Tensor, field: DeviceField, dts, checkpoint: str = "uniform:10", stats: dict | None = None):
+    """Differentiable N-step RK3 rollout (see the module docstring)."""
+    return RK3Rollout.apply(U0, field, list(dts), checkpoint, stats)
+
+
+def interior(U: torch.Tensor) -> torch.Tensor:
+    return U[:, NG:-NG, NG:-NG, NG:-NG]
+
+
+def mean_kinetic_energy(U: torch.Tensor) -> torch.Tensor:
+    """J = mean over interior cells of 0.5 rho |u|^2 = 0.5 |rho u|^2 / rho."""
+    I = interior(U)
+    return (0.5 * (I[1] * I[1] + I[2] * I[2] + I[3] * I[3]) / I[0]).mean()
+
+
+def state_tensor(field: DeviceField) -> torch.Tensor:
+    """The field's conserved state as a new (5, sz, sy, sx) CUDA tensor."""
+    U = torch.empty((5,) + field.shape, dtype=torch.float64, device="cuda")
+    field.set_stream(torch.cuda.current_stream().cuda_stream)
+    field.get_state(_ptrs(U))
+    return U

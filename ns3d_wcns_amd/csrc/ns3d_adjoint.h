@@ -1,0 +1,345 @@
+// ns3d_adjoint.h -- hand-written reverse mode (vector-Jacobian products) of
+// the FP64 face flux and the pointwise viscous terms, for the discrete
+// adjoint of the RHS.  Each function recomputes its forward values, then
+// propagates the output adjoint back to the inputs, accumulating (+=).
+//
+// The nonlinear WCNS5 / WENO5-JS weights are differentiated exactly.  The
+// non-smooth parts take the fixed subgradient of the tangent-linear model
+// (ns3d_dual.h): min / max follow the operand they return (the first on
+// ties), d|x| = sign(x) with sign(0) = +1, and every branch condition (the
+// HLLC wave pattern) is frozen at the forward values.
+//
+// Supported: component-wise WCNS5 and WENO5-JS interpolation with the HLLC
+// and Rusanov fluxes.
+#pragma once
+#include "ns3d_common.h"
+#include "ns3d_visc.h"
+
+namespace ns3d {
+
+// ------------------------------------------------------------ interpolation
+// Reverse mode of wcns5 (JS = false) / weno5js (JS = true) on f[0..4]: fb +=
+// yb dy/df; returns y.
+template <bool JS>
+NS3D_HD double weno_vjp(const double *f, double yb, double *fb) {
+    const double eps = 1e-6;
+    const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    const double t[3] = {f0 - 2.0 * f1 + f2, f1 - 2.0 * f2 + f3, f2 - 2.0 * f3 + f4};
+    const double s[3] = {f0 - 4.0 * f1 + 3.0 * f2, f1 - f3, 3.0 * f2 - 4.0 * f3 + f4};
+    double b[3], q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b[k] = (13.0 / 12.0) * t[k] * t[k] + 0.25 * s[k] * s[k];
+        q[k] = (eps + b[k]) * (eps + b[k]);
+    }
+    const double d0 = JS ? 0.1 : 1.0 / 16.0, d1 = JS ? 0.6 : 5.0 / 8.0, d2 = JS ? 0.3 : 5.0 / 16.0;
+    const double K = JS ? 6.0 : 8.0;
+    const double e[3] = {d0 * q[1] * q[2], d1 * q[0] * q[2], d2 * q[0] * q[1]};
+    // K x candidate values
+    const double pc[3][3] = {{JS ? 2.0 : 3.0, JS ? -7.0 : -10.0, JS ? 11.0 : 15.0},
+                             {-1.0, JS ? 5.0 : 6.0, JS ? 2.0 : 3.0},
+                             {JS ? 2.0 : 3.0, JS ? 5.0 : 6.0, -1.0}};
+    const double p[3] = {pc[0][0] * f0 + pc[0][1] * f1 + pc[0][2] * f2, pc[1][0] * f1 + pc[1][1] * f2 + pc[1][2] * f3,
+                         pc[2][0] * f2 + pc[2][1] * f3 + pc[2][2] * f4};
+    const double N = e[0] * p[0] + e[1] * p[1] + e[2] * p[2];
+    const double D = K * (e[0] + e[1] + e[2]);
+    const double y = N / D;
+    const double Nb = yb / D, Db = -yb * y / D;
+    double eb[3], pb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        eb[k] = Nb * p[k] + K * Db;
+        pb[k] = Nb * e[k];
+    }
+    const double qb[3] = {eb[1] * d1 * q[2] + eb[2] * d2 * q[1], eb[0] * d0 * q[2] + eb[2] * d2 * q[0],
+                          eb[0] * d0 * q[1] + eb[1] * d1 * q[0]};
+    double tb[3], sb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double bb = qb[k] * 2.0 * (eps + b[k]);
+        tb[k] = bb * (13.0 / 6.0) * t[k];
+        sb[k] = bb * 0.5 * s[k];
+    }
+    // t, s and p are linear in f
+    fb[0] += tb[0] + sb[0] + pb[0] * pc[0][0];
+    fb[1] += -2.0 * tb[0] + tb[1] - 4.0 * sb[0] + sb[1] + pb[0] * pc[0][1] + pb[1] * pc[1][0];
+    fb[2] += tb[0] - 2.0 * tb[1] + tb[2] + 3.0 * sb[0] + 3.0 * sb[2] + pb[0] * pc[0][2] + pb[1] * pc[1][1] +
+             pb[2] * pc[2][0];
+    fb[3] += tb[1] - 2.0 * tb[2] - sb[1] - 4.0 * sb[2] + pb[1] * pc[1][2] + pb[2] * pc[2][1];
+    fb[4] += tb[2] + sb[2] + pb[2] * pc[2][2];
+    return y;
+}
+
+// ------------------------------------------------------------ Riemann flux
+// Adjoints of the Prim fields of one side.
+struct PrimBar {
+    double rho, un, u1, u2, E, p, a, ir;
+};
+
+// Reverse mode of phys_flux: sb += (dF/ds)^T Fb.
+NS3D_HD void phys_flux_vjp(const Prim<double> &s, const double *Fb, PrimBar &sb) {
+    const double m = s.rho * s.un;
+    const double mb = Fb[0] + Fb[1] * s.un + Fb[2] * s.u1 + Fb[3] * s.u2;
+    sb.un += Fb[1] * m + Fb[4] * (s.E + s.p) + mb * s.rho;
+    sb.p += Fb[1] + Fb[4] * s.un;
+    sb.u1 += Fb[2] * m;
+    sb.u2 += Fb[3] * m;
+    sb.E += Fb[4] * s.un;
+    sb.rho += mb * s.un;
+}
+
+// Reverse mode of unpack: Ub += (ds/dU)^T sb.
+NS3D_HD void unpack_vjp(double g, const double *U, const Prim<double> &s, PrimBar sb, double *Ub) {
+    // a = sqrt(g p ir)
+    sb.p += sb.a * g * s.ir / (2.0 * s.a);
+    sb.ir += sb.a * g * s.p / (2.0 * s.a);
+    // p = (E - rho k / 2)(g - 1), k = un^2 + u1^2 + u2^2
+    const double k = s.un * s.un + s.u1 * s.u1 + s.u2 * s.u2;
+    sb.E += sb.p * (g - 1.0);
+    sb.rho -= sb.p * (g - 1.0) * 0.5 * k;
+    const double kb = -sb.p * (g - 1.0) * 0.5 * s.rho;
+    sb.un += 2.0 * kb * s.un;
+    sb.u1 += 2.0 * kb * s.u1;
+    sb.u2 += 2.0 * kb * s.u2;
+    // velocities = momenta * ir, ir = 1 / rho
+    Ub[1] += sb.un * s.ir;
+    Ub[2] += sb.u1 * s.ir;
+    Ub[3] += sb.u2 * s.ir;
+    sb.ir += sb.un * U[1] + sb.u1 * U[2] + sb.u2 * U[3];
+    sb.rho -= sb.ir * s.ir * s.ir;
+    Ub[0] += sb.rho;
+    Ub[4] += sb.E;
+}
+
+// Reverse mode of hllc (ns3d_device.h): ULb += (dF/dUL)^T Fb, URb likewise.
+NS3D_HD void hllc_vjp(const SchemeCoef<double> &c, const double *UL, const double *UR, const double *Fb, double *ULb,
+                      double *URb) {
+    const Prim<double> L = unpack(c, UL), Rr = unpack(c, UR);
+    PrimBar Lb{}, Rb{};
+    const double SLa = L.un - L.a, SLr = Rr.un - Rr.a;
+    const double SRa = L.un + L.a, SRr = Rr.un + Rr.a;
+    const bool selL = !(SLr < SLa), selR = !(SRr > SRa);  // fmin / fmax pick the first on ties
+    const double SL = selL ? SLa : SLr, SR = selR ? SRa : SRr;
+    if (SL >= 0.0) {
+        phys_flux_vjp(L, Fb, Lb);
+    } else if (SR <= 0.0) {
+        phys_flux_vjp(Rr, Fb, Rb);
+    } else {
+        double SLb = 0.0, SRb = 0.0;
+        const double num = Rr.rho * Rr.un * (SR - Rr.un) - L.rho * L.un * (SL - L.un) + L.p - Rr.p;
+        const double den = Rr.rho * (SR - Rr.un) - L.rho * (SL - L.un);
+        const bool roeavg = !(::fabs(den) > 1e-12);
+        double SM;
+        double sqL = 0.0, sqR = 0.0;
+        if (!roeavg) {
+            SM = num / den;
+        } else {
+            sqL = ::sqrt(L.rho);
+            sqR = ::sqrt(Rr.rho);
+            SM = (sqL * L.un + sqR * Rr.un) / (sqL + sqR);
+        }
+        const bool left = SM >= 0.0;
+        const Prim<double> &s = left ? L : Rr;
+        PrimBar &sb = left ? Lb : Rb;
+        const double S = left ? SL : SR;
+        const double *U = left ? UL : UR;
+        double *Ubd = left ? ULb : URb;
+        const double dS = S - s.un;
+        const double iq = 1.0 / (S - SM);
+        const double rs = s.rho * dS * iq;
+        const double Y = s.p / (s.rho * dS);
+        const double Z = SM + Y;
+        const double Es = s.E * s.ir + (SM - s.un) * Z;
+        // F = Fs + S (Us - U)
+        phys_flux_vjp(s, Fb, sb);
+        const double Us[5] = {rs, rs * SM, rs * s.u1, rs * s.u2, rs * Es};
+        double Sb = 0.0;
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            Sb += Fb[n] * (Us[n] - U[n]);
+            Ubd[n] -= S * Fb[n];
+        }
+        const double rsb = S * (Fb[0] + Fb[1] * SM + Fb[2] * s.u1 + Fb[3] * s.u2 + Fb[4] * Es);
+        double SMb = S * Fb[1] * rs;
+        sb.u1 += S * Fb[2] * rs;
+        sb.u2 += S * Fb[3] * rs;
+        const double Esb = S * Fb[4] * rs;
+        // Es = E ir + (SM - un) Z, Z = SM + p / (rho dS)
+        sb.E += Esb * s.ir;
+        sb.ir += Esb * s.E;
+        SMb += Esb * Z;
+        sb.un -= Esb * Z;
+        const double Zb = Esb * (SM - s.un);
+        SMb += Zb;
+        sb.p += Zb / (s.rho * dS);
+        sb.rho -= Zb * Y / s.rho;
+        double dSb = -Zb * Y / dS;
+        // rs = rho dS iq, iq = 1 / (S - SM), dS = S - un
+        sb.rho += rsb * dS * iq;
+        dSb += rsb * s.rho * iq;
+        const double iqb = rsb * s.rho * dS;
+        const double tq = -iqb * iq * iq;
+        Sb += tq;
+        SMb -= tq;
+        Sb += dSb;
+        sb.un -= dSb;
+        if (left) SLb += Sb;
+        else SRb += Sb;
+        // contact speed
+        if (!roeavg) {
+            const double numb = SMb / den, denb = -SMb * SM / den;
+            Rb.rho += numb * Rr.un * (SR - Rr.un) + denb * (SR - Rr.un);
+            Rb.un += numb * Rr.rho * (SR - 2.0 * Rr.un) - denb * Rr.rho;
+            SRb += numb * Rr.rho * Rr.un + denb * Rr.rho;
+            Lb.rho -= numb * L.un * (SL - L.un) + denb * (SL - L.un);
+            Lb.un -= numb * L.rho * (SL - 2.0 * L.un) - denb * L.rho;
+            SLb -= numb * L.rho * L.un + denb * L.rho;
+            Lb.p += numb;
+            Rb.p -= numb;
+        } else {
+            const double Dn = sqL + sqR;
+            const double Nmb = SMb / Dn, Dnb = -SMb * SM / Dn;
+            const double sqLb = Nmb * L.un + Dnb, sqRb = Nmb * Rr.un + Dnb;
+            Lb.un += Nmb * sqL;
+            Rb.un += Nmb * sqR;
+            Lb.rho += sqLb / (2.0 * sqL);
+            Rb.rho += sqRb / (2.0 * sqR);
+        }
+        // wave speeds
+        if (selL) {
+            Lb.un += SLb;
+            Lb.a -= SLb;
+        } else {
+            Rb.un += SLb;
+            Rb.a -= SLb;
+        }
+        if (selR) {
+            Lb.un += SRb;
+            Lb.a += SRb;
+        } else {
+            Rb.un += SRb;
+            Rb.a += SRb;
+        }
+    }
+    unpack_vjp(c.gamma, UL, L, Lb, ULb);
+    unpack_vjp(c.gamma, UR, Rr, Rb, URb);
+}
+
+// Reverse mode of rusanov.
+NS3D_HD void rusanov_vjp(const SchemeCoef<double> &c, const double *UL, const double *UR, const double *Fb,
+                         double *ULb, double *URb) {
+    const Prim<double> L = unpack(c, UL), Rr = unpack(c, UR);
+    PrimBar Lb{}, Rb{};
+    const double sl = ::fabs(L.un) + L.a, sr = ::fabs(Rr.un) + Rr.a;
+    const bool pickL = sl > sr;  // smax = sl > sr ? sl : sr
+    const double smax = pickL ? sl : sr;
+    double hb[5], smb = 0.0;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        hb[n] = 0.5 * Fb[n];
+        ULb[n] += 0.5 * smax * Fb[n];
+        URb[n] -= 0.5 * smax * Fb[n];
+        smb -= 0.5 * Fb[n] * (UR[n] - UL[n]);
+    }
+    phys_flux_vjp(L, hb, Lb);
+    phys_flux_vjp(Rr, hb, Rb);
+    if (pickL) {
+        Lb.un += L.un >= 0.0 ? smb : -smb;
+        Lb.a += smb;
+    } else {
+        Rb.un += Rr.un >= 0.0 ? smb : -smb;
+        Rb.a += smb;
+    }
+    unpack_vjp(c.gamma, UL, L, Lb, ULb);
+    unpack_vjp(c.gamma, UR, Rr, Rb, URb);
+}
+
+// Reverse mode of the component-wise face flux face_flux<INTERP, RS, false>
+// on the rotated stencil st[var][m] (cells i-2..i+3): stb += (dF/dst)^T Fb.
+template <int INTERP, int RS>
+NS3D_HD void face_vjp(const SchemeCoef<double> &c, const double (&st)[5][6], const double *Fb, double (&stb)[5][6]) {
+    static_assert(INTERP == I_WENO5 || INTERP == I_WENO5_JS, "adjoint: WCNS5 or WENO5-JS interpolation");
+    static_assert(RS == R_HLLC || RS == R_RUSANOV, "adjoint: HLLC or Rusanov flux");
+    constexpr bool JS = INTERP == I_WENO5_JS;
+    double UL[5], UR[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) interp_lr<INTERP>(c, st[n], UL[n], UR[n]);
+    double ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
+    if constexpr (RS == R_HLLC) hllc_vjp(c, UL, UR, Fb, ULb, URb);
+    else rusanov_vjp(c, UL, UR, Fb, ULb, URb);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        double fb[5] = {0, 0, 0, 0, 0};
+        const double fl[5] = {st[n][0], st[n][1], st[n][2], st[n][3], st[n][4]};
+        weno_vjp<JS>(fl, ULb[n], fb);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) stb[n][m] += fb[m];
+        double gb[5] = {0, 0, 0, 0, 0};
+        const double fr[5] = {st[n][5], st[n][4], st[n][3], st[n][2], st[n][1]};
+        weno_vjp<JS>(fr, URb[n], gb);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) stb[n][5 - m] += gb[m];
+    }
+}
+
+// ------------------------------------------------------------ viscous terms
+// Reverse mode of visc_local: from the adjoints Vb of the nine flux
+// components, accumulate gb (twelve gradients) and qb (u, v, w, T).
+NS3D_HD void visc_local_vjp(double mu_ref, double S_ref, double cp_pr, double u, double v, double w, double T,
+                            const double (&g)[4][3], const double (&Vb)[NVISC], double (&gb)[4][3],
+                            double (&qb)[4]) {
+    const double mu = sutherland(mu_ref, S_ref, T);
+    const double divU = g[0][0] + g[1][1] + g[2][2];
+    const double txx = 2.0 * mu * g[0][0] - (2.0 / 3.0) * mu * divU;
+    const double tyy = 2.0 * mu * g[1][1] - (2.0 / 3.0) * mu * divU;
+    const double tzz = 2.0 * mu * g[2][2] - (2.0 / 3.0) * mu * divU;
+    const double txy = mu * (g[0][1] + g[1][0]);
+    const double txz = mu * (g[0][2] + g[2][0]);
+    const double tyz = mu * (g[1][2] + g[2][1]);
+    const double kq = mu * cp_pr;
+    const double ex = Vb[VEX], ey = Vb[VEY], ez = Vb[VEZ];
+    // energy fluxes u . tau + kq grad T
+    double txxb = Vb[VXX] + ex * u, tyyb = Vb[VYY] + ey * v, tzzb = Vb[VZZ] + ez * w;
+    double txyb = Vb[VXY] + ex * v + ey * u, txzb = Vb[VXZ] + ex * w + ez * u, tyzb = Vb[VYZ] + ey * w + ez * v;
+    qb[0] += ex * txx + ey * txy + ez * txz;
+    qb[1] += ex * txy + ey * tyy + ez * tyz;
+    qb[2] += ex * txz + ey * tyz + ez * tzz;
+    const double kqb = ex * g[3][0] + ey * g[3][1] + ez * g[3][2];
+    gb[3][0] += ex * kq;
+    gb[3][1] += ey * kq;
+    gb[3][2] += ez * kq;
+    // stresses
+    double mub = kqb * cp_pr;
+    mub += txxb * (2.0 * g[0][0] - (2.0 / 3.0) * divU) + tyyb * (2.0 * g[1][1] - (2.0 / 3.0) * divU) +
+           tzzb * (2.0 * g[2][2] - (2.0 / 3.0) * divU) + txyb * (g[0][1] + g[1][0]) + txzb * (g[0][2] + g[2][0]) +
+           tyzb * (g[1][2] + g[2][1]);
+    const double divb = -(2.0 / 3.0) * mu * (txxb + tyyb + tzzb);
+    gb[0][0] += 2.0 * mu * txxb + divb;
+    gb[1][1] += 2.0 * mu * tyyb + divb;
+    gb[2][2] += 2.0 * mu * tzzb + divb;
+    gb[0][1] += mu * txyb;
+    gb[1][0] += mu * txyb;
+    gb[0][2] += mu * txzb;
+    gb[2][0] += mu * txzb;
+    gb[1][2] += mu * tyzb;
+    gb[2][1] += mu * tyzb;
+    // mu = mu_ref T^1.5 (1 + S) / (T + S): dmu/dT = mu (1.5 / T - 1 / (T + S))
+    qb[3] += mub * mu * (1.5 / T - 1.0 / (T + S_ref));
+}
+
+// Reverse mode of the cell primitives (u, v, w, T) of a conserved state U
+// (conservedToPrimitive, field_structures.h:444-466, with T = p / (rho Rgas)):
+// Ub += (dq/dU)^T qb.
+NS3D_HD void c2p_vjp(double gamma, double Rgas, const double *U, const double *qb, double *Ub) {
+    const double ir = 1.0 / U[0];
+    const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+    const double cT = (gamma - 1.0) / Rgas * ir;  // dT/dE
+    // T = (gamma - 1) / Rgas (E / rho - |u|^2 / 2)
+    Ub[0] += -(qb[0] * u + qb[1] * v + qb[2] * w) * ir + qb[3] * cT * (u * u + v * v + w * w - U[4] * ir);
+    Ub[1] += qb[0] * ir - qb[3] * cT * u;
+    Ub[2] += qb[1] * ir - qb[3] * cT * v;
+    Ub[3] += qb[2] * ir - qb[3] * cT * w;
+    Ub[4] += qb[3] * cT;
+}
+
+}  // namespace ns3d

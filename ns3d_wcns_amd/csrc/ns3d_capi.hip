@@ -36,6 +36,14 @@ struct ns3d_ctx {
     double *rz[5], *ry[5];  // compact partial RHS of the z and y pencils
     double *grad_slab;
     double *grad[12];
+    // tangent / adjoint workspace (allocated on first use)
+    double *adj_slab;
+    double *xg[5];   // ghosted copy of the tangent / adjoint input, ghosts filled
+    double *dq[4];   // ghosted (u, v, w, T) tangents
+    double *s1[5];   // stage-1 state of the step being differentiated
+    double *a0[5], *a1[5], *pr[5];  // stage accumulators and the RHS derivative
+    double *cw[30];  // compact: face contributions (adjoint) / face and viscous tangents
+    double *gb[12], *qb[4];
     double *partial;
     double *scal;  // [0] dt, [1] reduction result
     bool timing;
@@ -394,11 +402,254 @@ int ensure_grad(ns3d_ctx *c) {
     return NS3D_OK;
 }
 
+// ------------------------------------------------------------ derivatives
+int ensure_adj(ns3d_ctx *c) {
+    if (c->adj_slab) return NS3D_OK;
+    const size_t cs = field_stride(c->ncell), is = field_stride(c->nint);
+    constexpr size_t kGhosted = 5 + 4 + 5 + 5 + 5 + 5;  // xg, dq, s1, a0, a1, pr
+    constexpr size_t kCompact = 30 + 12 + 4;            // cw, gb, qb
+    const size_t total = kGhosted * cs + kCompact * is;
+    if (hipMalloc(&c->adj_slab, total * sizeof(double)) != hipSuccess) {
+        c->adj_slab = nullptr;
+        (void)hipGetLastError();
+        return fail(NS3D_E_NOMEM, "tangent/adjoint workspace: out of device memory for %zu doubles", total);
+    }
+    double *p = c->adj_slab;
+    for (int n = 0; n < 5; ++n, p += cs) c->xg[n] = p;
+    for (int n = 0; n < 4; ++n, p += cs) c->dq[n] = p;
+    for (int n = 0; n < 5; ++n, p += cs) c->s1[n] = p;
+    for (int n = 0; n < 5; ++n, p += cs) c->a0[n] = p;
+    for (int n = 0; n < 5; ++n, p += cs) c->a1[n] = p;
+    for (int n = 0; n < 5; ++n, p += cs) c->pr[n] = p;
+    for (int n = 0; n < 30; ++n, p += is) c->cw[n] = p;
+    for (int n = 0; n < 12; ++n, p += is) c->gb[n] = p;
+    for (int n = 0; n < 4; ++n, p += is) c->qb[n] = p;
+    if ((size_t)(p - c->adj_slab) != total) {  // the carve must match the allocation
+        (void)hipFree(c->adj_slab);
+        c->adj_slab = nullptr;
+        return fail(NS3D_E_HIP, "internal: tangent/adjoint workspace layout mismatch");
+    }
+    NS3D_HIP(hipMemsetAsync(c->adj_slab, 0, total * sizeof(double), c->stream));
+    return NS3D_OK;
+}
+
+// Primitives and ghost layers consistent with the conserved state.
+int refresh_state(ns3d_ctx *c) {
+    launch_c2p(conv_args(c), c->stream);
+    NS3D_LAUNCHED("c2p");
+    return halo(c, true);
+}
+
+int fill_ghosts(ns3d_ctx *c, double *const *f, int nf) {
+    HaloArgs h{};
+    h.b = c->b;
+    for (int n = 0; n < nf; ++n) h.f[n] = f[n];
+    h.nf = nf;
+    launch_halo(h, c->stream);
+    NS3D_LAUNCHED("halo");
+    return NS3D_OK;
+}
+
+int copy_fields(ns3d_ctx *c, double *const *dst, const double *const *src, int nf) {
+    for (int n = 0; n < nf; ++n)
+        NS3D_HIP(hipMemcpyAsync(dst[n], src[n], c->ncell * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return NS3D_OK;
+}
+
+int zero_fields(ns3d_ctx *c, double *const *f, int nf) {
+    for (int n = 0; n < nf; ++n) NS3D_HIP(hipMemsetAsync(f[n], 0, c->ncell * sizeof(double), c->stream));
+    return NS3D_OK;
+}
+
+AdjArgs adj_args(ns3d_ctx *c, double *const *out) {
+    AdjArgs a{};
+    for (int n = 0; n < 5; ++n) {
+        a.U[n] = conserved(c)[n];
+        a.Q[n] = prims(c)[n];
+        a.X[n] = c->xg[n];
+        a.out[n] = out[n];
+    }
+    for (int n = 0; n < 30; ++n) a.C[n] = c->cw[n];
+    for (int n = 0; n < 12; ++n) a.Gb[n] = c->gb[n];
+    for (int n = 0; n < 4; ++n) {
+        a.qb[n] = c->qb[n];
+        a.dQ[n] = c->dq[n];
+    }
+    for (int d = 0; d < 3; ++d)
+        for (int n = 0; n < 5; ++n) a.dF[d][n] = c->cw[5 * d + n];
+    for (int n = 0; n < NVISC; ++n) a.dV[n] = c->cw[15 + n];
+    a.b = c->b;
+    a.c = coef64(c->P);
+    const double h[3] = {c->grid.dx, c->grid.dy, c->grid.dz};
+    for (int d = 0; d < 3; ++d) {
+        a.ih[d] = 1.0 / h[d];
+        a.i60[d] = 1.0 / (60.0 * h[d]);
+    }
+    a.gamma = c->P.gamma;
+    a.Rgas = c->P.Rgas;
+    a.mu = c->P.mu;
+    a.S_ref = c->P.S_ref;
+    a.cp_pr = c->P.Cp / c->P.Pr;
+    return a;
+}
+
+// Tangent RHS at the current state of the tangent in c->xg (ghosts filled).
+int tangent_rhs_impl(ns3d_ctx *c, double *const *out) {
+    int rc;
+    if ((rc = zero_fields(c, out, 5))) return rc;
+    const AdjArgs a = adj_args(c, out);
+    launch_tl_prim(a, c->stream);
+    NS3D_LAUNCHED("tangent primitives");
+    if ((rc = fill_ghosts(c, c->dq, 4))) return rc;
+    launch_tl_rhs(a, c->P.interpolation, c->P.riemann_solver, c->P.char_recon != 0, c->stream);
+    NS3D_LAUNCHED("tangent rhs");
+    return NS3D_OK;
+}
+
+// Adjoint RHS at the current state of the adjoint in c->xg (ghosts filled).
+int adjoint_rhs_impl(ns3d_ctx *c, double *const *out) {
+    int rc;
+    if ((rc = zero_fields(c, out, 5))) return rc;
+    launch_adj_rhs(adj_args(c, out), c->P.interpolation, c->P.riemann_solver, c->stream);
+    NS3D_LAUNCHED("adjoint rhs");
+    return NS3D_OK;
+}
+
+int load_input(ns3d_ctx *c, const double *const *in) {
+    int rc;
+    if ((rc = copy_fields(c, c->xg, in, 5))) return rc;
+    return fill_ghosts(c, c->xg, 5);
+}
+
+int lincomb(ns3d_ctx *c, double *const *out, double alpha, double *const *A, double beta, double *const *B,
+            double gamma, double *const *P) {
+    LinComb l{};
+    for (int n = 0; n < 5; ++n) {
+        l.out[n] = out[n];
+        l.A[n] = A ? A[n] : nullptr;
+        l.B[n] = B ? B[n] : nullptr;
+        l.P[n] = P ? P[n] : nullptr;
+    }
+    l.alpha = alpha;
+    l.beta = beta;
+    l.gamma = gamma;
+    l.b = c->b;
+    launch_lincomb(l, c->stream);
+    NS3D_LAUNCHED("lincomb");
+    return NS3D_OK;
+}
+
+int check_ptrs(const void *const *p, int n) {
+    if (!p) return fail(NS3D_E_ARG, "null field array");
+    for (int q = 0; q < n; ++q)
+        if (!p[q]) return fail(NS3D_E_ARG, "null field pointer %d", q);
+    return NS3D_OK;
+}
+
+int check_adjoint(ns3d_ctx *c) {
+    if (!adjoint_supported(c->P.interpolation, c->P.riemann_solver, c->P.char_recon != 0))
+        return fail(NS3D_E_UNSUPPORTED,
+                    "the adjoint covers component-wise WCNS5 / WENO5-JS interpolation with HLLC or Rusanov flux "
+                    "(got interpolation %d, Riemann solver %d, char_recon %d)",
+                    c->P.interpolation, c->P.riemann_solver, c->P.char_recon);
+    return NS3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char *ns3d_last_error(void) { return g_err.c_str(); }
+
+int ns3d_copy_field_to_device(ns3d_ctx *c, int field, double *dst) {
+    NS3D_CTX(c);
+    if (field < 0 || field >= NS3D_NUM_CELL_FIELDS || !dst) return fail(NS3D_E_ARG, "bad field or pointer");
+    NS3D_HIP(hipMemcpyAsync(dst, c->cell[field], c->ncell * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_copy_field_from_device(ns3d_ctx *c, int field, const double *src) {
+    NS3D_CTX(c);
+    if (field < 0 || field >= NS3D_NUM_CELL_FIELDS || !src) return fail(NS3D_E_ARG, "bad field or pointer");
+    NS3D_HIP(hipMemcpyAsync(c->cell[field], src, c->ncell * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_set_state(ns3d_ctx *c, const double *const *U) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)U, 5))) return rc;
+    if ((rc = copy_fields(c, conserved(c), U, 5))) return rc;
+    return refresh_state(c);
+}
+
+int ns3d_get_state(ns3d_ctx *c, double *const *U) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)U, 5))) return rc;
+    return copy_fields(c, U, conserved(c), 5);
+}
+
+int ns3d_tangent_rhs(ns3d_ctx *c, const double *const *dU, double *const *out) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)dU, 5)) || (rc = check_ptrs((const void *const *)out, 5))) return rc;
+    if ((rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = load_input(c, dU))) return rc;
+    return tangent_rhs_impl(c, out);
+}
+
+int ns3d_adjoint_rhs(ns3d_ctx *c, const double *const *lam, double *const *out) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)lam, 5)) || (rc = check_ptrs((const void *const *)out, 5))) return rc;
+    if ((rc = check_adjoint(c)) || (rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = load_input(c, lam)))
+        return rc;
+    return adjoint_rhs_impl(c, out);
+}
+
+int ns3d_tangent_step(ns3d_ctx *c, double dt, double *const *dU) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)dU, 5))) return rc;
+    if ((rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = record0(c))) return rc;
+    if ((rc = copy_fields(c, c->a0, dU, 5))) return rc;  // delta U^n
+    static const double A[3] = {1.0, 0.75, 1.0 / 3.0}, B[3] = {0.0, 0.25, 2.0 / 3.0};
+    for (int s = 0; s < 3; ++s) {
+        if ((rc = load_input(c, dU)) || (rc = tangent_rhs_impl(c, c->pr))) return rc;
+        // delta U^(s+1) = A delta U^n + B delta U^(s) + B dt P  (stage 1: delta U^n + dt P)
+        const double g = s == 0 ? dt : B[s] * dt;
+        if ((rc = lincomb(c, dU, A[s], c->a0, B[s], s == 0 ? nullptr : dU, g, c->pr))) return rc;
+        if ((rc = fused_stage(c, s + 1, nullptr, dt))) return rc;
+    }
+    return NS3D_OK;
+}
+
+int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)adj, 5))) return rc;
+    if ((rc = check_adjoint(c)) || (rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = record0(c))) return rc;
+    // forward stages 1 and 2 (U^(1) kept, U^(2) left in the context)
+    if ((rc = fused_stage(c, 1, nullptr, dt)) || (rc = copy_fields(c, c->s1, conserved(c), 5)) ||
+        (rc = fused_stage(c, 2, nullptr, dt)))
+        return rc;
+    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3
+    if ((rc = load_input(c, adj)) || (rc = adjoint_rhs_impl(c, c->pr))) return rc;
+    if ((rc = lincomb(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr)) ||
+        (rc = lincomb(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr)))
+        return rc;
+    // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
+    if ((rc = copy_fields(c, conserved(c), c->s1, 5)) || (rc = refresh_state(c))) return rc;
+    if ((rc = load_input(c, c->a1)) || (rc = adjoint_rhs_impl(c, c->pr))) return rc;
+    if ((rc = lincomb(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr)) ||
+        (rc = lincomb(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr)))
+        return rc;
+    // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P
+    if ((rc = copy_fields(c, conserved(c), recorded(c), 5)) || (rc = refresh_state(c))) return rc;
+    if ((rc = load_input(c, c->a1)) || (rc = adjoint_rhs_impl(c, c->pr))) return rc;
+    if ((rc = zero_fields(c, adj, 5))) return rc;
+    return lincomb(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
+}
 
 int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params, int precision) {
     if (!out || !grid) return fail(NS3D_E_ARG, "ns3d_create: null argument");
@@ -447,6 +698,7 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     p += npart;
     c->scal = p;
     c->grad_slab = nullptr;
+    c->adj_slab = nullptr;
     c->timing = false;
     c->visc_fp32 = false;
     if (hipMemset(c->slab, 0, total * sizeof(double)) != hipSuccess) {
@@ -499,6 +751,7 @@ int ns3d_destroy(ns3d_ctx *c) {
         (void)hipEventDestroy(e.second.second);
     }
     if (c->grad_slab) (void)hipFree(c->grad_slab);
+    if (c->adj_slab) (void)hipFree(c->adj_slab);
     (void)hipFree(c->slab);
     delete c;
     return NS3D_OK;

@@ -3,6 +3,7 @@
 #include <algorithm>
 
 #include "ns3d_common.h"
+#include "ns3d_visc.h"
 
 namespace ns3d {
 
@@ -65,6 +66,39 @@ struct KeArgs {
     Box b;
     double vol;
 };
+
+// Tangent-linear / adjoint RHS arguments (ns3d_adjoint.hip).  Cell fields
+// are ghosted with filled face ghosts; work arrays are compact.
+struct AdjArgs {
+    const double *U[5];   // state
+    const double *Q[5];   // its primitives u, v, w, p, T
+    const double *X[5];   // tangent dU (tangent) or lambda (adjoint)
+    double *out[5];       // tangent RHS / adjoint (interior written)
+    double *C[30];        // adjoint: face contributions, slot m * 5 + var
+    double *Gb[12];       // adjoint: gradient adjoints
+    double *qb[4];        // adjoint: (u, v, w, T) adjoints
+    double *dQ[4];        // tangent: ghosted (u, v, w, T) tangents
+    double *dF[3][5];     // tangent: face flux tangents per direction
+    double *dV[NVISC];    // tangent: viscous flux tangents
+    Box b;
+    SchemeCoef<double> c;
+    double ih[3], i60[3];
+    double gamma, Rgas, mu, S_ref, cp_pr;
+};
+
+// out = alpha A + beta B + gamma P over the interior (null operands are 0).
+struct LinComb {
+    double *out[5];
+    const double *A[5], *B[5], *P[5];
+    double alpha, beta, gamma;
+    Box b;
+};
+
+bool adjoint_supported(int interp, int rs, bool chr);
+void launch_tl_prim(const AdjArgs &a, hipStream_t s);
+void launch_tl_rhs(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s);
+void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s);
+void launch_lincomb(const LinComb &l, hipStream_t s);
 
 long ke_partial_count(const Box &b);
 void launch_visc(const ViscArgs &a, hipStream_t s);

@@ -5,24 +5,10 @@
 
 namespace ns3d {
 
-// 6th-order central first derivative of a 7-point line without its centre,
-// times 1/(60 h) (diff_6th_central, ViscousFlux.cpp:27-29).
-template <class R>
-__device__ __forceinline__ R d6c(R fm3, R fm2, R fm1, R fp1, R fp2, R fp3, R i60h) {
-    return (fp3 - R(9) * fp2 + R(45) * fp1 - R(45) * fm1 + R(9) * fm2 - fm3) * i60h;
-}
-
 // 6th-order midpoint-to-node difference of faces i-5/2..i+5/2 times 1/h
 // (diff_6th_central_half, InviscousFlux.cpp:990-992).
 __device__ __forceinline__ double d6h(double f0, double f1, double f2, double f3, double f4, double f5, double ih) {
     return ((f3 - f2) * (75.0 / 64.0) + (f4 - f1) * (-25.0 / 384.0) + (f5 - f0) * (3.0 / 640.0)) * ih;
-}
-
-// Sutherland viscosity mu(T) = mu_ref T^1.5 (1 + S) / (T + S)
-// (SolverParams::get_mu, field_structures.h:96-106).
-template <class R>
-__device__ __forceinline__ R sutherland(R mu, R S, R T) {
-    return fdiv(mu * (T * fsqrt(T)) * (R(1) + S), T + S);
 }
 
 // ---------------------------------------------------------------- viscous
@@ -51,26 +37,10 @@ __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
             for (int d = 0; d < 3; ++d) a.G[3 * q + d][o] = g[q][d];
     }
     if (!a.V[0]) return;
-    const double T = a.Q[3][c];
-    const double mu = sutherland(a.mu, a.S_ref, T);
-    const double divU = g[0][0] + g[1][1] + g[2][2];
-    const double txx = 2.0 * mu * g[0][0] - (2.0 / 3.0) * mu * divU;
-    const double tyy = 2.0 * mu * g[1][1] - (2.0 / 3.0) * mu * divU;
-    const double tzz = 2.0 * mu * g[2][2] - (2.0 / 3.0) * mu * divU;
-    const double txy = mu * (g[0][1] + g[1][0]);
-    const double txz = mu * (g[0][2] + g[2][0]);
-    const double tyz = mu * (g[1][2] + g[2][1]);
-    const double kq = mu * a.cp_pr;
-    const double u = a.Q[0][c], v = a.Q[1][c], w = a.Q[2][c];
-    a.V[VXX][o] = txx;
-    a.V[VYY][o] = tyy;
-    a.V[VZZ][o] = tzz;
-    a.V[VXY][o] = txy;
-    a.V[VXZ][o] = txz;
-    a.V[VYZ][o] = tyz;
-    a.V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
-    a.V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
-    a.V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
+    double V[NVISC];
+    visc_local(a.mu, a.S_ref, a.cp_pr, a.Q[0][c], a.Q[1][c], a.Q[2][c], a.Q[3][c], g, V);
+#pragma unroll
+    for (int n = 0; n < NVISC; ++n) a.V[n][o] = V[n];
 }
 
 // The same viscous fluxes from a z-marching 64 x 4 column tile: each thread
@@ -161,27 +131,12 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
                 g[q][2] = d6c(wq[q][0], wq[q][1], wq[q][2], wq[q][4], wq[q][5], wq[q][6], i60[2]);
             }
             const R T = sizeof(R) == 4 ? wq[3][3] + R(1) : wq[3][3];
-            const R mu = sutherland(mu_ref, S_ref, T);
-            const R divU = g[0][0] + g[1][1] + g[2][2];
-            const R txx = R(2) * mu * g[0][0] - R(2.0 / 3.0) * mu * divU;
-            const R tyy = R(2) * mu * g[1][1] - R(2.0 / 3.0) * mu * divU;
-            const R tzz = R(2) * mu * g[2][2] - R(2.0 / 3.0) * mu * divU;
-            const R txy = mu * (g[0][1] + g[1][0]);
-            const R txz = mu * (g[0][2] + g[2][0]);
-            const R tyz = mu * (g[1][2] + g[2][1]);
-            const R kq = mu * cp_pr;
-            const R u = wq[0][3], v = wq[1][3], w = wq[2][3];
+            R Vc[NVISC];
+            visc_local(mu_ref, S_ref, cp_pr, wq[0][3], wq[1][3], wq[2][3], T, g, Vc);
             const long o = b.oidx(i, j, k);
             R *const *V = reinterpret_cast<R *const *>(a.V);
-            V[VXX][o] = txx;
-            V[VYY][o] = tyy;
-            V[VZZ][o] = tzz;
-            V[VXY][o] = txy;
-            V[VXZ][o] = txz;
-            V[VYZ][o] = tyz;
-            V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
-            V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
-            V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
+#pragma unroll
+            for (int n = 0; n < NVISC; ++n) V[n][o] = Vc[n];
         }
 #pragma unroll
         for (int m = 0; m < 6; ++m)
@@ -203,26 +158,10 @@ __global__ void __launch_bounds__(256) k_visc_from_grad(ViscArgs a) {
     for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int d = 0; d < 3; ++d) g[q][d] = a.G[3 * q + d][o];
-    const double T = a.Q[3][c];
-    const double mu = sutherland(a.mu, a.S_ref, T);
-    const double divU = g[0][0] + g[1][1] + g[2][2];
-    const double txx = 2.0 * mu * g[0][0] - (2.0 / 3.0) * mu * divU;
-    const double tyy = 2.0 * mu * g[1][1] - (2.0 / 3.0) * mu * divU;
-    const double tzz = 2.0 * mu * g[2][2] - (2.0 / 3.0) * mu * divU;
-    const double txy = mu * (g[0][1] + g[1][0]);
-    const double txz = mu * (g[0][2] + g[2][0]);
-    const double tyz = mu * (g[1][2] + g[2][1]);
-    const double kq = mu * a.cp_pr;
-    const double u = a.Q[0][c], v = a.Q[1][c], w = a.Q[2][c];
-    a.V[VXX][o] = txx;
-    a.V[VYY][o] = tyy;
-    a.V[VZZ][o] = tzz;
-    a.V[VXY][o] = txy;
-    a.V[VXZ][o] = txz;
-    a.V[VYZ][o] = tyz;
-    a.V[VEX][o] = u * txx + v * txy + w * txz + kq * g[3][0];
-    a.V[VEY][o] = u * txy + v * tyy + w * tyz + kq * g[3][1];
-    a.V[VEZ][o] = u * txz + v * tyz + w * tzz + kq * g[3][2];
+    double V[NVISC];
+    visc_local(a.mu, a.S_ref, a.cp_pr, a.Q[0][c], a.Q[1][c], a.Q[2][c], a.Q[3][c], g, V);
+#pragma unroll
+    for (int n = 0; n < NVISC; ++n) a.V[n][o] = V[n];
 }
 
 // ---------------------------------------------------------------- assembly

@@ -198,6 +198,45 @@ class DeviceField:
         check(self._lib.ns3d_total_kinetic_energy(self._h, ctypes.byref(out)))
         return out.value
 
+    # ------------------------------------------------ device buffers, derivatives
+    # Device arrays are addresses (ints) of sx*sy*sz doubles in the ghosted
+    # layout, five of them (rho, rho u, rho v, rho w, E) for a state.
+    def copy_field_to(self, name: str, dst: int):
+        check(self._lib.ns3d_copy_field_to_device(self._h, FIELDS[name], ctypes.c_void_p(dst)))
+
+    def copy_field_from(self, name: str, src: int):
+        check(self._lib.ns3d_copy_field_from_device(self._h, FIELDS[name], ctypes.c_void_p(src)))
+
+    def set_state(self, ptrs):
+        """Conserved state from five device arrays; primitives and ghost
+        layers are made consistent with it."""
+        check(self._lib.ns3d_set_state(self._h, _lib.ptr5(ptrs)))
+
+    def get_state(self, ptrs):
+        check(self._lib.ns3d_get_state(self._h, _lib.ptr5(ptrs)))
+
+    def tangent_rhs(self, dU, out):
+        """out = (dRHS/dU) dU at the current state."""
+        check(self._lib.ns3d_tangent_rhs(self._h, _lib.ptr5(dU), _lib.ptr5(out)))
+
+    def adjoint_rhs(self, lam, out):
+        """out = (dRHS/dU)^T lam at the current state."""
+        check(self._lib.ns3d_adjoint_rhs(self._h, _lib.ptr5(lam), _lib.ptr5(out)))
+
+    def tangent_step(self, dt: float, dU):
+        """One RK3 step of the state with fixed dt; dU becomes its tangent."""
+        check(self._lib.ns3d_tangent_step(self._h, dt, _lib.ptr5(dU)))
+
+    def adjoint_step(self, dt: float, adj):
+        """adj <- (dU^{n+1}/dU^n)^T adj for the step from the current state
+        U^n with fixed dt; the state stays U^n."""
+        check(self._lib.ns3d_adjoint_step(self._h, dt, _lib.ptr5(adj)))
+
+    def step(self, dt: float):
+        """recordConservedTo0 + runge_kutta_3 with a fixed dt."""
+        self.recordConservedTo0()
+        self.runge_kutta_3(dt)
+
     def snapshot(self) -> dict:
         """Interior copies of the conserved fields and the primitives."""
         return {name: self.interior(self.download(name)).copy() for name in CONSERVED + PRIMITIVE}
