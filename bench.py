@@ -66,6 +66,10 @@ def parse():
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed method and report its error")
     ap.add_argument("--err-steps", type=int, default=20,
                     help="steps of the mixed-vs-FP64 error comparison from the same state")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="also run the adjoint workload (128^3, 50 steps, checkpoint every 10, FD check)")
+    ap.add_argument("--adj-n", type=int, default=128)
+    ap.add_argument("--adj-steps", type=int, default=50)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU checker on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
@@ -124,6 +128,118 @@ def mixed_report(args, prim, G, P, ms_plain, timed_run):
             "spectrum_max_shell": float(f"{rep['spectrum_max']:.3e}"),
             "spectrum_per_shell": [float(f"{e:.2e}") for e in rep["spectrum_per_shell"]],
         },
+    }
+
+
+def adjoint_report(args):
+    """Discrete adjoint of an isotropic-turbulence run: --adj-n^3, Ma_t = 0.3,
+    Re = 100, WCNS5 + HLLC, --adj-steps RK3 steps at the CFL-0.5 time steps of
+    the forward run, checkpoint every 10 steps.  J = mean 0.5 rho |u|^2 at the
+    final time; gradient with respect to the initial conserved fields.
+    Reports the backward/forward cost ratio of the whole sweep (recomputation
+    included) and per step, central-difference checks (eps = 1e-6) on 8
+    seeded unit-norm directions in (rho u, rho v, rho w, E), and the
+    tangent/adjoint dot test of one step."""
+    import torch
+
+    from ns3d_wcns_amd import DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
+    from ns3d_wcns_amd.adjoint import central_difference, mean_kinetic_energy, rollout, state_tensor
+    from ns3d_wcns_amd.isotropic import isotropic_state
+
+    n, nsteps = args.adj_n, args.adj_steps
+    prim, ma = isotropic_state(n, mach_t=0.3, seed=2024)
+    P = SolverParams(Ma=ma, Re=100.0, Pr=0.72, cfl=0.5, interpolation=SCHEME_ALIASES["wcns5"],
+                     riemann_solver=RiemannSolver.HLLC).derive()
+    f = DeviceField(periodic_box(n), P)
+    f.set_stream(torch.cuda.current_stream().cuda_stream)
+    f.initialize_primitive(*prim)
+    U = state_tensor(f)
+    dts = []
+    for _ in range(nsteps):
+        dts.append(f.compute_timestep())
+        f.step(dts[-1])
+    policy = "uniform:10"
+    # warm-up of both sweeps, then the measured one
+    U0 = U.clone().requires_grad_(True)
+    mean_kinetic_energy(rollout(U0, f, dts[:2], policy)).backward()
+    stats = {}
+    U0 = U.clone().requires_grad_(True)
+    J = mean_kinetic_energy(rollout(U0, f, dts, policy, stats))
+    J.backward()
+    grad = U0.grad
+    # per-step costs: forward step, adjoint step (state loaded, stages recomputed inside)
+    adj = torch.zeros_like(U)
+    adj[:, 3:-3, 3:-3, 3:-3] = 1e-3
+    ptr = [adj[q].data_ptr() for q in range(5)]
+    Up = [U[q].data_ptr() for q in range(5)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    f.set_state(Up)
+    torch.cuda.synchronize()
+    e0.record()
+    for i in range(10):
+        f.step(dts[i])
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / 10
+    f.set_state(Up)
+    torch.cuda.synchronize()
+    e0.record()
+    for i in range(10):
+        f.adjoint_step(dts[i], ptr)
+    e1.record()
+    torch.cuda.synchronize()
+    adj_ms = e0.elapsed_time(e1) / 10
+    # central differences
+    g = torch.Generator(device="cpu").manual_seed(7)
+    worst, checks, cs_worst, wide = 0.0, [], 0.0, []
+    gnorm = float(torch.linalg.vector_norm(grad[1:, 3:-3, 3:-3, 3:-3]))
+    eps = 1e-6
+    for k in range(8):
+        d = torch.zeros_like(U)
+        d[1:, 3:-3, 3:-3, 3:-3] = torch.randn((4, n, n, n), generator=g, dtype=torch.float64).cuda()
+        d /= torch.linalg.vector_norm(d)
+        with torch.no_grad():
+            fd = float(central_difference(f, U, d, dts, eps))
+        ad = float((grad * d).sum())
+        rel = abs(fd - ad) / abs(ad)
+        worst = max(worst, rel)
+        checks.append(float(f"{rel:.2e}"))
+        # error relative to |grad J| |d| (Cauchy-Schwarz scale of any direction)
+        cs_worst = max(cs_worst, abs(fd - ad) / gnorm)
+        if k < 2:  # the rounding floor: the same check at eps = 1e-4
+            fd4 = float(central_difference(f, U, d, dts, 1e-4))
+            wide.append(float(f"{abs(fd4 - ad) / abs(ad):.2e}"))
+    # dot test of one step
+    gg = torch.Generator(device="cpu").manual_seed(8)
+    dU = torch.zeros_like(U)
+    lam = torch.zeros_like(U)
+    dU[:, 3:-3, 3:-3, 3:-3] = 1e-6 * torch.randn((5, n, n, n), generator=gg, dtype=torch.float64).cuda()
+    lam[:, 3:-3, 3:-3, 3:-3] = torch.randn((5, n, n, n), generator=gg, dtype=torch.float64).cuda()
+    tl, ad = dU.clone(), lam.clone()
+    f.set_state(Up)
+    f.tangent_step(dts[0], [tl[q].data_ptr() for q in range(5)])
+    f.set_state(Up)
+    f.adjoint_step(dts[0], [ad[q].data_ptr() for q in range(5)])
+    torch.cuda.synchronize()
+    a, b = float((tl * lam).sum()), float((dU * ad).sum())
+    f.close()
+    return {
+        "workload": f"isotropic {n}^3, Ma_t=0.3, Re=100, WCNS5+HLLC, {nsteps} RK3 steps, J = mean 0.5 rho |u|^2",
+        "checkpoint": policy,
+        "J": float(J.detach()),
+        "forward_sweep_s": round(stats["forward_s"], 4),
+        "backward_sweep_s": round(stats["backward_s"], 4),
+        "sweep_cost_ratio": round(stats["backward_s"] / stats["forward_s"], 3),
+        "forward_step_ms": round(fwd_ms, 4),
+        "adjoint_step_ms": round(adj_ms, 4),
+        "step_cost_ratio": round(adj_ms / fwd_ms, 3),
+        "recomputed_steps": stats["recomputed_steps"],
+        "fd_eps": eps,
+        "fd_rel_err": checks,
+        "fd_max_rel_err": float(f"{worst:.3e}"),
+        "fd_max_err_over_grad_norm": float(f"{cs_worst:.3e}"),
+        "fd_rel_err_eps_1e-4_first2": wide,
+        "dot_test_rel_err": float(f"{abs(a - b) / abs(a):.3e}"),
     }
 
 
@@ -229,6 +345,8 @@ def main():
     }
     if args.mixed:
         out["mixed"] = mixed_report(args, prim, G, P, ms, timed_run)
+    if args.adjoint:
+        out["adjoint"] = adjoint_report(args)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prim, G, P, args.cpu_threads)
     print(json.dumps(out))

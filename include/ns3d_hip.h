@@ -235,8 +235,9 @@ int ns3d_get_state(ns3d_ctx *ctx, double *const *d_U);
  * runge_kutta_3 step with a fixed dt (range_kutta.cpp:46-92; the C2P/P2C
  * round trip is taken as the identity) at the context's current conserved
  * state.  Arrays are five device pointers (rho, rho u, rho v, rho w, E) in
- * the ghosted layout; only interior cells are read, outputs have zero
- * ghosts.  The non-smooth parts (min/max wave speeds, |.|, branches) use a
+ * the ghosted layout; only interior cells are read.  The RHS outputs and
+ * the returned adjoint have zero ghosts; tangent_step leaves the periodic
+ * images in the ghosts of its array.  The non-smooth parts (min/max wave speeds, |.|, branches) use a
  * fixed subgradient, the same in the tangent and the adjoint.
  *   tangent_rhs:  out = (dRHS/dU) dU           any scheme combination
  *   adjoint_rhs:  out = (dRHS/dU)^T lam        component-wise WCNS5 or
@@ -264,6 +265,12 @@ int ns3d_check_visc_tangent(const ns3d_params *params, const double *q, const do
 int ns3d_check_visc_adjoint(const ns3d_params *params, const double *q, const double *g, const double *Vb,
                             double *gb, double *qb);
 int ns3d_check_c2p_adjoint(const ns3d_params *params, const double *U, const double *qb, double *Ub);
+/* Reverse of both reconstructions of one 5-cell stencil f (cells c-2..c+2;
+ * left state of face c+1/2 and right state of face c-1/2) with shared
+ * smoothness indicators (fb_pair), and the same from two separate
+ * reconstructions (fb_single). */
+int ns3d_check_weno_pair_adjoint(const ns3d_params *params, const double *f, double ybL, double ybR,
+                                 double *fb_pair, double *fb_single);
 
 #ifdef __cplusplus
 }

@@ -106,6 +106,7 @@ _SIGS = {
     "ns3d_check_visc_tangent": [ctypes.POINTER(CParams), _DP, _DP, _DP, _DP, _DP, _DP],
     "ns3d_check_visc_adjoint": [ctypes.POINTER(CParams), _DP, _DP, _DP, _DP, _DP],
     "ns3d_check_c2p_adjoint": [ctypes.POINTER(CParams), _DP, _DP, _DP],
+    "ns3d_check_weno_pair_adjoint": [ctypes.POINTER(CParams), _DP, ctypes.c_double, ctypes.c_double, _DP, _DP],
     "ns3d_interpolate_batch": [ctypes.POINTER(CParams), _VP, ctypes.c_int, ctypes.c_double, _VP, _VP],
     "ns3d_riemann_batch": [ctypes.POINTER(CParams), _VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP],
 }

@@ -188,6 +188,26 @@ def mean_kinetic_energy(U: torch.Tensor) -> torch.Tensor:
     return (0.5 * (I[1] * I[1] + I[2] * I[2] + I[3] * I[3]) / I[0]).mean()
 
 
+def kinetic_energy_density(U: torch.Tensor) -> torch.Tensor:
+    """0.5 rho |u|^2 per interior cell."""
+    I = interior(U)
+    return 0.5 * (I[1] * I[1] + I[2] * I[2] + I[3] * I[3]) / I[0]
+
+
+def central_difference(field: DeviceField, U0: torch.Tensor, d: torch.Tensor, dts, eps: float = 1e-6,
+                       checkpoint: str = "binomial:0") -> torch.Tensor:
+    """(J(U0 + eps d) - J(U0 - eps d)) / (2 eps) for J = mean_kinetic_energy.
+
+    The difference is taken cell by cell before the mean: J itself is a sum
+    of O(1) terms whose rounding (~1e-16 J log N) would swamp the O(eps |dJ|)
+    difference on large boxes, while the cell-wise differences are small and
+    sum accurately."""
+    with torch.no_grad():
+        Up = rollout((U0 + eps * d).contiguous(), field, dts, checkpoint)
+        Um = rollout((U0 - eps * d).contiguous(), field, dts, checkpoint)
+        return (kinetic_energy_density(Up) - kinetic_energy_density(Um)).mean() / (2 * eps)
+
+
 def state_tensor(field: DeviceField) -> torch.Tensor:
     """The field's conserved state as a new (5, sz, sy, sx) CUDA tensor."""
     U = torch.empty((5,) + field.shape, dtype=torch.float64, device="cuda")

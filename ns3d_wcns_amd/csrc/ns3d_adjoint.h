@@ -70,6 +70,98 @@ NS3D_HD double weno_vjp(const double *f, double yb, double *fb) {
     return y;
 }
 
+// Both reconstructions of one cell from its 5-cell stencil f = cells
+// c-2..c+2, sharing the smoothness indicators: yL = weno(f0..f4) (value at
+// the right face) and yR = weno(f4..f0) (value at the left face); the same
+// function values as wcns5 / weno5js up to rounding.
+template <bool JS>
+NS3D_HD void weno_pair(const double *f, double &yL, double &yR) {
+    const double eps = 1e-6;
+    const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    const double t0 = f0 - 2.0 * f1 + f2, t1 = f1 - 2.0 * f2 + f3, t2 = f2 - 2.0 * f3 + f4;
+    const double s0 = f0 - 4.0 * f1 + 3.0 * f2, s1 = f1 - f3, s2 = 3.0 * f2 - 4.0 * f3 + f4;
+    const double b0 = (13.0 / 12.0) * t0 * t0 + 0.25 * s0 * s0;
+    const double b1 = (13.0 / 12.0) * t1 * t1 + 0.25 * s1 * s1;
+    const double b2 = (13.0 / 12.0) * t2 * t2 + 0.25 * s2 * s2;
+    const double q0 = (eps + b0) * (eps + b0), q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2);
+    const double d0 = JS ? 0.1 : 1.0 / 16.0, d1 = JS ? 0.6 : 5.0 / 8.0, d2 = JS ? 0.3 : 5.0 / 16.0;
+    const double K = JS ? 6.0 : 8.0;
+    const double c0 = JS ? 2.0 : 3.0, c1 = JS ? -7.0 : -10.0, c2 = JS ? 11.0 : 15.0;
+    const double m1 = JS ? 5.0 : 6.0, m2 = JS ? 2.0 : 3.0;
+    const double q12 = q1 * q2, q02 = q0 * q2, q01 = q0 * q1;
+    {
+        const double e0 = d0 * q12, e1 = d1 * q02, e2 = d2 * q01;
+        const double p0 = c0 * f0 + c1 * f1 + c2 * f2, p1 = -f1 + m1 * f2 + m2 * f3, p2 = m2 * f2 + m1 * f3 - f4;
+        yL = (e0 * p0 + e1 * p1 + e2 * p2) / (K * (e0 + e1 + e2));
+    }
+    {
+        const double e0 = d0 * q01, e1 = d1 * q02, e2 = d2 * q12;
+        const double p0 = c0 * f4 + c1 * f3 + c2 * f2, p1 = -f3 + m1 * f2 + m2 * f1, p2 = m2 * f2 + m1 * f1 - f0;
+        yR = (e0 * p0 + e1 * p1 + e2 * p2) / (K * (e0 + e1 + e2));
+    }
+}
+
+// Reverse mode of both reconstructions of one cell from its 5-cell stencil
+// f[0..4] = cells c-2..c+2: yL = weno(f0..f4) (the value at the right face,
+// the left state of face c+1/2) and yR = weno(f4..f0) (the value at the left
+// face, the right state of face c-1/2).  The two share their smoothness
+// indicators (the mirrored stencil swaps beta_0 and beta_2), which are
+// formed once.  fb += ybL dyL/df + ybR dyR/df.
+template <bool JS>
+NS3D_HD void weno_pair_vjp(const double *f, double ybL, double ybR, double *fb) {
+    const double eps = 1e-6;
+    const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    const double t[3] = {f0 - 2.0 * f1 + f2, f1 - 2.0 * f2 + f3, f2 - 2.0 * f3 + f4};
+    const double s[3] = {f0 - 4.0 * f1 + 3.0 * f2, f1 - f3, 3.0 * f2 - 4.0 * f3 + f4};
+    double b[3], q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b[k] = (13.0 / 12.0) * t[k] * t[k] + 0.25 * s[k] * s[k];
+        q[k] = (eps + b[k]) * (eps + b[k]);
+    }
+    const double d0 = JS ? 0.1 : 1.0 / 16.0, d1 = JS ? 0.6 : 5.0 / 8.0, d2 = JS ? 0.3 : 5.0 / 16.0;
+    const double K = JS ? 6.0 : 8.0;
+    const double c0 = JS ? 2.0 : 3.0, c1 = JS ? -7.0 : -10.0, c2 = JS ? 11.0 : 15.0;  // K x candidate 0
+    const double m1 = JS ? 5.0 : 6.0, m2 = JS ? 2.0 : 3.0;                            // candidate 1: -1, m1, m2
+    // left (right-face) reconstruction: weights on (q0, q1, q2), stencils f0..f4
+    const double eL[3] = {d0 * q[1] * q[2], d1 * q[0] * q[2], d2 * q[0] * q[1]};
+    const double pL[3] = {c0 * f0 + c1 * f1 + c2 * f2, -f1 + m1 * f2 + m2 * f3, m2 * f2 + m1 * f3 - f4};
+    // mirrored: beta'_0 = beta_2, beta'_2 = beta_0
+    const double eR[3] = {d0 * q[1] * q[0], d1 * q[2] * q[0], d2 * q[2] * q[1]};
+    const double pR[3] = {c0 * f4 + c1 * f3 + c2 * f2, -f3 + m1 * f2 + m2 * f1, m2 * f2 + m1 * f1 - f0};
+    const double DL = K * (eL[0] + eL[1] + eL[2]), DR = K * (eR[0] + eR[1] + eR[2]);
+    const double yL = (eL[0] * pL[0] + eL[1] * pL[1] + eL[2] * pL[2]) / DL;
+    const double yR = (eR[0] * pR[0] + eR[1] * pR[1] + eR[2] * pR[2]) / DR;
+    const double NbL = ybL / DL, DbL = -ybL * yL / DL;
+    const double NbR = ybR / DR, DbR = -ybR * yR / DR;
+    double ebL[3], ebR[3], pbL[3], pbR[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ebL[k] = NbL * pL[k] + K * DbL;
+        pbL[k] = NbL * eL[k];
+        ebR[k] = NbR * pR[k] + K * DbR;
+        pbR[k] = NbR * eR[k];
+    }
+    // q adjoints: left e = (d0 q1 q2, d1 q0 q2, d2 q0 q1); mirrored e = (d0 q1 q0, d1 q2 q0, d2 q2 q1)
+    const double qb0 = ebL[1] * d1 * q[2] + ebL[2] * d2 * q[1] + ebR[0] * d0 * q[1] + ebR[1] * d1 * q[2];
+    const double qb1 = ebL[0] * d0 * q[2] + ebL[2] * d2 * q[0] + ebR[0] * d0 * q[0] + ebR[2] * d2 * q[2];
+    const double qb2 = ebL[0] * d0 * q[1] + ebL[1] * d1 * q[0] + ebR[1] * d1 * q[0] + ebR[2] * d2 * q[1];
+    const double qb[3] = {qb0, qb1, qb2};
+    double tb[3], sb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double bb = qb[k] * 2.0 * (eps + b[k]);
+        tb[k] = bb * (13.0 / 6.0) * t[k];
+        sb[k] = bb * 0.5 * s[k];
+    }
+    fb[0] += tb[0] + sb[0] + pbL[0] * c0 - pbR[2];
+    fb[1] += -2.0 * tb[0] + tb[1] - 4.0 * sb[0] + sb[1] + pbL[0] * c1 - pbL[1] + pbR[1] * m2 + pbR[2] * m1;
+    fb[2] += tb[0] - 2.0 * tb[1] + tb[2] + 3.0 * sb[0] + 3.0 * sb[2] + (pbL[0] + pbR[0]) * c2 +
+             (pbL[1] + pbR[1]) * m1 + (pbL[2] + pbR[2]) * m2;
+    fb[3] += tb[1] - 2.0 * tb[2] - sb[1] - 4.0 * sb[2] + pbL[1] * m2 + pbL[2] * m1 + pbR[0] * c1 - pbR[1];
+    fb[4] += tb[2] + sb[2] - pbL[2] + pbR[0] * c0;
+}
+
 // ------------------------------------------------------------ Riemann flux
 // Adjoints of the Prim fields of one side.
 struct PrimBar {

@@ -10,15 +10,18 @@
 //   k_tl_assemble tangent RHS: half-node differences + viscous divergence.
 // Adjoint (hand-written reverse mode, ns3d_adjoint.h; component-wise WCNS5 /
 // WENO5-JS with HLLC / Rusanov):
-//   k_adj_face    per face of one direction: the face adjoint from the
-//                 transposed half-node difference of lambda, then the
-//                 reverse of Riemann flux and interpolation, stored as the
-//                 contributions to the face's six stencil cells;
-//   k_adj_gather  per cell: sum of the contributions of its six faces;
-//   k_adj_visc_a  per cell: viscous flux adjoints from the transposed
-//                 divergence, reverse of stresses / heat flux -> gradient
-//                 and (u, v, w, T) adjoints;
-//   k_adj_visc_b  per cell: transposed gradient stencil and reverse of the
+//   k_adj_visc_a  per cell: face adjoints of the three directions (the
+//                 transposed half-node difference of lambda), viscous flux
+//                 adjoints from the transposed divergence, reverse of
+//                 stresses / heat flux -> gradient and (u, v, w, T) adjoints;
+//   per direction, marching along the direction (lanes across it):
+//   k_adj_riemann both reconstructions of each cell (shared smoothness
+//                 indicators) and the reverse of the Riemann flux of each
+//                 face from the face adjoint -> left / right state adjoints;
+//   k_adj_weno    reverse of each cell's two reconstructions, accumulated in
+//                 a register ring of its five stencil cells and added to the
+//                 result once complete; the last direction also adds the
+//                 transposed gradient stencil and the reverse of the
 //                 conserved -> primitive map.
 // All adjoint accumulation is by gathers (no atomics): results are
 // deterministic.
@@ -184,58 +187,256 @@ __global__ void __launch_bounds__(256) k_tl_assemble(AdjArgs a) {
 }
 
 // ------------------------------------------------------------ adjoint
-template <int DIR, int INTERP, int RS>
-__global__ void __launch_bounds__(256) k_adj_face(AdjArgs a) {
-    int i, j, k;
-    if (!cell_of(a.b, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
-    const Box &b = a.b;
-    const long st = gstride(b, DIR);
-    const long g0 = b.gidx(i, j, k);  // face g0 + 1/2 along DIR
-    const int var[5] = {0, P1, P2, P3, 4};
-    // face adjoint: Fb[j] = -(1/h) sum_m kW[m] lambda[j + 3 - m]
-    double Fb[5];
-#pragma unroll
-    for (int n = 0; n < 5; ++n) {
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) s += kW[m] * a.X[var[n]][g0 + (3 - m) * st];
-        Fb[n] = -s * a.ih[DIR];
-    }
-    double s[5][6], sb[5][6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        const long g = g0 + (m - 2) * st;
-#pragma unroll
-        for (int n = 0; n < 5; ++n) {
-            s[n][m] = a.U[var[n]][g];
-            sb[n][m] = 0.0;
-        }
-    }
-    face_vjp<INTERP, RS>(a.c, s, Fb, sb);
-    const long o = b.oidx(i, j, k);
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-        for (int n = 0; n < 5; ++n) a.C[m * 5 + var[n]][o] = sb[n][m];
-}
+// Column marching along DIR: lanes run along LA (x, or y for DIR = x), block
+// rows along OA, segments of ADJ_SEG cells along DIR in grid z.
+constexpr int ADJ_SEG = 16;
 
 template <int DIR>
-__global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
-    int i, j, k;
-    if (!cell_of(a.b, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+struct March {
+    static constexpr int LA = DIR == 0 ? 1 : 0;
+    static constexpr int OA = DIR == 2 ? 1 : 2;
+    int N, c0, c1;
+    long gbase, gst, obase, ost;
+    bool ok;
+    __device__ March(const Box &b) {
+        const int n[3] = {b.nx, b.ny, b.nz};
+        int ijk[3];
+        ijk[LA] = blockIdx.x * 64 + threadIdx.x;
+        ijk[OA] = blockIdx.y * 4 + threadIdx.y;
+        ijk[DIR] = 0;
+        ok = ijk[LA] < n[LA] && ijk[OA] < n[OA];
+        if (!ok) ijk[LA] = ijk[OA] = 0;
+        N = n[DIR];
+        c0 = blockIdx.z * ADJ_SEG;
+        c1 = min(c0 + ADJ_SEG, N);
+        gbase = b.gidx(ijk[0], ijk[1], ijk[2]);
+        obase = b.oidx(ijk[0], ijk[1], ijk[2]);
+        gst = DIR == 0 ? 1L : (DIR == 1 ? (long)b.sx : (long)b.sx * b.sy);
+        ost = DIR == 0 ? 1L : (DIR == 1 ? (long)b.nx : (long)b.nx * b.ny);
+    }
+    __device__ long g(int c) const { return gbase + (long)wrap(c, N) * gst; }
+    __device__ long o(int c) const { return obase + (long)wrap(c, N) * ost; }
+};
+
+template <int DIR>
+dim3 march_grid(const Box &b) {
+    const int n[3] = {b.nx, b.ny, b.nz};
+    constexpr int LA = March<DIR>::LA, OA = March<DIR>::OA;
+    return dim3((n[LA] + 63) / 64, (n[OA] + 3) / 4, (n[DIR] + ADJ_SEG - 1) / ADJ_SEG);
+}
+
+// Per face of DIR (marching): both reconstructions of every cell (shared
+// smoothness indicators), then the reverse of the Riemann flux of face
+// c-1/2 from the left state of cell c-1 and the right state of cell c, with
+// the face adjoint FB[DIR] -> left / right state adjoints SB[0..4], SB[5..9].
+template <int DIR, int INTERP, int RS>
+__global__ void __launch_bounds__(256) k_adj_riemann(AdjArgs a) {
+    const March<DIR> M(a.b);
+    if (!M.ok) return;
+    constexpr bool JS = INTERP == I_WENO5_JS;
+    constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
+    const int var[5] = {0, P1, P2, P3, 4};
+    double w[5][5];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const long g = M.g(M.c0 - 2 + m);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) w[n][m] = a.U[var[n]][g];
+    }
+    double ULp[5];
+    for (int c = M.c0; c <= M.c1; ++c) {
+        {
+            const long g = M.g(c + 2);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) w[n][4] = a.U[var[n]][g];
+        }
+        double UL[5], UR[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) weno_pair<JS>(w[n], UL[n], UR[n]);
+        if (c > M.c0) {  // face (c-1) + 1/2
+            const long of = M.o(c - 1);
+            double Fb[5], ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < 5; ++n) Fb[n] = a.FB[DIR][var[n]][of];
+            if constexpr (RS == R_HLLC) hllc_vjp(a.c, ULp, UR, Fb, ULb, URb);
+            else rusanov_vjp(a.c, ULp, UR, Fb, ULb, URb);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                a.SB[var[n]][of] = ULb[n];
+                a.SB[5 + var[n]][of] = URb[n];
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            ULp[n] = UL[n];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) w[n][m] = w[n][m + 1];
+        }
+    }
+}
+
+// Per cell along DIR (marching): reverse of the cell's two reconstructions
+// (shared indicators), accumulated in a ring of the five stencil cells; a
+// cell is complete once the reconstructions of its four neighbours are in,
+// and is added to out (FIRST: written).  The ring lives in LDS, one private
+// column of 25 doubles per thread (slot of cell x: x mod 5, uniform over the
+// block), which keeps the registers to the stencil window and the reverse.
+template <int DIR, bool JS, bool FIRST>
+__global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
+    __shared__ double ring[5][5][256];  // [cell mod 5][var][thread]
+    const March<DIR> M(a.b);
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    if (!M.ok) return;  // no barrier below: each thread owns its ring column
     const Box &b = a.b;
-    const long g = b.gidx(i, j, k);
+    double w[5][5];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const long g = M.g(M.c0 - 4 + m);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) w[n][m] = a.U[n][g];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) ring[m][n][tid] = 0.0;
+    // cell c's reconstructions reach cells c-2..c+2
+    for (int c = M.c0 - 2; c <= M.c1 + 1; ++c) {
+        {
+            const long g = M.g(c + 2);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) w[n][4] = a.U[n][g];
+        }
+        const long oc = M.o(c), om = M.o(c - 1);
+        const int base = c + 5 * ADJ_SEG;  // non-negative for the mod
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            double fb[5] = {0, 0, 0, 0, 0};
+            weno_pair_vjp<JS>(w[n], a.SB[n][oc], a.SB[5 + n][om], fb);
+            // only cells c0..c1-1 are accumulated: the slots of cells below
+            // c0 are those of c0 + 1.. (uniform condition over the block)
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                if (c - 2 + m >= M.c0 && c - 2 + m < M.c1) ring[(base - 2 + m) % 5][n][tid] += fb[m];
+        }
+        const int done = c - 2;  // all of its five contributors are in
+        if (done >= M.c0) {
+            const int slot = (base - 2) % 5;
+            const long g = M.g(done);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                a.out[n][g] = (FIRST ? 0.0 : a.out[n][g]) + ring[slot][n][tid];
+                ring[slot][n][tid] = 0.0;  // reused by cell done + 5
+            }
+        }
+        if (done + 1 >= M.c1) break;  // cells c0..c1-1 are out
+#pragma unroll
+        for (int n = 0; n < 5; ++n)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) w[n][m] = w[n][m + 1];
+    }
+}
+
+// The x direction, one block per x line (threads = cells, nx <= 256): the
+// line and its 2-cell periodic halo in LDS; both reconstructions of every
+// cell; the reverse of the Riemann flux of every face (state adjoints
+// written over the states they replace: each slot is read and written by
+// one thread); per variable the reverse of the reconstructions into LDS and
+// the gather of the five contributions of each cell; then the viscous part
+// (transposed gradient stencil and the reverse of the conserved ->
+// primitive map), added to out.
+template <bool JS, int RS>
+__global__ void __launch_bounds__(256) k_adj_xline(AdjArgs a) {
+    extern __shared__ double lds[];
+    const Box &b = a.b;
+    const int N = b.nx, W = N + 4;
+    const int t = threadIdx.x;
+    const int j = blockIdx.x % b.ny, k = blockIdx.x / b.ny;
+    double *Ul = lds;           // [5][W]: cells -2..N+1
+    double *A = Ul + 5 * W;     // [10][N]: left / right states, then their adjoints
+    double *Cc = A + 10 * N;    // [5][N]: contributions of one variable
+    const long row = b.gidx(0, j, k) - NG;  // ghosted index of x = -NG
+    for (int x = t; x < W; x += blockDim.x) {
+        const long g = row + NG + wrap(x - 2, N);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) Ul[n * W + x] = a.U[n][g];
+    }
+    __syncthreads();
+    const bool own = t < N;
+    if (own) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            const double f[5] = {Ul[n * W + t], Ul[n * W + t + 1], Ul[n * W + t + 2], Ul[n * W + t + 3],
+                                 Ul[n * W + t + 4]};
+            weno_pair<JS>(f, A[n * N + t], A[(5 + n) * N + t]);
+        }
+    }
+    __syncthreads();
+    const long o = b.oidx(0, j, k) + t;
+    if (own) {  // face t + 1/2: left state of cell t, right state of cell t + 1
+        const int tp = t + 1 < N ? t + 1 : 0;
+        double UL[5], UR[5], Fb[5], ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            UL[n] = A[n * N + t];
+            UR[n] = A[(5 + n) * N + tp];
+            Fb[n] = a.FB[0][n][o];
+        }
+        if constexpr (RS == R_HLLC) hllc_vjp(a.c, UL, UR, Fb, ULb, URb);
+        else rusanov_vjp(a.c, UL, UR, Fb, ULb, URb);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            A[n * N + t] = ULb[n];          // adjoint of the left state of face t + 1/2
+            A[(5 + n) * N + tp] = URb[n];   // slot tp: adjoint of the right state of face tp - 1/2
+        }
+    }
+    __syncthreads();
     double acc[5];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) acc[n] = a.out[n][g];
-    // cell p is stencil slot m of face p + 2 - m
+    for (int v = 0; v < 5; ++v) {
+        if (own) {
+            const double f[5] = {Ul[v * W + t], Ul[v * W + t + 1], Ul[v * W + t + 2], Ul[v * W + t + 3],
+                                 Ul[v * W + t + 4]};
+            double fb[5] = {0, 0, 0, 0, 0};
+            weno_pair_vjp<JS>(f, A[v * N + t], A[(5 + v) * N + t], fb);
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        const long o = owrap(b, i, j, k, DIR, 2 - m);
+            for (int m = 0; m < 5; ++m) Cc[m * N + t] = fb[m];
+        }
+        __syncthreads();
+        if (own) {
+            double sum = 0.0;
 #pragma unroll
-        for (int n = 0; n < 5; ++n) acc[n] += a.C[m * 5 + n][o];
+            for (int m = 0; m < 5; ++m) sum += Cc[m * N + wrap(t + 2 - m, N)];
+            acc[v] = sum;
+        }
+        __syncthreads();  // Cc is rewritten by the next variable
     }
+    if (!own) return;
+    const int i = t;
+    const long g = b.gidx(i, j, k);
+    double qb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) qb[q] = a.qb[q][o];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        long vo[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) vo[m] = owrap(b, i, j, k, d, m - 3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double *G = a.Gb[3 * q + d];
+            double sum = 0.0;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) sum += kD[m] * G[vo[m]];
+            qb[q] -= sum * a.i60[d];
+        }
+    }
+    double U[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        U[n] = a.U[n][g];
+        acc[n] += a.out[n][g];
+    }
+    c2p_vjp(a.gamma, a.Rgas, U, qb, acc);
 #pragma unroll
     for (int n = 0; n < 5; ++n) a.out[n][g] = acc[n];
 }
@@ -245,6 +446,7 @@ __global__ void __launch_bounds__(256) k_adj_visc_a(AdjArgs a) {
     if (!cell_of(a.b, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const Box &b = a.b;
     const long c = b.gidx(i, j, k);
+    const long o = b.oidx(i, j, k);
     const double *Qv[4] = {a.Q[0], a.Q[1], a.Q[2], a.Q[4]};
     double g[4][3], lg[4][3];
 #pragma unroll
@@ -254,8 +456,19 @@ __global__ void __launch_bounds__(256) k_adj_visc_a(AdjArgs a) {
         for (int q = 0; q < 4; ++q) {
             const double *f = Qv[q];
             g[q][d] = d6c(f[c - 3 * st], f[c - 2 * st], f[c - st], f[c + st], f[c + 2 * st], f[c + 3 * st], a.i60[d]);
-            const double *l = a.X[q + 1];  // lambda of rho u, rho v, rho w, E
-            lg[q][d] = d6c(l[c - 3 * st], l[c - 2 * st], l[c - st], l[c + st], l[c + 2 * st], l[c + 3 * st], a.i60[d]);
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            const double *l = a.X[n];
+            double lv[7];
+#pragma unroll
+            for (int m = 0; m < 7; ++m) lv[m] = l[c + (m - 3) * st];
+            // face adjoint of face c + 1/2: Fb = -(1/h) sum_m kW[m] lambda[c + 3 - m]
+            double sum = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) sum += kW[m] * lv[6 - m];
+            a.FB[d][n][o] = -sum * a.ih[d];
+            if (n > 0) lg[n - 1][d] = d6c(lv[0], lv[1], lv[2], lv[4], lv[5], lv[6], a.i60[d]);
         }
     }
     // transposed divergence (D6 is antisymmetric on the periodic box)
@@ -271,48 +484,12 @@ __global__ void __launch_bounds__(256) k_adj_visc_a(AdjArgs a) {
     Vb[VEZ] = -lg[3][2];
     double gb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, qb[4] = {0, 0, 0, 0};
     visc_local_vjp(a.mu, a.S_ref, a.cp_pr, a.Q[0][c], a.Q[1][c], a.Q[2][c], a.Q[4][c], g, Vb, gb, qb);
-    const long o = b.oidx(i, j, k);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) a.Gb[3 * q + d][o] = gb[q][d];
         a.qb[q][o] = qb[q];
     }
-}
-
-__global__ void __launch_bounds__(256) k_adj_visc_b(AdjArgs a) {
-    int i, j, k;
-    if (!cell_of(a.b, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    const Box &b = a.b;
-    const long o = b.oidx(i, j, k);
-    double qb[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) qb[q] = a.qb[q][o];
-    // transposed gradient stencil: qb -= D6_d(Gb[q][d])
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        long vo[7];
-#pragma unroll
-        for (int m = 0; m < 7; ++m) vo[m] = owrap(b, i, j, k, d, m - 3);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double *G = a.Gb[3 * q + d];
-            double s = 0.0;
-#pragma unroll
-            for (int m = 0; m < 7; ++m) s += kD[m] * G[vo[m]];
-            qb[q] -= s * a.i60[d];
-        }
-    }
-    const long g = b.gidx(i, j, k);
-    double U[5], Ub[5];
-#pragma unroll
-    for (int n = 0; n < 5; ++n) {
-        U[n] = a.U[n][g];
-        Ub[n] = a.out[n][g];
-    }
-    c2p_vjp(a.gamma, a.Rgas, U, qb, Ub);
-#pragma unroll
-    for (int n = 0; n < 5; ++n) a.out[n][g] = Ub[n];
 }
 
 // out = alpha A + beta B + gamma P on the interior cells; a null operand
@@ -364,19 +541,34 @@ void tl_face_i(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s) {
 }
 
 template <int DIR>
-void adj_face_d(const AdjArgs &a, int interp, int rs, hipStream_t s) {
+void adj_dir(const AdjArgs &a, int interp, int rs, hipStream_t s) {
     const bool js = interp == I_WENO5_JS, rus = rs == R_RUSANOV;
-    if (js && rus) hipLaunchKernelGGL((k_adj_face<DIR, I_WENO5_JS, R_RUSANOV>), cgrid(a.b), dim3(256), 0, s, a);
-    else if (js) hipLaunchKernelGGL((k_adj_face<DIR, I_WENO5_JS, R_HLLC>), cgrid(a.b), dim3(256), 0, s, a);
-    else if (rus) hipLaunchKernelGGL((k_adj_face<DIR, I_WENO5, R_RUSANOV>), cgrid(a.b), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_adj_face<DIR, I_WENO5, R_HLLC>), cgrid(a.b), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((k_adj_gather<DIR>), cgrid(a.b), dim3(256), 0, s, a);
+    const dim3 g = march_grid<DIR>(a.b), bl(64, 4);
+    if (js && rus) hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5_JS, R_RUSANOV>), g, bl, 0, s, a);
+    else if (js) hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5_JS, R_HLLC>), g, bl, 0, s, a);
+    else if (rus) hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5, R_RUSANOV>), g, bl, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5, R_HLLC>), g, bl, 0, s, a);
+    if (js) hipLaunchKernelGGL((k_adj_weno<DIR, true, DIR == 1>), g, bl, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_weno<DIR, false, DIR == 1>), g, bl, 0, s, a);
+}
+
+void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s) {
+    const bool js = interp == I_WENO5_JS, rus = rs == R_RUSANOV;
+    const int N = a.b.nx;
+    const dim3 g((unsigned)((long)a.b.ny * a.b.nz)), bl((unsigned)(((N + 63) / 64) * 64));
+    const size_t shm = (size_t)(5 * (N + 4) + 15 * N) * sizeof(double);
+    if (js && rus) hipLaunchKernelGGL((k_adj_xline<true, R_RUSANOV>), g, bl, shm, s, a);
+    else if (js) hipLaunchKernelGGL((k_adj_xline<true, R_HLLC>), g, bl, shm, s, a);
+    else if (rus) hipLaunchKernelGGL((k_adj_xline<false, R_RUSANOV>), g, bl, shm, s, a);
+    else hipLaunchKernelGGL((k_adj_xline<false, R_HLLC>), g, bl, shm, s, a);
 }
 }  // namespace
 
 bool adjoint_supported(int interp, int rs, bool chr) {
     return !chr && (interp == I_WENO5 || interp == I_WENO5_JS) && (rs == R_HLLC || rs == R_RUSANOV);
 }
+
+int adjoint_max_nx() { return 256; }  // 5 (nx + 4) + 15 nx doubles of LDS per block
 
 void launch_tl_prim(const AdjArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_tl_prim, cgrid(a.b), dim3(256), 0, s, a);
@@ -391,12 +583,13 @@ void launch_tl_rhs(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s
 }
 
 void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s) {
-    // out is zeroed by the caller; the three directions accumulate in turn
-    adj_face_d<0>(a, interp, rs, s);
-    adj_face_d<1>(a, interp, rs, s);
-    adj_face_d<2>(a, interp, rs, s);
+    // the face and viscous gradient adjoints first, then the y and z
+    // directions (y writes the interior of out, z accumulates), then the x
+    // direction with the viscous part (ghosts of out are not written)
     hipLaunchKernelGGL(k_adj_visc_a, cgrid(a.b), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_adj_visc_b, cgrid(a.b), dim3(256), 0, s, a);
+    adj_dir<1>(a, interp, rs, s);
+    adj_dir<2>(a, interp, rs, s);
+    adj_x(a, interp, rs, s);
 }
 
 void launch_lincomb(const LinComb &l, hipStream_t s) {

@@ -42,7 +42,7 @@ struct ns3d_ctx {
     double *dq[4];   // ghosted (u, v, w, T) tangents
     double *s1[5];   // stage-1 state of the step being differentiated
     double *a0[5], *a1[5], *pr[5];  // stage accumulators and the RHS derivative
-    double *cw[30];  // compact: face contributions (adjoint) / face and viscous tangents
+    double *cw[25];  // compact: face adjoints and state adjoints, or face and viscous tangents
     double *gb[12], *qb[4];
     double *partial;
     double *scal;  // [0] dt, [1] reduction result
@@ -407,7 +407,7 @@ int ensure_adj(ns3d_ctx *c) {
     if (c->adj_slab) return NS3D_OK;
     const size_t cs = field_stride(c->ncell), is = field_stride(c->nint);
     constexpr size_t kGhosted = 5 + 4 + 5 + 5 + 5 + 5;  // xg, dq, s1, a0, a1, pr
-    constexpr size_t kCompact = 30 + 12 + 4;            // cw, gb, qb
+    constexpr size_t kCompact = 25 + 12 + 4;            // cw, gb, qb
     const size_t total = kGhosted * cs + kCompact * is;
     if (hipMalloc(&c->adj_slab, total * sizeof(double)) != hipSuccess) {
         c->adj_slab = nullptr;
@@ -421,7 +421,7 @@ int ensure_adj(ns3d_ctx *c) {
     for (int n = 0; n < 5; ++n, p += cs) c->a0[n] = p;
     for (int n = 0; n < 5; ++n, p += cs) c->a1[n] = p;
     for (int n = 0; n < 5; ++n, p += cs) c->pr[n] = p;
-    for (int n = 0; n < 30; ++n, p += is) c->cw[n] = p;
+    for (int n = 0; n < 25; ++n, p += is) c->cw[n] = p;
     for (int n = 0; n < 12; ++n, p += is) c->gb[n] = p;
     for (int n = 0; n < 4; ++n, p += is) c->qb[n] = p;
     if ((size_t)(p - c->adj_slab) != total) {  // the carve must match the allocation
@@ -469,7 +469,9 @@ AdjArgs adj_args(ns3d_ctx *c, double *const *out) {
         a.X[n] = c->xg[n];
         a.out[n] = out[n];
     }
-    for (int n = 0; n < 30; ++n) a.C[n] = c->cw[n];
+    for (int d = 0; d < 3; ++d)
+        for (int n = 0; n < 5; ++n) a.FB[d][n] = c->cw[5 * d + n];
+    for (int n = 0; n < 10; ++n) a.SB[n] = c->cw[15 + n];
     for (int n = 0; n < 12; ++n) a.Gb[n] = c->gb[n];
     for (int n = 0; n < 4; ++n) {
         a.qb[n] = c->qb[n];
@@ -493,11 +495,12 @@ AdjArgs adj_args(ns3d_ctx *c, double *const *out) {
     return a;
 }
 
-// Tangent RHS at the current state of the tangent in c->xg (ghosts filled).
-int tangent_rhs_impl(ns3d_ctx *c, double *const *out) {
+// Tangent RHS (interior of out) at the current state, of the tangent X
+// (ghosts filled).
+int tangent_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
     int rc;
-    if ((rc = zero_fields(c, out, 5))) return rc;
-    const AdjArgs a = adj_args(c, out);
+    AdjArgs a = adj_args(c, out);
+    for (int n = 0; n < 5; ++n) a.X[n] = X[n];
     launch_tl_prim(a, c->stream);
     NS3D_LAUNCHED("tangent primitives");
     if ((rc = fill_ghosts(c, c->dq, 4))) return rc;
@@ -506,15 +509,17 @@ int tangent_rhs_impl(ns3d_ctx *c, double *const *out) {
     return NS3D_OK;
 }
 
-// Adjoint RHS at the current state of the adjoint in c->xg (ghosts filled).
-int adjoint_rhs_impl(ns3d_ctx *c, double *const *out) {
-    int rc;
-    if ((rc = zero_fields(c, out, 5))) return rc;
-    launch_adj_rhs(adj_args(c, out), c->P.interpolation, c->P.riemann_solver, c->stream);
+// Adjoint RHS (interior of out) at the current state, of the adjoint X
+// (ghosts filled).
+int adjoint_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
+    AdjArgs a = adj_args(c, out);
+    for (int n = 0; n < 5; ++n) a.X[n] = X[n];
+    launch_adj_rhs(a, c->P.interpolation, c->P.riemann_solver, c->stream);
     NS3D_LAUNCHED("adjoint rhs");
     return NS3D_OK;
 }
 
+// A caller's input into the workspace copy with filled ghosts.
 int load_input(ns3d_ctx *c, const double *const *in) {
     int rc;
     if ((rc = copy_fields(c, c->xg, in, 5))) return rc;
@@ -547,6 +552,9 @@ int check_ptrs(const void *const *p, int n) {
 }
 
 int check_adjoint(ns3d_ctx *c) {
+    if (c->b.nx > adjoint_max_nx())
+        return fail(NS3D_E_UNSUPPORTED, "the adjoint holds one x line per block: nx <= %d (got %d)",
+                    adjoint_max_nx(), c->b.nx);
     if (!adjoint_supported(c->P.interpolation, c->P.riemann_solver, c->P.char_recon != 0))
         return fail(NS3D_E_UNSUPPORTED,
                     "the adjoint covers component-wise WCNS5 / WENO5-JS interpolation with HLLC or Rusanov flux "
@@ -595,7 +603,8 @@ int ns3d_tangent_rhs(ns3d_ctx *c, const double *const *dU, double *const *out) {
     int rc;
     if ((rc = check_ptrs((const void *const *)dU, 5)) || (rc = check_ptrs((const void *const *)out, 5))) return rc;
     if ((rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = load_input(c, dU))) return rc;
-    return tangent_rhs_impl(c, out);
+    if ((rc = zero_fields(c, out, 5))) return rc;  // zero ghosts
+    return tangent_rhs_impl(c, c->xg, out);
 }
 
 int ns3d_adjoint_rhs(ns3d_ctx *c, const double *const *lam, double *const *out) {
@@ -604,7 +613,8 @@ int ns3d_adjoint_rhs(ns3d_ctx *c, const double *const *lam, double *const *out) 
     if ((rc = check_ptrs((const void *const *)lam, 5)) || (rc = check_ptrs((const void *const *)out, 5))) return rc;
     if ((rc = check_adjoint(c)) || (rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = load_input(c, lam)))
         return rc;
-    return adjoint_rhs_impl(c, out);
+    if ((rc = zero_fields(c, out, 5))) return rc;  // zero ghosts
+    return adjoint_rhs_impl(c, c->xg, out);
 }
 
 int ns3d_tangent_step(ns3d_ctx *c, double dt, double *const *dU) {
@@ -615,7 +625,8 @@ int ns3d_tangent_step(ns3d_ctx *c, double dt, double *const *dU) {
     if ((rc = copy_fields(c, c->a0, dU, 5))) return rc;  // delta U^n
     static const double A[3] = {1.0, 0.75, 1.0 / 3.0}, B[3] = {0.0, 0.25, 2.0 / 3.0};
     for (int s = 0; s < 3; ++s) {
-        if ((rc = load_input(c, dU)) || (rc = tangent_rhs_impl(c, c->pr))) return rc;
+        // the tangent's ghosts are filled in place (interior values unchanged)
+        if ((rc = fill_ghosts(c, dU, 5)) || (rc = tangent_rhs_impl(c, dU, c->pr))) return rc;
         // delta U^(s+1) = A delta U^n + B delta U^(s) + B dt P  (stage 1: delta U^n + dt P)
         const double g = s == 0 ? dt : B[s] * dt;
         if ((rc = lincomb(c, dU, A[s], c->a0, B[s], s == 0 ? nullptr : dU, g, c->pr))) return rc;
@@ -633,20 +644,21 @@ int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
     if ((rc = fused_stage(c, 1, nullptr, dt)) || (rc = copy_fields(c, c->s1, conserved(c), 5)) ||
         (rc = fused_stage(c, 2, nullptr, dt)))
         return rc;
-    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3
-    if ((rc = load_input(c, adj)) || (rc = adjoint_rhs_impl(c, c->pr))) return rc;
+    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3 (the adjoint's
+    // ghosts are filled in place; its ghosts are zeroed on return)
+    if ((rc = fill_ghosts(c, adj, 5)) || (rc = adjoint_rhs_impl(c, adj, c->pr))) return rc;
     if ((rc = lincomb(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr)) ||
         (rc = lincomb(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr)))
         return rc;
     // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
     if ((rc = copy_fields(c, conserved(c), c->s1, 5)) || (rc = refresh_state(c))) return rc;
-    if ((rc = load_input(c, c->a1)) || (rc = adjoint_rhs_impl(c, c->pr))) return rc;
+    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
     if ((rc = lincomb(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr)) ||
         (rc = lincomb(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr)))
         return rc;
     // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P
     if ((rc = copy_fields(c, conserved(c), recorded(c), 5)) || (rc = refresh_state(c))) return rc;
-    if ((rc = load_input(c, c->a1)) || (rc = adjoint_rhs_impl(c, c->pr))) return rc;
+    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
     if ((rc = zero_fields(c, adj, 5))) return rc;
     return lincomb(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
 }

@@ -141,3 +141,24 @@ extern "C" int ns3d_check_c2p_adjoint(const ns3d_params *p, const double *U, con
     c2p_vjp(p->gamma, p->Rgas, U, qb, Ub);
     return NS3D_OK;
 }
+
+extern "C" int ns3d_check_weno_pair_adjoint(const ns3d_params *p, const double *f, double ybL, double ybR,
+                                            double *fb_pair, double *fb_single) {
+    if (!p || !f || !fb_pair || !fb_single) return NS3D_E_ARG;
+    if (p->interpolation != I_WENO5 && p->interpolation != I_WENO5_JS) return NS3D_E_UNSUPPORTED;
+    const bool js = p->interpolation == I_WENO5_JS;
+    for (int m = 0; m < 5; ++m) fb_pair[m] = fb_single[m] = 0.0;
+    if (js) weno_pair_vjp<true>(f, ybL, ybR, fb_pair);
+    else weno_pair_vjp<false>(f, ybL, ybR, fb_pair);
+    const double g[5] = {f[4], f[3], f[2], f[1], f[0]};
+    double gb[5] = {0, 0, 0, 0, 0};
+    if (js) {
+        weno_vjp<true>(f, ybL, fb_single);
+        weno_vjp<true>(g, ybR, gb);
+    } else {
+        weno_vjp<false>(f, ybL, fb_single);
+        weno_vjp<false>(g, ybR, gb);
+    }
+    for (int m = 0; m < 5; ++m) fb_single[m] += gb[4 - m];
+    return NS3D_OK;
+}

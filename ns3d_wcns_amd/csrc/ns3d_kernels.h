@@ -74,7 +74,8 @@ struct AdjArgs {
     const double *Q[5];   // its primitives u, v, w, p, T
     const double *X[5];   // tangent dU (tangent) or lambda (adjoint)
     double *out[5];       // tangent RHS / adjoint (interior written)
-    double *C[30];        // adjoint: face contributions, slot m * 5 + var
+    double *FB[3][5];     // adjoint: face adjoints per direction
+    double *SB[10];       // adjoint: left / right state adjoints of one direction's faces
     double *Gb[12];       // adjoint: gradient adjoints
     double *qb[4];        // adjoint: (u, v, w, T) adjoints
     double *dQ[4];        // tangent: ghosted (u, v, w, T) tangents
@@ -95,6 +96,7 @@ struct LinComb {
 };
 
 bool adjoint_supported(int interp, int rs, bool chr);
+int adjoint_max_nx();  // the x-line adjoint kernel holds one line per block
 void launch_tl_prim(const AdjArgs &a, hipStream_t s);
 void launch_tl_rhs(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s);
 void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s);

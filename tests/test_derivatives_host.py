@@ -170,3 +170,21 @@ def test_c2p_adjoint_matches_central_differences():
         h = 1e-6
         ref = np.array([(q_of(U + h * e) - q_of(U - h * e)) @ qb / (2 * h) for e in np.eye(5)])
         assert np.max(np.abs(Ub - ref)) <= 1e-6 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("interp", [Interpolation.WENO5, Interpolation.WENO5_JS], ids=lambda x: x.name)
+def test_paired_reconstruction_adjoint(interp):
+    """The cell-wise reverse of both reconstructions of a stencil (shared
+    smoothness indicators, used by the adjoint kernels) equals the sum of the
+    two separate reverses, on smooth and on discontinuous stencils."""
+    lib = load_library()
+    cp = params(0.2, interp).to_c()
+    rng = np.random.Generator(np.random.PCG64(31))
+    for trial in range(20):
+        f = np.cumsum(rng.normal(size=5)) if trial % 2 else 1.0 + 0.01 * rng.normal(size=5)
+        if trial % 5 == 0:
+            f[3:] += 5.0  # a jump
+        a, b = np.zeros(5), np.zeros(5)
+        ybL, ybR = rng.normal(size=2)
+        assert lib.ns3d_check_weno_pair_adjoint(ctypes.byref(cp), _p(f), ybL, ybR, _p(a), _p(b)) == 0
+        assert np.max(np.abs(a - b)) <= 1e-13 * max(1.0, np.max(np.abs(b))), (a, b)
