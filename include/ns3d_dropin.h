@@ -1,0 +1,155 @@
+/* ns3d_dropin.h -- single-GPU drop-in for the reference's hot-path call
+ * surface (C++).
+ *
+ * The types and free functions below carry the names, members and argument
+ * lists of the reference's field_structures.h / ns3d_func.h for the RHS +
+ * RK3 path (GridDesc, LocalDesc, SolverParams, Field3D, runge_kutta_3,
+ * compute_rhs, compute_timestep, ...), so that a time-advance loop written
+ * against them (maincycle.cpp:7-113) compiles and runs unchanged on one GPU
+ * of one periodic box.  The implementation (libns3d_dropin.so) keeps a
+ * device mirror of each Field3D in libns3d_hip (include/ns3d_hip.h).
+ *
+ * Coherence.  By default every hot-path function uploads the host fields it
+ * reads and downloads the fields the reference writes (faithful mode: the
+ * host vectors are the state between calls).  ns3d_set_resident(F, true)
+ * keeps the state on the device instead: runge_kutta_3, compute_timestep,
+ * compute_rhs and the Field3D members act on the mirror, and the host
+ * vectors are refreshed by ns3d_sync_host(F) (before diagnostics or output)
+ * and pushed by ns3d_sync_device(F).
+ *
+ * One rank, periodic in all directions: CartDecomp describes the single
+ * rank and HaloRequests has nothing to wait for.  Errors are printed to
+ * std::cerr with an "ns3d:" prefix and the function returns, as the
+ * reference's functions do (InviscousFlux.cpp:19-31).
+ */
+#ifndef NS3D_DROPIN_H
+#define NS3D_DROPIN_H
+
+#include <string>
+#include <vector>
+
+struct GridDesc {  // field_structures.h:33-38
+    int global_nx = 0, global_ny = 0, global_nz = 0;
+    double x0 = 0., y0 = 0., z0 = 0.;
+    double Lx = 1., Ly = 1., Lz = 1.;
+    double dx = 1., dy = 1., dz = 1.;
+};
+
+struct CartDecomp {  // one rank of a periodic 1x1x1 decomposition
+    int rank = 0;
+    int size = 1;
+    int dims[3] = {1, 1, 1};
+    int coords[3] = {0, 0, 0};
+    int periods[3] = {1, 1, 1};
+};
+
+struct LocalDesc {  // field_structures.h:50-60
+    int nx = 0, ny = 0, nz = 0;
+    int ngx = 1, ngy = 1, ngz = 1;
+    int sx = 0, sy = 0, sz = 0;
+    int ox = 0, oy = 0, oz = 0;
+    int nbr_xm = 0, nbr_xp = 0, nbr_ym = 0, nbr_yp = 0, nbr_zm = 0, nbr_zp = 0;
+};
+
+inline int idx3(int i, int j, int k, const LocalDesc &L) noexcept { return (k * L.sy + j) * L.sx + i; }
+inline int idx_fx(int i, int j, int k, const LocalDesc &L) noexcept { return (k * L.sy + j) * (L.sx - 1) + i; }
+inline int idx_fy(int i, int j, int k, const LocalDesc &L) noexcept { return (k * (L.sy - 1) + j) * L.sx + i; }
+inline int idx_fz(int i, int j, int k, const LocalDesc &L) noexcept { return (k * L.sy + j) * L.sx + i; }
+
+struct SolverParams {  // numerical and control fields of field_structures.h:86-186
+    double gamma = 1.4;
+    double Pr = 0.71;
+    double Ma = 0.4;
+    double Re = 500.0;
+    double cfl = 0.1;
+    double dt_fixed = -1.0;
+    double mu = 1.0 / Re;
+    double S_ref = 110.4 / 288.15;
+    double Cv = 1.0 / (gamma * (gamma - 1.0) * Ma * Ma);
+    double Cp = Cv * gamma;
+    double Rgas = 1.0 / (Ma * Ma * gamma);
+    double get_mu(double T) const;  // Sutherland
+    enum class Interpolation { WENO5, ZERO, MDCD_LINEAR, MDCD_HYBRID, WENO5_JS };
+    enum class RiemannSolver { Roe, Rusanov, HLLC, HLL, HLLC_p, AUSM };
+    enum class BCType { Periodic, Wall, Symmetry, Outflow, Inflow };
+    RiemannSolver riemann_solver = RiemannSolver::Roe;
+    Interpolation interpolation = Interpolation::WENO5;
+    double mdcd_diss = 0.01;
+    double mdcd_disp = 0.0463783;
+    int stencil = 6;
+    int ghost_layers = 3;
+    bool char_recon = false;
+    BCType bc_xmin = BCType::Periodic, bc_xmax = BCType::Periodic;
+    BCType bc_ymin = BCType::Periodic, bc_ymax = BCType::Periodic;
+    BCType bc_zmin = BCType::Periodic, bc_zmax = BCType::Periodic;
+    int max_steps = 1;
+    int monitor_Stepfreq = 1;
+    double output_Timefreq = 1.0;
+    double TotalTime = 1.0;
+    bool monitor_res = true;
+    bool monitor_energy = true;
+};
+
+struct HaloRequests {};  // nothing outstanding on one GPU
+
+struct ns3d_mirror;  // device state of one Field3D (ns3d_dropin.cpp)
+
+struct Field3D {  // the hot-path members of field_structures.h:196-509
+    LocalDesc L;
+    std::vector<double> rho, rhou, rhov, rhow, E;
+    std::vector<double> rho0, rhou0, rhov0, rhow0, E0;
+    std::vector<double> u, v, w, p, T;
+    std::vector<double> rhs_rho, rhs_rhou, rhs_rhov, rhs_rhow, rhs_E;
+    double global_Etot = 0.0;
+    // filled on demand by the stepwise functions (empty until then)
+    std::vector<double> du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz, dT_dx, dT_dy, dT_dz;
+    std::vector<double> Fvflux_mass, Fvflux_momx, Fvflux_momy, Fvflux_momz, Fvflux_E;
+    std::vector<double> Hvflux_mass, Hvflux_momx, Hvflux_momy, Hvflux_momz, Hvflux_E;
+    std::vector<double> Gvflux_mass, Gvflux_momx, Gvflux_momy, Gvflux_momz, Gvflux_E;
+    std::vector<double> flux_fx_mass, flux_fx_momx, flux_fx_momy, flux_fx_momz, flux_fx_E;
+    std::vector<double> flux_fy_mass, flux_fy_momx, flux_fy_momy, flux_fy_momz, flux_fy_E;
+    std::vector<double> flux_fz_mass, flux_fz_momx, flux_fz_momy, flux_fz_momz, flux_fz_E;
+
+    Field3D() = default;
+    Field3D(const Field3D &o);  // host copy; the copy has no device mirror
+    Field3D &operator=(const Field3D &) = delete;
+    ~Field3D();
+
+    // sx = nx + 2 ngx, ...; rho = 1, everything else 0 (Field3D::allocate)
+    void allocate(const LocalDesc &Ld);
+    int I(int i, int j, int k) const noexcept { return idx3(i, j, k, L); }
+    void conservedToPrimitive(const SolverParams &par);
+    void primitiveToConserved(const SolverParams &par);
+    void recordConservedTo0();
+
+    ns3d_mirror *dev = nullptr;
+};
+
+// solver.in reader with the reference's keys and derived values
+// (initialization.cpp:36-200); CartDecomp gets the periodicity.
+bool read_solver_params_from_file(const std::string &fname, SolverParams &P, GridDesc &G, CartDecomp &C);
+
+// The one-rank local description of G (ghost layers from P).
+LocalDesc make_local_desc(const GridDesc &G, const SolverParams &P);
+
+// Hot path (ns3d_func.h:60-140).
+void apply_boundary(Field3D &F, GridDesc &G, CartDecomp &C, const SolverParams &P);
+void apply_boundary_halfnode_flux(Field3D &F, const GridDesc &G, CartDecomp &C, const SolverParams &P);
+void compute_invis_flux(Field3D &F, const SolverParams &P);
+void compute_invis_dflux(Field3D &F, const SolverParams &P, const GridDesc &G);
+void compute_gradients(Field3D &F, const GridDesc &G);
+void compute_viscous_flux(Field3D &F, const SolverParams &P);
+void exchange_halos_viscous_flux(Field3D &F, CartDecomp &C, LocalDesc &L, HaloRequests &reqs);
+void compute_vis_flux(Field3D &F, const GridDesc &G);
+void compute_rhs(Field3D &F, CartDecomp &C, GridDesc &G, SolverParams &P, HaloRequests &reqs);
+void runge_kutta_3(Field3D &F, CartDecomp &C, GridDesc &G, SolverParams &P, HaloRequests &reqs, double dt);
+double compute_timestep(Field3D &F, const GridDesc &G, const SolverParams &P);
+// sum 0.5 rho |u|^2 dV over the interior into F.global_Etot (postprocess.cpp:103-128)
+void compute_total_energy(Field3D &F, const GridDesc &G, CartDecomp &C, const SolverParams &P);
+
+// Mirror control (no counterpart in the reference).
+void ns3d_set_resident(Field3D &F, bool resident);
+void ns3d_sync_host(Field3D &F);    // device state -> host vectors (conserved, primitives, RHS)
+void ns3d_sync_device(Field3D &F);  // host conserved fields -> device (primitives and ghosts refreshed)
+
+#endif /* NS3D_DROPIN_H */
