@@ -70,6 +70,9 @@ def parse():
                     help="also run the adjoint workload (128^3, 50 steps, checkpoint every 10, FD check)")
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--adj-steps", type=int, default=50)
+    ap.add_argument("--sweep", type=int, default=1,
+                    help="also time {WCNS5, WENO5-JS, MDCD} x {HLLC, Rusanov} on the same workload")
+    ap.add_argument("--sweep-steps", type=int, default=5)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU checker on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
@@ -129,6 +132,36 @@ def mixed_report(args, prim, G, P, ms_plain, timed_run):
             "spectrum_per_shell": [float(f"{e:.2e}") for e in rep["spectrum_per_shell"]],
         },
     }
+
+
+def sweep_report(args, prim, ma, G, stream):
+    """Throughput of each {WCNS5, WENO5-JS, MDCD hybrid} x {HLLC, Rusanov}
+    combination on the same state (FP64, --sweep-steps timed steps after one
+    warm-up step)."""
+    import torch
+
+    from ns3d_wcns_amd import DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams
+
+    cells = G.global_nx * G.global_ny * G.global_nz
+    res = {}
+    for scheme in ("wcns5", "weno5_js", "mdcd"):
+        for rs in ("HLLC", "Rusanov"):
+            P = SolverParams(Ma=ma, Re=100.0, Pr=0.72, cfl=0.5, interpolation=SCHEME_ALIASES[scheme],
+                             riemann_solver=RiemannSolver[rs]).derive()
+            dev = DeviceField(G, P)
+            dev.set_stream(stream.cuda_stream)
+            dev.initialize_primitive(*prim)
+            dev.advance_steps(1)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            dev.advance_steps(args.sweep_steps)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.sweep_steps
+            dev.close()
+            res[f"{scheme.upper()}+{rs}"] = {"ms_per_step": round(ms, 4), "Mcell_steps_per_s": round(cells / ms / 1e3, 1)}
+    return res
 
 
 def adjoint_report(args):
@@ -345,6 +378,8 @@ def main():
     }
     if args.mixed:
         out["mixed"] = mixed_report(args, prim, G, P, ms, timed_run)
+    if args.sweep:
+        out["sweep"] = sweep_report(args, prim, ma, G, stream)
     if args.adjoint:
         out["adjoint"] = adjoint_report(args)
     if args.cpu_baseline:
