@@ -219,6 +219,21 @@ int ns3d_interpolate_batch(const ns3d_params *params, const double *d_stencils, 
 int ns3d_riemann_batch(const ns3d_params *params, const double *d_UL, const double *d_UR, int n, int dim,
                        double *d_F, void *stream);
 
+/* ---- isotropic-turbulence observables on the device ----
+ * energy_spectrum: E(k) of the velocity fields (compute_energy_spectrum_rank0,
+ * EnergySpectrum.cpp:63-156): forward 3D FFT normalised by 1/N, 0.5 (|U|^2 +
+ * |V|^2 + |W|^2) summed over shells lround(|k|), k = 0..nshell-1 (the
+ * reference writes nshell = 2 nx + 1).
+ * turbulence_statistics (compute_turbulence_statistics,
+ * ana_isotropic.cpp:238-305): mean kinetic energy 0.5 <|u|^2>, TKE (half the
+ * summed component variances), u_rms, nu = <mu(T)> / <rho> and the spectral
+ * dissipation 2 nu sum k^2 E(k). */
+typedef struct ns3d_turbulence {
+    double kinetic_energy, tke, u_rms, nu, dissipation;
+} ns3d_turbulence;
+int ns3d_energy_spectrum(ns3d_ctx *ctx, double *Ek, int nshell);
+int ns3d_turbulence_statistics(ns3d_ctx *ctx, ns3d_turbulence *out);
+
 /* ---- device-to-device field copies (no counterpart in the reference) ----
  * For harnesses that keep states in their own device buffers (sx*sy*sz
  * doubles, the ghosted layout), e.g. checkpoints of the adjoint. */

@@ -46,6 +46,12 @@ class CFlops(ctypes.Structure):
                                                 "per_step")]
 
 
+class CTurbulence(ctypes.Structure):
+    """``ns3d_turbulence`` of include/ns3d_hip.h."""
+
+    _fields_ = [(n, ctypes.c_double) for n in ("kinetic_energy", "tke", "u_rms", "nu", "dissipation")]
+
+
 class NS3DError(RuntimeError):
     """A nonzero status from the device library, with its message."""
 
@@ -93,6 +99,8 @@ _SIGS = {
     "ns3d_total_kinetic_energy": [_VP, _DP],
     "ns3d_set_kernel_timing": [_VP, ctypes.c_int],
     "ns3d_kernel_times": [_VP, _DP, ctypes.POINTER(ctypes.c_int)],
+    "ns3d_energy_spectrum": [_VP, _DP, ctypes.c_int],
+    "ns3d_turbulence_statistics": [_VP, ctypes.POINTER(CTurbulence)],
     "ns3d_copy_field_to_device": [_VP, ctypes.c_int, _VP],
     "ns3d_copy_field_from_device": [_VP, ctypes.c_int, _VP],
     "ns3d_set_state": [_VP, _VP5],

@@ -287,7 +287,6 @@ __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
     const March<DIR> M(a.b);
     const int tid = threadIdx.y * 64 + threadIdx.x;
     if (!M.ok) return;  // no barrier below: each thread owns its ring column
-    const Box &b = a.b;
     double w[5][5];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {

@@ -19,6 +19,7 @@
 
 #include "../../include/ns3d_hip.h"
 #include "ns3d_kernels.h"
+#include "ns3d_observe.h"
 
 using namespace ns3d;
 
@@ -44,6 +45,10 @@ struct ns3d_ctx {
     double *a0[5], *a1[5], *pr[5];  // stage accumulators and the RHS derivative
     double *cw[25];  // compact: face adjoints and state adjoints, or face and viscous tangents
     double *gb[12], *qb[4];
+    // observables work memory (allocated on first use) and the FFT plan
+    double *obs_work;
+    long obs_cap;
+    void *fft_plan;
     double *partial;
     double *scal;  // [0] dt, [1] reduction result
     bool timing;
@@ -711,6 +716,9 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     c->scal = p;
     c->grad_slab = nullptr;
     c->adj_slab = nullptr;
+    c->obs_work = nullptr;
+    c->obs_cap = 0;
+    c->fft_plan = nullptr;
     c->timing = false;
     c->visc_fp32 = false;
     if (hipMemset(c->slab, 0, total * sizeof(double)) != hipSuccess) {
@@ -764,6 +772,8 @@ int ns3d_destroy(ns3d_ctx *c) {
     }
     if (c->grad_slab) (void)hipFree(c->grad_slab);
     if (c->adj_slab) (void)hipFree(c->adj_slab);
+    if (c->obs_work) (void)hipFree(c->obs_work);
+    destroy_fft_plan(c->fft_plan);
     (void)hipFree(c->slab);
     delete c;
     return NS3D_OK;
@@ -1047,6 +1057,84 @@ int ns3d_total_kinetic_energy(ns3d_ctx *c, double *out) {
     NS3D_LAUNCHED("kinetic energy");
     NS3D_HIP(hipMemcpyAsync(out, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_obs(ns3d_ctx *c, long doubles) {
+    if (c->obs_cap >= doubles) return NS3D_OK;
+    if (c->obs_work) (void)hipFree(c->obs_work);
+    c->obs_work = nullptr;
+    c->obs_cap = 0;
+    if (hipMalloc(&c->obs_work, (size_t)doubles * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(NS3D_E_NOMEM, "observables: out of device memory for %ld doubles", doubles);
+    }
+    c->obs_cap = doubles;
+    return NS3D_OK;
+}
+
+int spectrum(ns3d_ctx *c, int nshell, double *host) {
+    int rc;
+    if ((rc = ensure_obs(c, spectrum_work_doubles(c->b, nshell)))) return rc;
+    const double *vel[3] = {c->cell[NS3D_U], c->cell[NS3D_V], c->cell[NS3D_W]};
+    if (launch_spectrum(c->b, vel, nshell, c->obs_work, c->fft_plan, c->stream))
+        return fail(NS3D_E_HIP, "hipFFT failed for the %d x %d x %d transform", c->b.nz, c->b.ny, c->b.nx);
+    NS3D_LAUNCHED("energy spectrum");
+    NS3D_HIP(hipMemcpyAsync(host, spectrum_result(c->b, nshell, c->obs_work), (size_t)nshell * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ns3d_energy_spectrum(ns3d_ctx *c, double *Ek, int nshell) {
+    NS3D_CTX(c);
+    if (!Ek || nshell < 1) return fail(NS3D_E_ARG, "energy spectrum: bad output");
+    return spectrum(c, nshell, Ek);
+}
+
+int ns3d_turbulence_statistics(ns3d_ctx *c, ns3d_turbulence *out) {
+    NS3D_CTX(c);
+    if (!out) return fail(NS3D_E_ARG, "null output");
+    const int nshell = 2 * c->b.nx + 1;
+    std::vector<double> ek(nshell);
+    int rc;
+    if ((rc = spectrum(c, nshell, ek.data()))) return rc;
+    if ((rc = ensure_obs(c, std::max(c->obs_cap, moment_work_doubles())))) return rc;
+    MomentArgs m{};
+    m.rho = c->cell[NS3D_RHO];
+    m.u = c->cell[NS3D_U];
+    m.v = c->cell[NS3D_V];
+    m.w = c->cell[NS3D_W];
+    m.T = c->cell[NS3D_T];
+    m.b = c->b;
+    m.mu = c->P.mu;
+    m.S_ref = c->P.S_ref;
+    m.partial = c->obs_work;
+    m.out = c->obs_work + moment_work_doubles() - 8;
+    launch_moments(m, c->stream);
+    NS3D_LAUNCHED("moments");
+    double s[8];
+    NS3D_HIP(hipMemcpyAsync(s, m.out, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    NS3D_HIP(hipStreamSynchronize(c->stream));
+    const double n = (double)c->nint;
+    const double nu = (s[1] / n) / (s[0] / n);
+    out->kinetic_energy = 0.5 * (s[5] + s[6] + s[7]) / n;
+    out->u_rms = std::sqrt((s[5] + s[6] + s[7]) / (3.0 * n));
+    double tke = 0.0;
+    for (int q = 0; q < 3; ++q) tke += s[5 + q] / n - (s[2 + q] / n) * (s[2 + q] / n);
+    out->tke = 0.5 * tke;
+    out->nu = nu;
+    double eps = 0.0;
+    for (int k = 0; k < nshell; ++k) eps += 2.0 * nu * (double)k * (double)k * ek[k];
+    out->dissipation = eps;
     return NS3D_OK;
 }
 

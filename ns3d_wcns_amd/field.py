@@ -237,6 +237,22 @@ class DeviceField:
         self.recordConservedTo0()
         self.runge_kutta_3(dt)
 
+    # ---------------------------------------------------------------- observables
+    def energy_spectrum(self, nshell: int | None = None) -> np.ndarray:
+        """E(k), k = 0..nshell-1 (default 2 nx + 1), computed on the device
+        from the context's velocity fields (EnergySpectrum.cpp:63-156)."""
+        n = nshell or 2 * self.nx + 1
+        out = np.zeros(n)
+        check(self._lib.ns3d_energy_spectrum(self._h, _dptr(out), n))
+        return out
+
+    def turbulence_statistics(self) -> dict:
+        """Kinetic energy, TKE, u_rms, nu and the spectral dissipation,
+        computed on the device (ana_isotropic.cpp:238-305)."""
+        t = _lib.CTurbulence()
+        check(self._lib.ns3d_turbulence_statistics(self._h, ctypes.byref(t)))
+        return {n: getattr(t, n) for n, _ in _lib.CTurbulence._fields_}
+
     def snapshot(self) -> dict:
         """Interior copies of the conserved fields and the primitives."""
         return {name: self.interior(self.download(name)).copy() for name in CONSERVED + PRIMITIVE}

@@ -4,7 +4,9 @@ CFL 0.5, 100 RK3 steps), and the device kinetic-energy sum against the host.
 
 Tolerances (norm-wise relative): the dt sequence 1e-12; the conserved fields
 1e-10; kinetic energy, TKE and the spectral dissipation 1e-11; E(k) 1e-9 for
-every shell holding at least 1e-12 of the peak shell energy.
+every shell holding at least 1e-12 of the peak shell energy.  The device
+observables (hipFFT spectrum, fixed-order moment sums) against the host
+formulas on the same fields: 1e-12.
 """
 import numpy as np
 import pytest
@@ -71,3 +73,30 @@ def test_device_kinetic_energy_matches_host_sum(gpu):
     ek = energy_spectrum(u, v, w)
     assert abs(ek.sum() - 0.5 * np.mean(u * u + v * v + w * w)) <= 1e-12 * ek.sum()
     dev.close()
+
+
+@pytest.mark.parametrize("n", [32, (24, 20, 16)])
+def test_device_spectrum_and_statistics_match_host(gpu, n):
+    prim, ma = state_for(n)
+    P = params(ma)
+    G = box(n)
+    dev = DeviceField(G, P)
+    dev.initialize_primitive(*prim)
+    dev.advance_steps(2)
+    f = {q: dev.interior(dev.download(q)) for q in ("rho", "u", "v", "w", "T")}
+    host = turbulence_statistics(f["rho"], f["u"], f["v"], f["w"], f["T"], P)
+    ek = dev.energy_spectrum()
+    eh = host["spectrum"]
+    big = eh >= 1e-12 * eh.max()
+    assert np.max(np.abs(ek[big] - eh[big]) / eh[big]) < 1e-12
+    assert np.all(np.abs(ek[~big]) <= 1e-12 * eh.max())
+    st = dev.turbulence_statistics()
+    for k in ("kinetic_energy", "tke", "u_rms", "nu", "dissipation"):
+        assert abs(st[k] - host[k]) <= 1e-12 * abs(host[k]), (k, st[k], host[k])
+    dev.close()
+
+
+def state_for(n):
+    from cases import state
+
+    return state(n, 0.5, noise=0.01, seed=17)
