@@ -70,35 +70,10 @@ NS3D_HD double weno_vjp(const double *f, double yb, double *fb) {
     return y;
 }
 
-// Both reconstructions of one cell from its 5-cell stencil f = cells
-// c-2..c+2, sharing the smoothness indicators: yL = weno(f0..f4) (value at
-// the right face) and yR = weno(f4..f0) (value at the left face); the same
-// function values as wcns5 / weno5js up to rounding.
+// Both reconstructions of one cell (recon_pair, ns3d_device.h) in FP64.
 template <bool JS>
 NS3D_HD void weno_pair(const double *f, double &yL, double &yR) {
-    const double eps = 1e-6;
-    const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
-    const double t0 = f0 - 2.0 * f1 + f2, t1 = f1 - 2.0 * f2 + f3, t2 = f2 - 2.0 * f3 + f4;
-    const double s0 = f0 - 4.0 * f1 + 3.0 * f2, s1 = f1 - f3, s2 = 3.0 * f2 - 4.0 * f3 + f4;
-    const double b0 = (13.0 / 12.0) * t0 * t0 + 0.25 * s0 * s0;
-    const double b1 = (13.0 / 12.0) * t1 * t1 + 0.25 * s1 * s1;
-    const double b2 = (13.0 / 12.0) * t2 * t2 + 0.25 * s2 * s2;
-    const double q0 = (eps + b0) * (eps + b0), q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2);
-    const double d0 = JS ? 0.1 : 1.0 / 16.0, d1 = JS ? 0.6 : 5.0 / 8.0, d2 = JS ? 0.3 : 5.0 / 16.0;
-    const double K = JS ? 6.0 : 8.0;
-    const double c0 = JS ? 2.0 : 3.0, c1 = JS ? -7.0 : -10.0, c2 = JS ? 11.0 : 15.0;
-    const double m1 = JS ? 5.0 : 6.0, m2 = JS ? 2.0 : 3.0;
-    const double q12 = q1 * q2, q02 = q0 * q2, q01 = q0 * q1;
-    {
-        const double e0 = d0 * q12, e1 = d1 * q02, e2 = d2 * q01;
-        const double p0 = c0 * f0 + c1 * f1 + c2 * f2, p1 = -f1 + m1 * f2 + m2 * f3, p2 = m2 * f2 + m1 * f3 - f4;
-        yL = (e0 * p0 + e1 * p1 + e2 * p2) / (K * (e0 + e1 + e2));
-    }
-    {
-        const double e0 = d0 * q01, e1 = d1 * q02, e2 = d2 * q12;
-        const double p0 = c0 * f4 + c1 * f3 + c2 * f2, p1 = -f3 + m1 * f2 + m2 * f1, p2 = m2 * f2 + m1 * f1 - f0;
-        yR = (e0 * p0 + e1 * p1 + e2 * p2) / (K * (e0 + e1 + e2));
-    }
+    recon_pair<JS ? I_WENO5_JS : I_WENO5>(f, yL, yR);
 }
 
 // Reverse mode of both reconstructions of one cell from its 5-cell stencil

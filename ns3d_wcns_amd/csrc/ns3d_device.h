@@ -203,6 +203,43 @@ NS3D_HD R interp6(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5, R 
     else return mdcd_hybrid(c, f0, f1, f2, f3, f4, f5, off);
 }
 
+// Both reconstructions of one cell from its 5-cell stencil f = cells c-2..c+2
+// for the component-wise WCNS5 / WENO5-JS schemes: yL = interp(f0..f4) (the
+// cell's value at its right face, the left state of face c+1/2) and yR =
+// interp(f4..f0) (the value at its left face, the right state of face
+// c-1/2).  The mirrored stencil has the same smoothness indicators with
+// beta_0 and beta_2 swapped, so they are formed once; the values equal
+// wcns5 / weno5js up to rounding.
+template <int INTERP, class R>
+NS3D_HD void recon_pair(const R *f, R &yL, R &yR) {
+    static_assert(INTERP == I_WENO5 || INTERP == I_WENO5_JS, "recon_pair: WCNS5 or WENO5-JS");
+    constexpr bool JS = INTERP == I_WENO5_JS;
+    const R eps = R(1e-6);
+    const R f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    const R t0 = f0 - R(2) * f1 + f2, s0 = f0 - R(4) * f1 + R(3) * f2;
+    const R t1 = f1 - R(2) * f2 + f3, s1 = f1 - f3;
+    const R t2 = f2 - R(2) * f3 + f4, s2 = R(3) * f2 - R(4) * f3 + f4;
+    const R b0 = R(13.0 / 12.0) * t0 * t0 + R(0.25) * s0 * s0;
+    const R b1 = R(13.0 / 12.0) * t1 * t1 + R(0.25) * s1 * s1;
+    const R b2 = R(13.0 / 12.0) * t2 * t2 + R(0.25) * s2 * s2;
+    const R q0 = (eps + b0) * (eps + b0), q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2);
+    const R d0 = JS ? R(0.1) : R(1.0 / 16.0), d1 = JS ? R(0.6) : R(5.0 / 8.0), d2 = JS ? R(0.3) : R(5.0 / 16.0);
+    const R K = JS ? R(6) : R(8);
+    const R c0 = JS ? R(2) : R(3), c1 = JS ? R(-7) : R(-10), c2 = JS ? R(11) : R(15);
+    const R m1 = JS ? R(5) : R(6), m2 = JS ? R(2) : R(3);
+    const R q12 = q1 * q2, q02 = q0 * q2, q01 = q0 * q1;
+    {
+        const R e0 = d0 * q12, e1 = d1 * q02, e2 = d2 * q01;
+        const R p0 = c0 * f0 + c1 * f1 + c2 * f2, p1 = -f1 + m1 * f2 + m2 * f3, p2 = m2 * f2 + m1 * f3 - f4;
+        yL = fdiv(e0 * p0 + e1 * p1 + e2 * p2, K * (e0 + e1 + e2));
+    }
+    {
+        const R e0 = d0 * q01, e1 = d1 * q02, e2 = d2 * q12;
+        const R p0 = c0 * f4 + c1 * f3 + c2 * f2, p1 = -f3 + m1 * f2 + m2 * f1, p2 = m2 * f2 + m1 * f1 - f0;
+        yR = fdiv(e0 * p0 + e1 * p1 + e2 * p2, K * (e0 + e1 + e2));
+    }
+}
+
 // Left (+1) and right (-1) states of one 6-point line s[0..5] = cells i-2..i+3
 // (interpolate_select, interpolation.cpp:13-77: the right state reverses the
 // stencil).
