@@ -88,6 +88,10 @@ struct SolverParams {  // numerical and control fields of field_structures.h:86-
     double TotalTime = 1.0;
     bool monitor_res = true;
     bool monitor_energy = true;
+    bool post_basicfield = true;         // field.h5 at the output times (maincycle.cpp:94-98)
+    bool isotropic_analyse = true;       // parsed; the spectral post-processing is not part of this library
+    bool restart = false;                // initial state from restart_file (main.cpp:69-84)
+    std::string restart_file = "field.h5";
 };
 
 struct HaloRequests {};  // nothing outstanding on one GPU
@@ -147,7 +151,26 @@ double compute_timestep(Field3D &F, const GridDesc &G, const SolverParams &P);
 // sum 0.5 rho |u|^2 dV over the interior into F.global_Etot (postprocess.cpp:103-128)
 void compute_total_energy(Field3D &F, const GridDesc &G, CartDecomp &C, const SolverParams &P);
 
+// field.h5 output and restart with serial HDF5 on the one rank: attributes
+// global_nx/ny/nz (int), time, dx, dy, dz (double); datasets X Y Z rho u v w
+// E p T of shape (global_nz, global_ny, global_nx), IEEE F64LE, x fastest
+// (output.cpp:19-179, initialization.cpp:270-420).  The HDF5 library is
+// loaded when first needed (NS3D_HDF5_LIBRARY, else libhdf5.so.103 from the
+// loader path or /opt/conda/lib); without it these print the reason and
+// return false, as the reference's serial-HDF5 build does.  A device-resident
+// Field3D is synchronised to the host before it is written and pushed to the
+// device after it is read.
+bool write_hdf5_field(const Field3D &F, const GridDesc &G, const CartDecomp &C, const std::string &filepath,
+                      double time);
+// output/time_<int(time * 1e5)>/field.h5 (output.cpp:181-203)
+void write_tecplot_field(const Field3D &F, const GridDesc &G, const CartDecomp &C, const SolverParams &P,
+                         double time = 0.0);
+// interior rho, u, v, w, E, p, T from the file; rho u = rho * u, ... (initialization.cpp:270-420)
+bool initialize_from_hdf5(Field3D &F, const GridDesc &G, const CartDecomp &C, const SolverParams &P,
+                          const std::string &filename);
+
 // Mirror control (no counterpart in the reference).
+bool ns3d_is_resident(const Field3D &F);
 void ns3d_set_resident(Field3D &F, bool resident);
 void ns3d_sync_host(Field3D &F);    // device state -> host vectors (conserved, primitives, RHS)
 void ns3d_sync_device(Field3D &F);  // host conserved fields -> device (primitives and ghosts refreshed)

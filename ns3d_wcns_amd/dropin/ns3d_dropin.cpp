@@ -249,7 +249,13 @@ bool read_solver_params_from_file(const std::string &fname, SolverParams &P, Gri
         std::cerr << "Cannot open " << fname << "\n";
         return false;
     }
+    // defaults first (initialization.cpp:48-52): every field of P, the grid
+    // size and origin of G
+    P = SolverParams();
+    G.global_nx = G.global_ny = G.global_nz = 16;
+    G.x0 = G.y0 = G.z0 = 0.0;
     using BC = SolverParams::BCType;
+    auto flag = [](const std::string &v) { return v == "true" || v == "ture" || v == "yes" || v == "1" || v == "on"; };
     auto bc_of = [](const std::string &v) {
         if (v == "inflow") return BC::Inflow;
         if (v == "wall") return BC::Wall;
@@ -307,6 +313,12 @@ bool read_solver_params_from_file(const std::string &fname, SolverParams &P, Gri
             else if (k == "bc_ymax") P.bc_ymax = bc_of(v);
             else if (k == "bc_zmin") P.bc_zmin = bc_of(v);
             else if (k == "bc_zmax") P.bc_zmax = bc_of(v);
+            else if (k == "restart") P.restart = flag(v);
+            else if (k == "restart_file") P.restart_file = val;
+            else if (k == "post_basicfield") P.post_basicfield = flag(v);
+            else if (k == "isotropicanalyse") P.isotropic_analyse = flag(v);
+            else if (k == "monitor_res") P.monitor_res = flag(v);
+            else if (k == "monitor_energy") P.monitor_energy = flag(v);
         } catch (const std::exception &) {
             std::cerr << "ns3d: bad value for " << k << ": " << val << "\n";
             return false;
@@ -444,6 +456,8 @@ void compute_total_energy(Field3D &F, const GridDesc &G, CartDecomp &, const Sol
 }
 
 // ------------------------------------------------------------ mirror control
+bool ns3d_is_resident(const Field3D &F) { return resident(F); }
+
 void ns3d_set_resident(Field3D &F, bool on) {
     if (!F.dev) F.dev = new ns3d_mirror();
     if (on == F.dev->resident) return;

@@ -116,6 +116,12 @@ class SolverParams:
     monitor_Stepfreq: int = 1
     output_Timefreq: float = 1.0
     TotalTime: float = 1.0
+    monitor_res: bool = True
+    monitor_energy: bool = True
+    post_basicfield: bool = True
+    isotropic_analyse: bool = True
+    restart: bool = False
+    restart_file: str = "field.h5"
     # derived (set by derive(); struct defaults follow field_structures.h:93-115)
     mu: float = 1.0 / 500.0
     Cv: float = 1.0 / (1.4 * 0.4 * 0.4 * 0.4)
@@ -180,6 +186,11 @@ def _parse_bool(v: str) -> bool:
     return v.lower() in ("true", "ture", "yes", "1", "on")
 
 
+def _flag(v: str) -> bool:
+    """The reference's boolean spelling set (initialization.cpp:30-34)."""
+    return v in ("true", "ture", "yes", "1", "on")
+
+
 def read_solver_params_from_file(fname: str):
     """Parse a ``solver.in`` file: ``key = value`` lines, keys and values
     case-insensitive, ``#`` comments, unknown keys ignored, unknown BC strings
@@ -231,6 +242,12 @@ def read_solver_params_from_file(fname: str):
         elif k == "monitor_stepfreq": P.monitor_Stepfreq = int(val)
         elif k == "output_timefreq": P.output_Timefreq = float(val)
         elif k == "totaltime": P.TotalTime = float(val)
+        elif k == "restart": P.restart = _flag(v)
+        elif k == "restart_file": P.restart_file = val
+        elif k == "post_basicfield": P.post_basicfield = _flag(v)
+        elif k == "isotropicanalyse": P.isotropic_analyse = _flag(v)
+        elif k == "monitor_res": P.monitor_res = _flag(v)
+        elif k == "monitor_energy": P.monitor_energy = _flag(v)
         if k in bcs:
             P.bc[bcs[k]] = bcmap.get(v, BCType.Periodic)
     periods = [int(P.bc[2 * a] == BCType.Periodic and P.bc[2 * a + 1] == BCType.Periodic) for a in range(3)]
