@@ -271,8 +271,10 @@ int halo(ns3d_ctx *c, bool all) {
 }
 
 // One fused stage (stage 0: RHS only): viscous fluxes, then the z, y and x
-// pencils; stages 1..3 update the state in place and refill the ghost layers.
-int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false) {
+// pencils; stages 1..3 update the state in place.  The kernels read no ghost
+// layer (periodic images instead); with ghosts the layers of the new state are
+// refilled for the callers that read them.
+int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false, bool ghosts = true) {
     {
         Timed t(c, NS3D_K_VISCOUS);
         launch_visc_march(visc_args(c, false, true), c->precision, c->stream);
@@ -317,15 +319,17 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
         }
     }
     NS3D_LAUNCHED("pencils");
-    if (stage == 0) return NS3D_OK;
+    if (stage == 0 || !ghosts) return NS3D_OK;
     return halo(c, true);
 }
 
 // record: stage 1 also stores U0 (interior cells) from the state it reads.
+// The intermediate stage states are not handed out, so only the last stage
+// refills the ghost layers.
 int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false) {
     int rc;
     for (int stage = 1; stage <= 3; ++stage)
-        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1))) return rc;
+        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, stage == 3))) return rc;
     return NS3D_OK;
 }
 

@@ -46,7 +46,8 @@ __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
 // The same viscous fluxes from a z-marching 64 x 4 column tile: each thread
 // keeps u, v, w, T of planes k-3..k+3 of its own column in registers (z
 // derivatives); each plane's tile with a 3-cell x/y halo is staged in LDS
-// (x and y derivatives).  Reads the ghosted primitives (ghost layers filled).
+// (x and y derivatives).  Cells beyond the box are read from their periodic
+// image.
 constexpr int VZ_SEG = 16;
 
 // R = double: the plain method.  R = float: the mixed method's FP32 viscous
@@ -62,26 +63,27 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int i = i0 + tx, j = j0 + ty;
     const bool own = i < b.nx && j < b.ny;
-    // threads of a partial tile whose column is a ghost column (i < nx + 3,
-    // j < ny + 3) still stage its value: their tile slots are neighbours of
-    // the owned cells; columns beyond the ghost layers are never read
-    const bool inbox = i < b.nx + 3 && j < b.ny + 3;
+    // every tile slot holds the periodic image of its cell (no ghost layer is
+    // read); slots beyond the first three columns / rows past the box are
+    // staged but never read
+    auto per = [](int x, int n) { return ((x % n) + n) % n; };
     const int k0 = blockIdx.z * VZ_SEG, k1 = min(k0 + VZ_SEG, b.nz);
     const long sxy = (long)b.sx * b.sy;
     const double tref = sizeof(R) == 4 ? 1.0 : 0.0;
     auto ld = [&](int q, long g) -> R { return q == 3 ? (R)(a.Q[3][g] - tref) : (R)a.Q[q][g]; };
     const R i60[3] = {(R)a.i60[0], (R)a.i60[1], (R)a.i60[2]};
     const R mu_ref = (R)a.mu, S_ref = (R)a.S_ref, cp_pr = (R)a.cp_pr;
-    // ghosted index of (i, j, 0); clamped to cell (0, 0, 0) outside the ghosted box
-    const long col = b.gidx(inbox ? i : 0, inbox ? j : 0, 0);
+    // ghosted index of the periodic image of (i, j, 0)
+    const long col = b.gidx(per(i, b.nx), per(j, b.ny), 0);
     R wq[4][7];
 #pragma unroll
     for (int m = 0; m < 6; ++m)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) wq[q][m] = ld(q, col + (long)(k0 - 3 + m) * sxy);
+        for (int q = 0; q < 4; ++q) wq[q][m] = ld(q, col + (long)wrap(k0 - 3 + m, b.nz) * sxy);
     // halo cells of a plane: 6 rows x 64 above/below, then 4 rows x 6 left/right
     int hi[2], hj[2];
     bool hv[2];
+    long hc[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int e = tid + 256 * h;
@@ -97,16 +99,16 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
         }
         hi[h] = x;
         hj[h] = y;
-        // inside the ghosted box: x in [-3, nx + 3), y in [-3, ny + 3)
-        hv[h] = e < 6 * TX + 6 * TY && i0 + x < b.nx + 3 && j0 + y < b.ny + 3;
+        hv[h] = e < 6 * TX + 6 * TY;
+        hc[h] = b.gidx(per(i0 + x, b.nx), per(j0 + y, b.ny), 0);
     }
     for (int k = k0; k < k1; ++k) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) wq[q][6] = ld(q, col + (long)(k + 3) * sxy);
+        for (int q = 0; q < 4; ++q) wq[q][6] = ld(q, col + (long)wrap(k + 3, b.nz) * sxy);
         R hvv[2][4];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const long g = b.gidx(i0 + hi[h], j0 + hj[h], k);
+            const long g = hc[h] + (long)k * sxy;
 #pragma unroll
             for (int q = 0; q < 4; ++q) hvv[h][q] = hv[h] ? ld(q, g) : R(0);
         }

@@ -5,12 +5,16 @@
 // the viscous-flux divergence of that direction (compute_vis_flux,
 // ViscousFlux.cpp:210-255) without writing face fluxes to memory.
 //
-//   x pencil: one block per (group of) periodic x line(s); the ghosted line is
-//             staged in LDS, one thread per face, exactly N faces per line.
+//   x pencil: one block per (group of) periodic x line(s); the line with
+//             three periodically wrapped cells on each side is staged in LDS,
+//             one thread per face, exactly N faces per line.
 //   y / z pencils: one thread per (x, other, segment) column marching along
 //             the line with a register window of the six stencil cells and six
 //             running cell sums; lanes run along x, so every load is one
 //             coalesced row.  Segments of L cells compute L + 5 faces.
+// None of them reads a ghost layer: stencil cells beyond the box are taken
+// from the periodic image, so the state's ghost layers need to be current
+// only where the state is handed out.
 //
 // Accumulation: the z and y pencils store their parts (-dH/dz + dGv/dz and
 // -dG/dy + dHv/dy) in compact partial arrays; the x pencil sums them with its
@@ -129,10 +133,13 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
             // viscous fluxes
             double uv[5][2], vv[4];
             const bool tail = t + P < W;
+            // line slot x holds cell x - NG of the periodic line (the ghost
+            // layers are not read)
+            const long g0 = row + NG + wrap(t - NG, N), g1 = row + NG + wrap(t + P - NG, N);
 #pragma unroll
             for (int n = 0; n < 5; ++n) {
-                uv[n][0] = a.U[n][row + t];
-                uv[n][1] = tail ? a.U[n][row + t + P] : 0.0;
+                uv[n][0] = a.U[n][g0];
+                uv[n][1] = tail ? a.U[n][g1] : 0.0;
             }
 #pragma unroll
             for (int n = 0; n < 4; ++n) vv[n] = visc_at<R>(a.V[kViscSlot[0][n]], orow + t);
@@ -145,7 +152,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
             for (int n = 0; n < 4; ++n) Vl[n * N + t] = vv[n];
         } else {
             for (int n = 0; n < 5; ++n)
-                for (int x = t; x < W; x += P) Ul[n * W + x] = a.U[n][row + x];
+                for (int x = t; x < W; x += P) Ul[n * W + x] = a.U[n][row + NG + wrap(x - NG, N)];
             for (int n = 0; n < 4; ++n)
                 for (int x = t; x < N; x += P) Vl[n * N + x] = visc_at<R>(a.V[kViscSlot[0][n]], orow + x);
         }
@@ -227,7 +234,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
 // f-3 is complete (faces f-6..f-1, viscous fluxes f-6..f) and is emitted
 // into the pencil's compact partial-RHS array.
 template <int DIR, int INTERP, int RS, bool CHAR, class R>
-__global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf) {
+__device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
     const Box &b = a.b;
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int other = blockIdx.y * 4 + threadIdx.y;  // z for the y pencil, y for the z pencil
@@ -251,16 +258,24 @@ __global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf
         u[3] = (R)a.U[P3][g];
         u[4] = mixed ? (R)(a.U[4][g] - a.ref.E_ref) : (R)a.U[4][g];
     };
-    // pre-scaled weights: face f -> cells f-2..f+3 (slots 1..6), viscous flux
-    // of cell f -> cells f-3..f+3 (slots 0..6)
+    // Cell-sum weights: face f -> cells f-2..f+3 (-kHalfW, slots 1..6),
+    // viscous flux of cell f -> cells f-3..f+3 (central weights / 60, slots
+    // 0..6, none for the cell itself).  FP64 component-wise: pre-scaled by
+    // 1/h.  Otherwise unscaled compile-time constants, which occupy no vector
+    // registers (the kernel then fits two waves per SIMD characteristic-wise,
+    // three with FP32 faces component-wise), and the cell sum is scaled when
+    // emitted.
+    constexpr bool pre = sizeof(R) == 8 && !CHAR;
+    const double sc = pre ? a.ih : 1.0, se = pre ? 1.0 : a.ih;
     double wf[7], wv[7];
     wf[0] = 0.0;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) wf[q + 1] = -a.ih * kHalfW[q];
+    for (int q = 0; q < 6; ++q) wf[q + 1] = -sc * kHalfW[q];
     {
         const double d[7] = {1.0, -9.0, 45.0, 0.0, -45.0, 9.0, -1.0};
+        const double s60 = pre ? a.i60 : 1.0 / 60.0;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) wv[q] = d[q] * a.i60;
+        for (int q = 0; q < 7; ++q) wv[q] = d[q] * s60;
     }
     R w[5][6];
     {
@@ -313,14 +328,15 @@ __global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf
         }
 #pragma unroll
         for (int q = 0; q < 7; ++q)
+            if (pre || q != 3)
 #pragma unroll
-            for (int n = 1; n < 5; ++n) acc[q][n] += wv[q] * vv[n - 1];
+                for (int n = 1; n < 5; ++n) acc[q][n] += wv[q] * vv[n - 1];
         const int c = f - 3;
         if (c >= c0) {
             const long o = obase + (long)c * ost;
             double *const *dst = DIR == 2 ? a.Rz : a.Ry;
 #pragma unroll
-            for (int n = 0; n < 5; ++n) dst[n][o] = acc[0][n];
+            for (int n = 0; n < 5; ++n) dst[n][o] = acc[0][n] * se;
         }
 #pragma unroll
         for (int q = 0; q < 6; ++q)
@@ -335,6 +351,37 @@ __global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf
     }
 }
 
+// The compiler's own register budget: two waves per SIMD for FP64
+// component-wise and for FP32 faces characteristic-wise.
+template <int DIR, int INTERP, int RS, bool CHAR, class R>
+__global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf) {
+    pencil_m<DIR, INTERP, RS, CHAR, R>(a, cf);
+}
+
+// FP64 characteristic-wise: held to two waves per SIMD (256 registers; a few
+// values wait in scratch across the march, outside its loop) instead of one.
+template <int DIR, int INTERP, int RS, bool CHAR, class R>
+__global__ void __launch_bounds__(256, 2) k_pencil_m2(PencilArgs a, SchemeCoef<R> cf) {
+    pencil_m<DIR, INTERP, RS, CHAR, R>(a, cf);
+}
+
+// FP32 face arithmetic, component-wise: fits three waves per SIMD (168
+// registers) without spilling.
+template <int DIR, int INTERP, int RS, bool CHAR, class R>
+__global__ void __launch_bounds__(256, 3) k_pencil_m3(PencilArgs a, SchemeCoef<R> cf) {
+    pencil_m<DIR, INTERP, RS, CHAR, R>(a, cf);
+}
+
+template <int DIR, int INTERP, int RS, bool CHAR, class R>
+inline void launch_pencil_m(dim3 g, const PencilArgs &a, const SchemeCoef<R> &cf, hipStream_t s) {
+    if constexpr (sizeof(R) == 4 && !CHAR)
+        hipLaunchKernelGGL((k_pencil_m3<DIR, INTERP, RS, CHAR, R>), g, dim3(64, 4), 0, s, a, cf);
+    else if constexpr (sizeof(R) == 8 && CHAR)
+        hipLaunchKernelGGL((k_pencil_m2<DIR, INTERP, RS, CHAR, R>), g, dim3(64, 4), 0, s, a, cf);
+    else
+        hipLaunchKernelGGL((k_pencil_m<DIR, INTERP, RS, CHAR, R>), g, dim3(64, 4), 0, s, a, cf);
+}
+
 // ------------------------------------------------------------------ launch
 template <int INTERP, int RS, bool CHAR, class R>
 inline void launch_pencils_t(int dir, const PencilArgs &a, const SchemeCoef<R> &cf, hipStream_t s) {
@@ -344,17 +391,15 @@ inline void launch_pencils_t(int dir, const PencilArgs &a, const SchemeCoef<R> &
         az.seg = b.nz < PENCIL_SEG ? b.nz : PENCIL_SEG;
         az.ih = a.ihd[2];
         az.i60 = a.ihd[2] / 60.0;
-        hipLaunchKernelGGL((k_pencil_m<2, INTERP, RS, CHAR, R>),
-                           dim3((b.nx + 63) / 64, (b.ny + 3) / 4, (b.nz + az.seg - 1) / az.seg), dim3(64, 4), 0, s,
-                           az, cf);
+        launch_pencil_m<2, INTERP, RS, CHAR, R>(dim3((b.nx + 63) / 64, (b.ny + 3) / 4, (b.nz + az.seg - 1) / az.seg),
+                                                az, cf, s);
     } else if (dir == 1) {
         PencilArgs ay = a;
         ay.seg = b.ny < PENCIL_SEG ? b.ny : PENCIL_SEG;
         ay.ih = a.ihd[1];
         ay.i60 = a.ihd[1] / 60.0;
-        hipLaunchKernelGGL((k_pencil_m<1, INTERP, RS, CHAR, R>),
-                           dim3((b.nx + 63) / 64, (b.nz + 3) / 4, (b.ny + ay.seg - 1) / ay.seg), dim3(64, 4), 0, s,
-                           ay, cf);
+        launch_pencil_m<1, INTERP, RS, CHAR, R>(dim3((b.nx + 63) / 64, (b.nz + 3) / 4, (b.ny + ay.seg - 1) / ay.seg),
+                                                ay, cf, s);
     } else {
         // one thread per face of a line (N rounded up to 64 lanes, up to
         // 512); lines shorter than 256 share a 256-thread block; longer lines loop
