@@ -73,6 +73,18 @@ def parse():
     ap.add_argument("--sweep", type=int, default=1,
                     help="also time {WCNS5, WENO5-JS, MDCD} x {HLLC, Rusanov} on the same workload")
     ap.add_argument("--sweep-steps", type=int, default=5)
+    ap.add_argument("--scale-mixed", type=int, default=0,
+                    help="n of the at-scale mixed-vs-plain run (0: off; 512: Ma_t 0.3, Re 1600, 500 steps)")
+    ap.add_argument("--scale-mixed-steps", type=int, default=500)
+    ap.add_argument("--scale-adjoint", type=int, default=0,
+                    help="n of the at-scale adjoint run (0: off; 512: Ma_t 0.5, Re 1600, 200 steps, mixed "
+                         "forward, FP64 adjoint, J = time-integrated dissipation)")
+    ap.add_argument("--scale-adjoint-steps", type=int, default=200)
+    ap.add_argument("--scale-adjoint-snapshots", type=int, default=12,
+                    help="binomial checkpoint snapshots (12 x 5.6 GB at 512^3)")
+    ap.add_argument("--scale-adjoint-fd", type=int, default=4, help="FD directions")
+    ap.add_argument("--scale-adjoint-fp64", type=int, default=1,
+                    help="also the FP64-forward gradient of the same objective, for comparison")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU checker on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
@@ -276,6 +288,234 @@ def adjoint_report(args):
     }
 
 
+def progress(msg):
+    print(f"[bench] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
+
+
+def device_run(G, P, precision, prim, nsteps, label, chunk=50):
+    """nsteps CFL-controlled steps on the device; returns (field, ms per step)."""
+    import torch
+
+    from ns3d_wcns_amd import DeviceField
+
+    f = DeviceField(G, P, precision=precision)
+    stream = torch.cuda.current_stream()
+    f.set_stream(stream.cuda_stream)
+    f.initialize_primitive(*prim)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms, done = 0.0, 0
+    while done < nsteps:
+        k = min(chunk, nsteps - done)
+        e0.record(stream)
+        f.advance_steps(k)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms += e0.elapsed_time(e1)
+        done += k
+        progress(f"{label}: {done}/{nsteps} steps, {ms / done:.2f} ms/step")
+    return f, ms / nsteps
+
+
+def scale_mixed_report(args):
+    """Mixed vs plain at scale: isotropic --scale-mixed^3, Ma_t = 0.3,
+    Re = 1600, WCNS5 + HLLC, --scale-mixed-steps CFL-0.5 steps of each method
+    from the same state.  Errors of the mixed run against the FP64 run, all
+    on the device: norm-wise per conserved field, kinetic energy, TKE, the
+    spectral dissipation and E(k) per shell (hipFFT)."""
+    import torch
+
+    from ns3d_wcns_amd import PRECISION_FP64, PRECISION_MIXED, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
+    from ns3d_wcns_amd.adjoint import interior, state_tensor
+    from ns3d_wcns_amd.isotropic import isotropic_state
+
+    n, nsteps = args.scale_mixed, args.scale_mixed_steps
+    progress(f"scale-mixed: initial state {n}^3")
+    prim, ma = isotropic_state(n, mach_t=0.3, seed=12345)
+    P = SolverParams(Ma=ma, Re=1600.0, Pr=0.72, cfl=0.5, interpolation=SCHEME_ALIASES["wcns5"],
+                     riemann_solver=RiemannSolver.HLLC).derive()
+    G = periodic_box(n)
+    res = {}
+    for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+        f, ms = device_run(G, P, prec, prim, nsteps, f"scale-mixed {name}")
+        res[name] = {"ms": ms, "U": state_tensor(f), "stats": f.turbulence_statistics(),
+                     "spec": f.energy_spectrum(n // 2 + 1), "ke": f.total_kinetic_energy()}
+        f.close()
+    a, b = res["mixed"], res["fp64"]
+    fields = {}
+    for q, name in enumerate(("rho", "rhou", "rhov", "rhow", "E")):
+        x, y = interior(a["U"])[q], interior(b["U"])[q]
+        fields[name] = float(f"{float((x - y).abs().max() / y.abs().max()):.3e}")
+    rel = lambda x, y: float(f"{abs(x - y) / abs(y):.3e}")  # noqa: E731
+    ea, eb = a["spec"], b["spec"]
+    shells = [float(f"{abs(ea[k] - eb[k]) / eb[k]:.2e}") if eb[k] > 0 else 0.0 for k in range(1, n // 2 + 1)]
+    big = [e for k, e in enumerate(shells, start=1) if eb[k] >= 1e-6 * eb[1:].max()]
+    cells = n ** 3
+    return {
+        "workload": f"isotropic {n}^3, Ma_t=0.3, Re=1600, WCNS5+HLLC, {nsteps} RK3 steps at CFL 0.5, each method",
+        "fp64": {"ms_per_step": round(b["ms"], 3), "Mcell_steps_per_s": round(cells / b["ms"] / 1e3, 1)},
+        "mixed": {"ms_per_step": round(a["ms"], 3), "Mcell_steps_per_s": round(cells / a["ms"] / 1e3, 1)},
+        "speedup": round(b["ms"] / a["ms"], 3),
+        "error_vs_fp64": {
+            "fields": fields,
+            "kinetic_energy": rel(a["ke"], b["ke"]),
+            "tke": rel(a["stats"]["tke"], b["stats"]["tke"]),
+            "dissipation": rel(a["stats"]["dissipation"], b["stats"]["dissipation"]),
+            "spectrum_max_shell": max(big) if big else 0.0,
+            "spectrum_per_shell": shells,
+        },
+    }
+
+
+def scale_adjoint_report(args):
+    """Adjoint at scale: isotropic --scale-adjoint^3, Ma_t = 0.5, Re = 1600,
+    WCNS5 + HLLC, --scale-adjoint-steps steps at the CFL-0.5 time steps of a
+    first forward run.  Forward in the mixed FP32/FP64 method, discrete
+    adjoint in FP64 about its trajectory, binomial checkpointing with
+    --scale-adjoint-snapshots snapshots.  J = sum_n dt_n eps(U_{n+1}), the
+    time-integrated volume-mean viscous dissipation; the gradient with
+    respect to the initial velocity at fixed rho and p (dJ/du_c = rho
+    dJ/d(rho u_c) + rho u_c dJ/dE), checked by central differences of J
+    along --scale-adjoint-fd seeded unit-norm velocity directions.  The
+    forward's FP32 rounding makes J(U0 + eps d) noisy at the 1e-7 level, so
+    the differences use eps = 1e-4 (the first direction also 1e-3); the
+    gradient of the same objective with an FP64 forward is reported beside
+    it (--scale-adjoint-fp64 1), which separates the adjoint's accuracy
+    from the forward's rounding.  Costs: the adjoint step against the
+    forward step (per step, events) and the whole backward sweep against the
+    forward sweep."""
+    import torch
+
+    from ns3d_wcns_amd import PRECISION_MIXED, DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
+    from ns3d_wcns_amd.adjoint import (binomial_beta, dissipation_rate, interior, state_tensor, time_integral,
+                                       time_integral_value)
+    from ns3d_wcns_amd.isotropic import isotropic_state
+
+    n, nsteps, snaps = args.scale_adjoint, args.scale_adjoint_steps, args.scale_adjoint_snapshots
+    progress(f"scale-adjoint: initial state {n}^3")
+    prim, ma = isotropic_state(n, mach_t=0.5, seed=12345)
+    P = SolverParams(Ma=ma, Re=1600.0, Pr=0.72, cfl=0.5, interpolation=SCHEME_ALIASES["wcns5"],
+                     riemann_solver=RiemannSolver.HLLC).derive()
+    G = periodic_box(n)
+    f = DeviceField(G, P, precision=PRECISION_MIXED)
+    stream = torch.cuda.current_stream()
+    f.set_stream(stream.cuda_stream)
+    f.initialize_primitive(*prim)
+    del prim
+    U = state_tensor(f)
+    dts = []
+    for i in range(nsteps):
+        dts.append(f.compute_timestep())
+        f.step(dts[-1])
+        if (i + 1) % 50 == 0:
+            progress(f"scale-adjoint: time steps {i + 1}/{nsteps}")
+    density = lambda X: dissipation_rate(X, P, G)  # noqa: E731
+    policy = f"binomial:{snaps}"
+    stats = {}
+    U0 = U.clone().requires_grad_(True)
+    progress("scale-adjoint: forward sweep")
+    J = time_integral(U0, f, dts, density, policy, stats)
+    progress(f"scale-adjoint: forward {stats['forward_s']:.1f} s, backward sweep")
+    J.backward()
+    progress(f"scale-adjoint: backward {stats['backward_s']:.1f} s")
+    gU = U0.grad
+    U0 = None
+    # dJ/du_c = rho dJ/d(rho u_c) + rho u_c dJ/dE at fixed rho and p
+    I = interior(U)
+    rho = I[0]
+    gvel = torch.stack([rho * interior(gU)[1 + c] + I[1 + c] * interior(gU)[4] for c in range(3)])
+    del gU
+    # per-step costs
+    ptr = lambda t: [t[q].data_ptr() for q in range(5)]  # noqa: E731
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    f.set_state(ptr(U))
+    e0.record(stream)
+    for i in range(5):
+        f.step(dts[i])
+    e1.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / 5
+    lam = torch.zeros_like(U)
+    lam[:, 3:-3, 3:-3, 3:-3] = 1e-3
+    f.set_state(ptr(U))
+    e0.record(stream)
+    for i in range(5):
+        f.adjoint_step(dts[i], ptr(lam))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    adj_ms = e0.elapsed_time(e1) / 5
+    del lam
+    # central differences along velocity directions (rho, p fixed)
+    g = torch.Generator(device="cpu").manual_seed(99)
+    gam = P.gamma
+    checks = []
+    for k in range(args.scale_adjoint_fd):
+        d = torch.randn((3, n, n, n), generator=g, dtype=torch.float64).cuda()
+        d /= torch.linalg.vector_norm(d)
+        ad = float((gvel * d).sum())
+        row = {"adjoint": ad}
+        for eps in ((1e-4, 1e-3) if k == 0 else (1e-4,)):
+            vals = []
+            for sgn in (1.0, -1.0):
+                V = U.clone()
+                Iv = interior(V)
+                vel = [I[1 + c] / rho + sgn * eps * d[c] for c in range(3)]
+                p = (gam - 1.0) * (I[4] - 0.5 * (I[1] ** 2 + I[2] ** 2 + I[3] ** 2) / rho)
+                for c in range(3):
+                    Iv[1 + c] = rho * vel[c]
+                Iv[4] = p / (gam - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+                vals.append(time_integral_value(f, V, dts, density))
+                del V, Iv, vel, p
+            fd = (vals[0] - vals[1]) / (2 * eps)
+            row[f"fd_eps_{eps:g}"] = fd
+            row[f"rel_err_eps_{eps:g}"] = float(f"{abs(fd - ad) / abs(ad):.3e}")
+        checks.append(row)
+        progress(f"scale-adjoint: FD direction {k}: {row}")
+    f.close()
+    fp64_cmp = None
+    if args.scale_adjoint_fp64:
+        from ns3d_wcns_amd import PRECISION_FP64
+
+        progress("scale-adjoint: FP64 forward + adjoint of the same objective")
+        f64 = DeviceField(G, P, precision=PRECISION_FP64)
+        f64.set_stream(stream.cuda_stream)
+        U0 = U.clone().requires_grad_(True)
+        st64 = {}
+        J64 = time_integral(U0, f64, dts, density, policy, st64)
+        J64.backward()
+        g64 = torch.stack([rho * interior(U0.grad)[1 + c] + I[1 + c] * interior(U0.grad)[4] for c in range(3)])
+        U0 = None
+        f64.close()
+        fp64_cmp = {
+            "J_fp64_forward": float(J64.detach()),
+            "J_rel_diff": float(f"{abs(float(J.detach()) - float(J64.detach())) / abs(float(J64.detach())):.3e}"),
+            "grad_rel_diff_norm": float(f"{float(torch.linalg.vector_norm(gvel - g64) / torch.linalg.vector_norm(g64)):.3e}"),
+            "grad_rel_diff_max": float(f"{float((gvel - g64).abs().max() / g64.abs().max()):.3e}"),
+            "forward_sweep_s": round(st64["forward_s"], 2),
+            "backward_sweep_s": round(st64["backward_s"], 2),
+        }
+        del g64
+    t = 1
+    while binomial_beta(snaps, t) < nsteps:
+        t += 1
+    return {
+        "workload": f"isotropic {n}^3, Ma_t=0.5, Re=1600, WCNS5+HLLC, {nsteps} RK3 steps at CFL 0.5; mixed forward, "
+                    "FP64 adjoint; J = sum_n dt_n <tau:grad u>(U_n+1); gradient w.r.t. initial velocity (rho, p fixed)",
+        "checkpoint": policy,
+        "checkpoint_GB": round(snaps * 5 * (n + 6) ** 3 * 8 / 1e9, 1),
+        "binomial_sweeps_bound": t,
+        "J": float(J.detach()),
+        "forward_step_ms": round(fwd_ms, 3),
+        "adjoint_step_ms": round(adj_ms, 3),
+        "step_cost_ratio": round(adj_ms / fwd_ms, 3),
+        "forward_sweep_s": round(stats["forward_s"], 2),
+        "backward_sweep_s": round(stats["backward_s"], 2),
+        "sweep_cost_ratio": round(stats["backward_s"] / stats["forward_s"], 3),
+        "recomputed_steps": stats["recomputed_steps"],
+        "fd": checks,
+        "vs_fp64_forward": fp64_cmp,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -382,6 +622,10 @@ def main():
         out["sweep"] = sweep_report(args, prim, ma, G, stream)
     if args.adjoint:
         out["adjoint"] = adjoint_report(args)
+    if args.scale_mixed:
+        out["mixed_scale"] = scale_mixed_report(args)
+    if args.scale_adjoint:
+        out["adjoint_scale"] = scale_adjoint_report(args)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prim, G, P, args.cpu_threads)
     print(json.dumps(out))

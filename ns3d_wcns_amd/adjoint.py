@@ -19,6 +19,15 @@ of the map U0 -> U_N at the given dt sequence (for a CFL-controlled run,
 record the dt sequence of the forward run and pass it).  Gradients reach
 interior cells only; ghost entries of the input are ignored by the forward
 (periodic ghosts are refilled) and get zero gradient.
+
+``time_integral(U0, field, dts, density, checkpoint=...)`` is the scalar
+J = sum_n dt_n density(U_{n+1}) over the same trajectory (the right end
+point of every step), e.g. the time-integrated viscous dissipation with
+``density = dissipation_rate``.  Its backward adds the source
+dt_{n-1} d density / dU_n to the adjoint after each reverse step; the
+density's own gradient comes from torch autograd on the state tensor.
+The forward may run the mixed FP32/FP64 method (the field's precision); the
+adjoint is then the FP64 linearisation about that trajectory.
 """
 from __future__ import annotations
 
@@ -60,12 +69,15 @@ def binomial_beta(s: int, t: int) -> int:
 
 
 class _Sweep:
-    """State handling of one reverse sweep on a DeviceField."""
+    """State handling of one reverse sweep on a DeviceField.  source(U, i),
+    when given, is added to the adjoint after the reverse of step i (the
+    objective's gradient at the state U = U_i, for i >= 1)."""
 
-    def __init__(self, field: DeviceField, dts, stats: dict):
+    def __init__(self, field: DeviceField, dts, stats: dict, source=None):
         self.field = field
         self.dts = [float(d) for d in dts]
         self.stats = stats
+        self.source = source
 
     def load(self, U: torch.Tensor):
         self.field.set_state(_ptrs(U))
@@ -85,6 +97,8 @@ class _Sweep:
         self.load(U)
         self.field.adjoint_step(self.dts[i], _ptrs(adj))
         self.stats["adjoint_steps"] += 1
+        if self.source is not None and i >= 1:
+            adj += self.source(U, i)
 
     def reverse_binomial(self, i: int, n: int, s: int, Ui: torch.Tensor, adj: torch.Tensor):
         """Reverse steps i..i+n-1 from the state Ui at step i with s free
@@ -114,6 +128,35 @@ class _Sweep:
         del Um
         self.stats["live"] -= 1
         self.reverse_binomial(i, m, s, Ui, adj)
+
+
+def _reverse(sweep: _Sweep, kind: str, arg: int, saved: dict, n: int, adj: torch.Tensor):
+    """The reverse sweep over steps n-1..0 under a checkpoint policy; adj
+    holds the adjoint of U_n on entry and of U_0 on return."""
+    if kind == "all":
+        for i in range(n - 1, -1, -1):
+            sweep.adjoint(i, saved[i], adj)
+    elif kind == "uniform":
+        starts = list(range(0, n, arg))
+        for s0 in reversed(starts):
+            s1 = min(s0 + arg, n)
+            states = [saved[s0]]
+            sweep.load(saved[s0])
+            for i in range(s0, s1 - 1):
+                sweep.advance(i, 1)
+                states.append(sweep.save())
+            for i in range(s1 - 1, s0 - 1, -1):
+                sweep.adjoint(i, states[i - s0], adj)
+    else:
+        sweep.reverse_binomial(0, n, arg, saved[0], adj)
+
+
+def _interior_only(g: torch.Tensor) -> torch.Tensor:
+    adj = g.detach().contiguous().clone()
+    ghost = torch.ones_like(adj, dtype=torch.bool)
+    ghost[:, NG:-NG, NG:-NG, NG:-NG] = False
+    adj[ghost] = 0.0
+    return adj
 
 
 class RK3Rollout(torch.autograd.Function):
@@ -147,30 +190,90 @@ class RK3Rollout(torch.autograd.Function):
         sweep, kind, arg, saved, n = ctx.sweep, ctx.kind, ctx.arg, ctx.saved, ctx.n
         sweep.field.set_stream(torch.cuda.current_stream().cuda_stream)
         t0 = time.perf_counter()
-        adj = gU.detach().contiguous().clone()
-        # only interior cells carry gradient
-        ghost = torch.ones_like(adj, dtype=torch.bool)
-        ghost[:, NG:-NG, NG:-NG, NG:-NG] = False
-        adj[ghost] = 0.0
-        if kind == "all":
-            for i in range(n - 1, -1, -1):
-                sweep.adjoint(i, saved[i], adj)
-        elif kind == "uniform":
-            starts = list(range(0, n, arg))
-            for s0 in reversed(starts):
-                s1 = min(s0 + arg, n)
-                states = [saved[s0]]
-                sweep.load(saved[s0])
-                for i in range(s0, s1 - 1):
-                    sweep.advance(i, 1)
-                    states.append(sweep.save())
-                for i in range(s1 - 1, s0 - 1, -1):
-                    sweep.adjoint(i, states[i - s0], adj)
-        else:
-            sweep.reverse_binomial(0, n, arg, saved[0], adj)
+        adj = _interior_only(gU)  # only interior cells carry gradient
+        _reverse(sweep, kind, arg, saved, n, adj)
         torch.cuda.synchronize()
         sweep.stats["backward_s"] = time.perf_counter() - t0
         return adj, None, None, None, None
+
+
+class RK3TimeIntegral(torch.autograd.Function):
+    """J = sum_n dt_n density(U_{n+1}) over fixed time steps, with the
+    discrete adjoint (plus the density's gradient at every state) as
+    backward."""
+
+    @staticmethod
+    def forward(ctx, U0, field: DeviceField, dts, density, checkpoint: str, stats: dict | None):
+        _check_state(U0, field)
+        n = len(dts)
+        kind, arg = parse_checkpoint(checkpoint, n)
+        field.set_stream(torch.cuda.current_stream().cuda_stream)
+        stats = stats if stats is not None else {}
+
+        def source(U, i):
+            with torch.enable_grad():
+                X = U.detach().requires_grad_(True)
+                (g,) = torch.autograd.grad(density(X), X)
+            return (sweep.dts[i - 1] * gJ_scale[0]) * g
+
+        gJ_scale = [1.0]
+        sweep = _Sweep(field, dts, stats, source)
+        for k in ("recomputed_steps", "adjoint_steps", "peak_snapshots", "live"):
+            sweep.stats.setdefault(k, 0)
+        t0 = time.perf_counter()
+        saved = {0: U0.detach().clone()}
+        sweep.load(saved[0])
+        J = torch.zeros((), dtype=torch.float64, device=U0.device)
+        for i in range(n):
+            field.step(sweep.dts[i])
+            Ui = sweep.save()
+            with torch.no_grad():
+                J += sweep.dts[i] * density(Ui)
+            if kind == "all" or (kind == "uniform" and (i + 1) % arg == 0 and i + 1 < n):
+                saved[i + 1] = Ui
+            del Ui
+        last = sweep.save()
+        torch.cuda.synchronize()
+        sweep.stats["forward_s"] = time.perf_counter() - t0
+        sweep.stats["stored_states"] = len(saved)
+        ctx.sweep, ctx.kind, ctx.arg, ctx.saved, ctx.n, ctx.last = sweep, kind, arg, saved, n, last
+        ctx.scale = gJ_scale
+        return J
+
+    @staticmethod
+    def backward(ctx, gJ):
+        sweep, n = ctx.sweep, ctx.n
+        sweep.field.set_stream(torch.cuda.current_stream().cuda_stream)
+        t0 = time.perf_counter()
+        ctx.scale[0] = float(gJ)
+        # adjoint of U_N: the last step's source
+        adj = torch.zeros_like(ctx.last)
+        adj += sweep.source(ctx.last, n)
+        _reverse(sweep, ctx.kind, ctx.arg, ctx.saved, n, adj)
+        torch.cuda.synchronize()
+        sweep.stats["backward_s"] = time.perf_counter() - t0
+        return adj, None, None, None, None, None
+
+
+def time_integral(U0: torch.Tensor, field: DeviceField, dts, density, checkpoint: str = "uniform:10",
+                  stats: dict | None = None):
+    """Differentiable J = sum_n dt_n density(U_{n+1}) over the rollout (see
+    the module docstring)."""
+    return RK3TimeIntegral.apply(U0, field, list(dts), density, checkpoint, stats)
+
+
+def time_integral_value(field: DeviceField, U0: torch.Tensor, dts, density) -> float:
+    """J = sum_n dt_n density(U_{n+1}) without checkpoints (forward only)."""
+    with torch.no_grad():
+        field.set_stream(torch.cuda.current_stream().cuda_stream)
+        field.set_state(_ptrs(U0.contiguous()))
+        U = torch.empty_like(U0)
+        J = 0.0
+        for dt in dts:
+            field.step(float(dt))
+            field.get_state(_ptrs(U))
+            J += float(dt) * float(density(U))
+        return J
 
 
 def rollout(U0: torch.Tensor, field: DeviceField, dts, checkpoint: str = "uniform:10", stats: dict | None = None):
@@ -186,6 +289,37 @@ def mean_kinetic_energy(U: torch.Tensor) -> torch.Tensor:
     """J = mean over interior cells of 0.5 rho |u|^2 = 0.5 |rho u|^2 / rho."""
     I = interior(U)
     return (0.5 * (I[1] * I[1] + I[2] * I[2] + I[3] * I[3]) / I[0]).mean()
+
+
+def d6(f: torch.Tensor, axis: int, h: float) -> torch.Tensor:
+    """6th-order central first derivative on the periodic interior along a
+    tensor axis (compute_gradients' periodic stencil, ViscousFlux.cpp:27-29)."""
+    r = lambda k: torch.roll(f, -k, axis)  # noqa: E731  r(k)[i] = f[i + k]
+    return (r(3) - 9.0 * r(2) + 45.0 * r(1) - 45.0 * r(-1) + 9.0 * r(-2) - r(-3)) / (60.0 * h)
+
+
+def dissipation_rate(U: torch.Tensor, params, grid) -> torch.Tensor:
+    """Volume mean over the interior of the viscous dissipation tau_ij
+    du_i/dx_j, with the solver's ingredients: u = rho u / rho,
+    p = (gamma - 1)(E - rho |u|^2 / 2), T = p / (rho R), Sutherland mu(T)
+    (field_structures.h:96-106), tau_ij = mu (du_i/dx_j + du_j/dx_i) -
+    2/3 mu div(u) delta_ij (ViscousFlux.cpp:151-208) and 6th-order central
+    gradients on the periodic box."""
+    I = interior(U)
+    rho = I[0]
+    vel = [I[1] / rho, I[2] / rho, I[3] / rho]
+    p = (params.gamma - 1.0) * (I[4] - 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
+    T = p / (rho * params.Rgas)
+    mu = params.mu * T ** 1.5 * (1.0 + params.S_ref) / (T + params.S_ref)
+    h = (grid.dx, grid.dy, grid.dz)
+    axis = (2, 1, 0)  # x, y, z of a (nz, ny, nx) array
+    g = [[d6(vel[i], axis[j], h[j]) for j in range(3)] for i in range(3)]
+    div = g[0][0] + g[1][1] + g[2][2]
+    eps = -2.0 / 3.0 * mu * div * div
+    for i in range(3):
+        for j in range(3):
+            eps = eps + mu * (g[i][j] + g[j][i]) * g[i][j]
+    return eps.mean()
 
 
 def kinetic_energy_density(U: torch.Tensor) -> torch.Tensor:

@@ -335,7 +335,8 @@ __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
     }
 }
 
-// The x direction, one block per x line (threads = cells, nx <= 256): the
+// The x direction, one block per x line (threads = cells, nx <= 512; 82 KB of
+// LDS at 512, within gfx950's 160 KB per workgroup): the
 // line and its 2-cell periodic halo in LDS; both reconstructions of every
 // cell; the reverse of the Riemann flux of every face (state adjoints
 // written over the states they replace: each slot is read and written by
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
 // (transposed gradient stencil and the reverse of the conserved ->
 // primitive map), added to out.
 template <bool JS, int RS>
-__global__ void __launch_bounds__(256) k_adj_xline(AdjArgs a) {
+__global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a) {
     extern __shared__ double lds[];
     const Box &b = a.b;
     const int N = b.nx, W = N + 4;
@@ -567,7 +568,7 @@ bool adjoint_supported(int interp, int rs, bool chr) {
     return !chr && (interp == I_WENO5 || interp == I_WENO5_JS) && (rs == R_HLLC || rs == R_RUSANOV);
 }
 
-int adjoint_max_nx() { return 256; }  // 5 (nx + 4) + 15 nx doubles of LDS per block
+int adjoint_max_nx() { return 512; }  // 5 (nx + 4) + 15 nx doubles of LDS per block
 
 void launch_tl_prim(const AdjArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_tl_prim, cgrid(a.b), dim3(256), 0, s, a);
