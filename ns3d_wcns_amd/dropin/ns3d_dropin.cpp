@@ -54,7 +54,19 @@ ns3d_params to_c(const SolverParams &P) {
 
 // The field's mirror with the given spacing (nullptr: keep the current one
 // or, for a new mirror, use unit spacing until a call supplies it).
+bool all_periodic(const SolverParams &P) {
+    using BC = SolverParams::BCType;
+    return P.bc_xmin == BC::Periodic && P.bc_xmax == BC::Periodic && P.bc_ymin == BC::Periodic &&
+           P.bc_ymax == BC::Periodic && P.bc_zmin == BC::Periodic && P.bc_zmax == BC::Periodic;
+}
+
 ns3d_ctx *mirror(Field3D &F, const SolverParams *P, const GridDesc *G) {
+    if (P && !all_periodic(*P)) {
+        // the device path is the one-rank periodic box: refuse rather than
+        // silently treat a Wall / Symmetry / Outflow / Inflow face as periodic
+        std::cerr << "ns3d: only periodic boundaries are supported (bc_xmin .. bc_zmax must all be Periodic)\n";
+        return nullptr;
+    }
     if (!F.dev) F.dev = new ns3d_mirror();
     ns3d_mirror &m = *F.dev;
     const bool regrid = G && (!m.grid_known || m.dx != G->dx || m.dy != G->dy || m.dz != G->dz);

@@ -137,6 +137,10 @@ int main(int argc, char **argv) {
     }
     Field3D F;
     F.allocate(make_local_desc(G, P));
+    if (!(C.periods[0] && C.periods[1] && C.periods[2]) && std::strcmp(argv[2], "--convert") != 0) {
+        std::cerr << "ns3d: only periodic boundaries are supported (bc_xmin .. bc_zmax must all be Periodic)\n";
+        return 1;
+    }
     if (std::strcmp(argv[2], "--convert") == 0) {
         // host only: no hot-path function is called
         if (argc < 5) {

@@ -141,6 +141,11 @@ class SolverParams:
         return self
 
     def to_c(self) -> CParams:
+        """The C-ABI parameter block.  The device path is the one-rank
+        periodic box: any other boundary type is refused here."""
+        if any(b != BCType.Periodic for b in self.bc):
+            raise ValueError("only periodic boundaries are supported on the device path "
+                             f"(got {[b.name for b in self.bc]})")
         return CParams(
             self.gamma, self.Pr, self.Ma, self.Re, self.cfl, self.mu, self.S_ref, self.Cv, self.Cp,
             self.Rgas, self.mdcd_diss, self.mdcd_disp, int(self.interpolation), int(self.riemann_solver),

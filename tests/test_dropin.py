@@ -157,3 +157,23 @@ def test_dropin_time_advance_matches_checker(gpu, tmp_path, mode):
     ke = ref.total_kinetic()
     assert abs(energy[-1] - ke) <= 1e-12 * abs(ke), (energy[-1], ke)
     ref.close()
+
+
+def test_dropin_refuses_non_periodic_boundaries(tmp_path):
+    """A Wall / Symmetry / Outflow / Inflow face is refused with a message
+    (no silent periodic treatment), before any device work."""
+    f = tmp_path / "solver.in"
+    f.write_text("global_nx = 8\nglobal_ny = 8\nglobal_nz = 8\nbc_xmin = Wall\n")
+    r = subprocess.run([DRIVER, str(f), "-", str(tmp_path / "final.bin")], capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "only periodic boundaries are supported" in r.stderr
+
+
+def test_python_params_refuse_non_periodic_boundaries():
+    from ns3d_wcns_amd import SolverParams
+    from ns3d_wcns_amd.params import BCType
+
+    P = SolverParams().derive()
+    P.bc[3] = BCType.Outflow
+    with pytest.raises(ValueError, match="only periodic"):
+        P.to_c()
