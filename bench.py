@@ -182,7 +182,9 @@ def adjoint_report(args):
     the forward run, checkpoint every 10 steps.  J = mean 0.5 rho |u|^2 at the
     final time; gradient with respect to the initial conserved fields.
     Reports the backward/forward cost ratio of the whole sweep (recomputation
-    included) and per step, central-difference checks (eps = 1e-6) on 8
+    included; the uniform policy keeps the stage states of the recomputed
+    segment) and per step (adjoint step with its stages recomputed, and
+    with them stored), central-difference checks (eps = 1e-6) on 8
     seeded unit-norm directions in (rho u, rho v, rho w, E), and the
     tangent/adjoint dot test of one step."""
     import torch
@@ -234,6 +236,19 @@ def adjoint_report(args):
     e1.record()
     torch.cuda.synchronize()
     adj_ms = e0.elapsed_time(e1) / 10
+    # the same with the stage states stored by the recomputation (what the
+    # uniform checkpoint sweep does)
+    S1, S2 = torch.empty_like(U), torch.empty_like(U)
+    f.set_state(Up)
+    f.step_stages(dts[0], [S1[q].data_ptr() for q in range(5)], [S2[q].data_ptr() for q in range(5)])
+    torch.cuda.synchronize()
+    e0.record()
+    for i in range(10):
+        f.adjoint_step_stages(dts[0], ptr, [S1[q].data_ptr() for q in range(5)], [S2[q].data_ptr() for q in range(5)])
+    e1.record()
+    torch.cuda.synchronize()
+    adj_stored_ms = e0.elapsed_time(e1) / 10
+    del S1, S2
     # central differences
     g = torch.Generator(device="cpu").manual_seed(7)
     worst, checks, cs_worst, wide = 0.0, [], 0.0, []
@@ -278,6 +293,8 @@ def adjoint_report(args):
         "forward_step_ms": round(fwd_ms, 4),
         "adjoint_step_ms": round(adj_ms, 4),
         "step_cost_ratio": round(adj_ms / fwd_ms, 3),
+        "adjoint_step_stored_stages_ms": round(adj_stored_ms, 4),
+        "step_cost_ratio_stored_stages": round(adj_stored_ms / fwd_ms, 3),
         "recomputed_steps": stats["recomputed_steps"],
         "fd_eps": eps,
         "fd_rel_err": checks,
@@ -372,22 +389,22 @@ def scale_adjoint_report(args):
     first forward run.  Forward in the mixed FP32/FP64 method, discrete
     adjoint in FP64 about its trajectory, binomial checkpointing with
     --scale-adjoint-snapshots snapshots.  J = sum_n dt_n eps(U_{n+1}), the
-    time-integrated volume-mean viscous dissipation; the gradient with
-    respect to the initial velocity at fixed rho and p (dJ/du_c = rho
-    dJ/d(rho u_c) + rho u_c dJ/dE), checked by central differences of J
-    along --scale-adjoint-fd seeded unit-norm velocity directions.  The
-    forward's FP32 rounding makes J(U0 + eps d) noisy at the 1e-7 level, so
-    the differences use eps = 1e-4 (the first direction also 1e-3); the
-    gradient of the same objective with an FP64 forward is reported beside
-    it (--scale-adjoint-fp64 1), which separates the adjoint's accuracy
-    from the forward's rounding.  Costs: the adjoint step against the
-    forward step (per step, events) and the whole backward sweep against the
-    forward sweep."""
+    time-integrated volume-mean viscous dissipation (its gradient formed
+    slab by slab); the gradient with respect to the initial velocity at
+    fixed rho and p (dJ/du_c = rho dJ/d(rho u_c) + rho u_c dJ/dE), checked
+    by central differences of J along --scale-adjoint-fd seeded velocity
+    directions of unit RMS per cell.  The forward's FP32 rounding makes J
+    noisy at about 1e-7 relative, so the differences use eps = 1e-4 (the
+    first direction also 1e-3, where the O(eps^2) truncation shows); the gradient of the same objective with an
+    FP64 forward is reported beside it (--scale-adjoint-fp64 1), which
+    separates the adjoint's accuracy from the forward's rounding.  Costs:
+    the adjoint step against the forward step (per step, events) and the
+    whole backward sweep against the forward sweep."""
     import torch
 
     from ns3d_wcns_amd import PRECISION_MIXED, DeviceField, RiemannSolver, SCHEME_ALIASES, SolverParams, periodic_box
-    from ns3d_wcns_amd.adjoint import (binomial_beta, dissipation_rate, interior, state_tensor, time_integral,
-                                       time_integral_value)
+    from ns3d_wcns_amd.adjoint import (binomial_beta, dissipation_rate, dissipation_rate_grad, interior,
+                                       state_tensor, time_integral, time_integral_value)
     from ns3d_wcns_amd.isotropic import isotropic_state
 
     n, nsteps, snaps = args.scale_adjoint, args.scale_adjoint_steps, args.scale_adjoint_snapshots
@@ -409,21 +426,26 @@ def scale_adjoint_report(args):
         if (i + 1) % 50 == 0:
             progress(f"scale-adjoint: time steps {i + 1}/{nsteps}")
     density = lambda X: dissipation_rate(X, P, G)  # noqa: E731
+    density_grad = lambda X: dissipation_rate_grad(X, P, G)  # noqa: E731
     policy = f"binomial:{snaps}"
-    stats = {}
-    U0 = U.clone().requires_grad_(True)
-    progress("scale-adjoint: forward sweep")
-    J = time_integral(U0, f, dts, density, policy, stats)
-    progress(f"scale-adjoint: forward {stats['forward_s']:.1f} s, backward sweep")
-    J.backward()
-    progress(f"scale-adjoint: backward {stats['backward_s']:.1f} s")
-    gU = U0.grad
-    U0 = None
-    # dJ/du_c = rho dJ/d(rho u_c) + rho u_c dJ/dE at fixed rho and p
-    I = interior(U)
-    rho = I[0]
-    gvel = torch.stack([rho * interior(gU)[1 + c] + I[1 + c] * interior(gU)[4] for c in range(3)])
-    del gU
+
+    def gradient(field, label):
+        st = {}
+        U0 = U.clone().requires_grad_(True)
+        progress(f"scale-adjoint: {label} forward sweep")
+        J = time_integral(U0, field, dts, density, policy, st, density_grad=density_grad)
+        progress(f"scale-adjoint: {label} forward {st['forward_s']:.1f} s, backward sweep")
+        J.backward()
+        progress(f"scale-adjoint: {label} backward {st['backward_s']:.1f} s")
+        Jv, gU = float(J.detach()), U0.grad
+        del J, U0
+        I = interior(U)
+        gv = torch.stack([I[0] * interior(gU)[1 + c] + I[1 + c] * interior(gU)[4] for c in range(3)])
+        del gU
+        torch.cuda.empty_cache()
+        return Jv, gv, st
+
+    J, gvel, stats = gradient(f, "mixed")
     # per-step costs
     ptr = lambda t: [t[q].data_ptr() for q in range(5)]  # noqa: E731
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -444,13 +466,15 @@ def scale_adjoint_report(args):
     torch.cuda.synchronize()
     adj_ms = e0.elapsed_time(e1) / 5
     del lam
-    # central differences along velocity directions (rho, p fixed)
+    progress(f"scale-adjoint: forward step {fwd_ms:.2f} ms, adjoint step {adj_ms:.2f} ms")
+    # central differences along velocity directions (rho, p fixed), unit RMS per cell
     g = torch.Generator(device="cpu").manual_seed(99)
     gam = P.gamma
+    I = interior(U)
+    rho = I[0]
     checks = []
     for k in range(args.scale_adjoint_fd):
         d = torch.randn((3, n, n, n), generator=g, dtype=torch.float64).cuda()
-        d /= torch.linalg.vector_norm(d)
         ad = float((gvel * d).sum())
         row = {"adjoint": ad}
         for eps in ((1e-4, 1e-3) if k == 0 else (1e-4,)):
@@ -463,31 +487,28 @@ def scale_adjoint_report(args):
                 for c in range(3):
                     Iv[1 + c] = rho * vel[c]
                 Iv[4] = p / (gam - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+                del vel, p
                 vals.append(time_integral_value(f, V, dts, density))
-                del V, Iv, vel, p
+                del V, Iv
             fd = (vals[0] - vals[1]) / (2 * eps)
             row[f"fd_eps_{eps:g}"] = fd
             row[f"rel_err_eps_{eps:g}"] = float(f"{abs(fd - ad) / abs(ad):.3e}")
         checks.append(row)
         progress(f"scale-adjoint: FD direction {k}: {row}")
+        del d
     f.close()
+    torch.cuda.empty_cache()
     fp64_cmp = None
     if args.scale_adjoint_fp64:
         from ns3d_wcns_amd import PRECISION_FP64
 
-        progress("scale-adjoint: FP64 forward + adjoint of the same objective")
         f64 = DeviceField(G, P, precision=PRECISION_FP64)
         f64.set_stream(stream.cuda_stream)
-        U0 = U.clone().requires_grad_(True)
-        st64 = {}
-        J64 = time_integral(U0, f64, dts, density, policy, st64)
-        J64.backward()
-        g64 = torch.stack([rho * interior(U0.grad)[1 + c] + I[1 + c] * interior(U0.grad)[4] for c in range(3)])
-        U0 = None
+        J64, g64, st64 = gradient(f64, "FP64")
         f64.close()
         fp64_cmp = {
-            "J_fp64_forward": float(J64.detach()),
-            "J_rel_diff": float(f"{abs(float(J.detach()) - float(J64.detach())) / abs(float(J64.detach())):.3e}"),
+            "J_fp64_forward": J64,
+            "J_rel_diff": float(f"{abs(J - J64) / abs(J64):.3e}"),
             "grad_rel_diff_norm": float(f"{float(torch.linalg.vector_norm(gvel - g64) / torch.linalg.vector_norm(g64)):.3e}"),
             "grad_rel_diff_max": float(f"{float((gvel - g64).abs().max() / g64.abs().max()):.3e}"),
             "forward_sweep_s": round(st64["forward_s"], 2),
@@ -503,7 +524,7 @@ def scale_adjoint_report(args):
         "checkpoint": policy,
         "checkpoint_GB": round(snaps * 5 * (n + 6) ** 3 * 8 / 1e9, 1),
         "binomial_sweeps_bound": t,
-        "J": float(J.detach()),
+        "J": J,
         "forward_step_ms": round(fwd_ms, 3),
         "adjoint_step_ms": round(adj_ms, 3),
         "step_cost_ratio": round(adj_ms / fwd_ms, 3),
@@ -511,6 +532,7 @@ def scale_adjoint_report(args):
         "backward_sweep_s": round(stats["backward_s"], 2),
         "sweep_cost_ratio": round(stats["backward_s"] / stats["forward_s"], 3),
         "recomputed_steps": stats["recomputed_steps"],
+        "fd_direction": "velocity, unit RMS per cell",
         "fd": checks,
         "vs_fp64_forward": fp64_cmp,
     }

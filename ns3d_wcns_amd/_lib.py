@@ -109,6 +109,8 @@ _SIGS = {
     "ns3d_adjoint_rhs": [_VP, _VP5, _VP5],
     "ns3d_tangent_step": [_VP, ctypes.c_double, _VP5],
     "ns3d_adjoint_step": [_VP, ctypes.c_double, _VP5],
+    "ns3d_step_stages": [_VP, ctypes.c_double, _VP5, _VP5],
+    "ns3d_adjoint_step_stages": [_VP, ctypes.c_double, _VP5, _VP5, _VP5],
     "ns3d_check_face_tangent": [ctypes.POINTER(CParams), _DP, _DP, _DP, _DP],
     "ns3d_check_face_adjoint": [ctypes.POINTER(CParams), _DP, _DP, _DP],
     "ns3d_check_visc_tangent": [ctypes.POINTER(CParams), _DP, _DP, _DP, _DP, _DP, _DP],

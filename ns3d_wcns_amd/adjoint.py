@@ -10,7 +10,8 @@ needs from checkpoints:
 
 - ``"all"``: every state kept;
 - ``"uniform:K"``: every K-th state kept; each segment is recomputed once,
-  its K states kept during its reverse sweep;
+  its K states and (keep_stages, the default) their RK3 stage states kept
+  during its reverse sweep, so that the adjoint steps recompute nothing;
 - ``"binomial:S"``: S snapshots placed by the binomial (revolve) schedule,
   for a fixed memory budget.
 
@@ -73,11 +74,12 @@ class _Sweep:
     when given, is added to the adjoint after the reverse of step i (the
     objective's gradient at the state U = U_i, for i >= 1)."""
 
-    def __init__(self, field: DeviceField, dts, stats: dict, source=None):
+    def __init__(self, field: DeviceField, dts, stats: dict, source=None, keep_stages: bool = True):
         self.field = field
         self.dts = [float(d) for d in dts]
         self.stats = stats
         self.source = source
+        self.keep_stages = keep_stages
 
     def load(self, U: torch.Tensor):
         self.field.set_state(_ptrs(U))
@@ -93,9 +95,20 @@ class _Sweep:
             self.field.step(self.dts[j])
         self.stats["recomputed_steps"] += m
 
-    def adjoint(self, i: int, U: torch.Tensor, adj: torch.Tensor):
+    def advance_stages(self, i: int):
+        """Step i from the loaded state, keeping its RK3 stage states."""
+        U1 = torch.empty((5,) + self.field.shape, dtype=torch.float64, device="cuda")
+        U2 = torch.empty_like(U1)
+        self.field.step_stages(self.dts[i], _ptrs(U1), _ptrs(U2))
+        self.stats["recomputed_steps"] += 1
+        return U1, U2
+
+    def adjoint(self, i: int, U: torch.Tensor, adj: torch.Tensor, stages=None):
         self.load(U)
-        self.field.adjoint_step(self.dts[i], _ptrs(adj))
+        if stages is None:
+            self.field.adjoint_step(self.dts[i], _ptrs(adj))
+        else:
+            self.field.adjoint_step_stages(self.dts[i], _ptrs(adj), _ptrs(stages[0]), _ptrs(stages[1]))
         self.stats["adjoint_steps"] += 1
         if self.source is not None and i >= 1:
             adj += self.source(U, i)
@@ -140,13 +153,21 @@ def _reverse(sweep: _Sweep, kind: str, arg: int, saved: dict, n: int, adj: torch
         starts = list(range(0, n, arg))
         for s0 in reversed(starts):
             s1 = min(s0 + arg, n)
-            states = [saved[s0]]
+            states, stages = [saved[s0]], []
             sweep.load(saved[s0])
-            for i in range(s0, s1 - 1):
-                sweep.advance(i, 1)
-                states.append(sweep.save())
+            if sweep.keep_stages:
+                # every step of the segment once, its two stage states kept:
+                # the adjoint steps then recompute nothing
+                for i in range(s0, s1):
+                    stages.append(sweep.advance_stages(i))
+                    if i < s1 - 1:
+                        states.append(sweep.save())
+            else:
+                for i in range(s0, s1 - 1):
+                    sweep.advance(i, 1)
+                    states.append(sweep.save())
             for i in range(s1 - 1, s0 - 1, -1):
-                sweep.adjoint(i, states[i - s0], adj)
+                sweep.adjoint(i, states[i - s0], adj, stages.pop() if stages else None)
     else:
         sweep.reverse_binomial(0, n, arg, saved[0], adj)
 
@@ -163,12 +184,12 @@ class RK3Rollout(torch.autograd.Function):
     """U0 -> U_N over fixed time steps, with the discrete adjoint as backward."""
 
     @staticmethod
-    def forward(ctx, U0, field: DeviceField, dts, checkpoint: str, stats: dict | None):
+    def forward(ctx, U0, field: DeviceField, dts, checkpoint: str, stats: dict | None, keep_stages: bool):
         _check_state(U0, field)
         n = len(dts)
         kind, arg = parse_checkpoint(checkpoint, n)
         field.set_stream(torch.cuda.current_stream().cuda_stream)
-        sweep = _Sweep(field, dts, stats if stats is not None else {})
+        sweep = _Sweep(field, dts, stats if stats is not None else {}, keep_stages=keep_stages)
         for k in ("recomputed_steps", "adjoint_steps", "peak_snapshots", "live"):
             sweep.stats.setdefault(k, 0)
         t0 = time.perf_counter()
@@ -194,7 +215,7 @@ class RK3Rollout(torch.autograd.Function):
         _reverse(sweep, kind, arg, saved, n, adj)
         torch.cuda.synchronize()
         sweep.stats["backward_s"] = time.perf_counter() - t0
-        return adj, None, None, None, None
+        return adj, None, None, None, None, None
 
 
 class RK3TimeIntegral(torch.autograd.Function):
@@ -203,7 +224,8 @@ class RK3TimeIntegral(torch.autograd.Function):
     backward."""
 
     @staticmethod
-    def forward(ctx, U0, field: DeviceField, dts, density, checkpoint: str, stats: dict | None):
+    def forward(ctx, U0, field: DeviceField, dts, density, checkpoint: str, stats: dict | None, keep_stages: bool,
+                density_grad):
         _check_state(U0, field)
         n = len(dts)
         kind, arg = parse_checkpoint(checkpoint, n)
@@ -211,13 +233,16 @@ class RK3TimeIntegral(torch.autograd.Function):
         stats = stats if stats is not None else {}
 
         def source(U, i):
-            with torch.enable_grad():
-                X = U.detach().requires_grad_(True)
-                (g,) = torch.autograd.grad(density(X), X)
+            if density_grad is not None:
+                g = density_grad(U)
+            else:
+                with torch.enable_grad():
+                    X = U.detach().requires_grad_(True)
+                    (g,) = torch.autograd.grad(density(X), X)
             return (sweep.dts[i - 1] * gJ_scale[0]) * g
 
         gJ_scale = [1.0]
-        sweep = _Sweep(field, dts, stats, source)
+        sweep = _Sweep(field, dts, stats, source, keep_stages)
         for k in ("recomputed_steps", "adjoint_steps", "peak_snapshots", "live"):
             sweep.stats.setdefault(k, 0)
         t0 = time.perf_counter()
@@ -252,14 +277,16 @@ class RK3TimeIntegral(torch.autograd.Function):
         _reverse(sweep, ctx.kind, ctx.arg, ctx.saved, n, adj)
         torch.cuda.synchronize()
         sweep.stats["backward_s"] = time.perf_counter() - t0
-        return adj, None, None, None, None, None
+        return adj, None, None, None, None, None, None, None
 
 
 def time_integral(U0: torch.Tensor, field: DeviceField, dts, density, checkpoint: str = "uniform:10",
-                  stats: dict | None = None):
+                  stats: dict | None = None, keep_stages: bool = True, density_grad=None):
     """Differentiable J = sum_n dt_n density(U_{n+1}) over the rollout (see
-    the module docstring)."""
-    return RK3TimeIntegral.apply(U0, field, list(dts), density, checkpoint, stats)
+    the module docstring).  density_grad(U), when given, is the density's
+    gradient (e.g. dissipation_rate_grad, which bounds the memory on large
+    boxes); otherwise torch autograd of density."""
+    return RK3TimeIntegral.apply(U0, field, list(dts), density, checkpoint, stats, keep_stages, density_grad)
 
 
 def time_integral_value(field: DeviceField, U0: torch.Tensor, dts, density) -> float:
@@ -276,9 +303,10 @@ def time_integral_value(field: DeviceField, U0: torch.Tensor, dts, density) -> f
         return J
 
 
-def rollout(U0: torch.Tensor, field: DeviceField, dts, checkpoint: str = "uniform:10", stats: dict | None = None):
+def rollout(U0: torch.Tensor, field: DeviceField, dts, checkpoint: str = "uniform:10", stats: dict | None = None,
+            keep_stages: bool = True):
     """Differentiable N-step RK3 rollout (see the module docstring)."""
-    return RK3Rollout.apply(U0, field, list(dts), checkpoint, stats)
+    return RK3Rollout.apply(U0, field, list(dts), checkpoint, stats, keep_stages)
 
 
 def interior(U: torch.Tensor) -> torch.Tensor:
@@ -298,6 +326,38 @@ def d6(f: torch.Tensor, axis: int, h: float) -> torch.Tensor:
     return (r(3) - 9.0 * r(2) + 45.0 * r(1) - 45.0 * r(-1) + 9.0 * r(-2) - r(-3)) / (60.0 * h)
 
 
+def _dissipation_field(I: torch.Tensor, params, grid, zh: int = 0) -> torch.Tensor:
+    """tau_ij du_i/dx_j per cell from interior conserved fields I (5, z, y, x).
+    zh = 0: I is the whole periodic interior.  zh = 3: I is a z-slab with
+    three halo planes on each side; the result covers the slab's own planes
+    (z derivatives from the halo instead of the periodic image)."""
+    rho = I[0]
+    vel = [I[1] / rho, I[2] / rho, I[3] / rho]
+    h = (grid.dx, grid.dy, grid.dz)
+
+    def own(f):
+        return f[zh:f.shape[0] - zh] if zh else f
+
+    def dz(f):
+        if not zh:
+            return d6(f, 0, h[2])
+        n = f.shape[0] - 2 * zh
+        s = lambda k: f[zh + k:zh + k + n]  # noqa: E731  s(k)[i] = f[i + k]
+        return (s(3) - 9.0 * s(2) + 45.0 * s(1) - 45.0 * s(-1) + 9.0 * s(-2) - s(-3)) / (60.0 * h[2])
+
+    g = [[d6(own(vel[i]), 2, h[0]), d6(own(vel[i]), 1, h[1]), dz(vel[i])] for i in range(3)]
+    r, v = own(rho), [own(c) for c in vel]
+    p = (params.gamma - 1.0) * (own(I[4]) - 0.5 * r * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
+    T = p / (r * params.Rgas)
+    mu = params.mu * T ** 1.5 * (1.0 + params.S_ref) / (T + params.S_ref)
+    div = g[0][0] + g[1][1] + g[2][2]
+    eps = -2.0 / 3.0 * mu * div * div
+    for i in range(3):
+        for j in range(3):
+            eps = eps + mu * (g[i][j] + g[j][i]) * g[i][j]
+    return eps
+
+
 def dissipation_rate(U: torch.Tensor, params, grid) -> torch.Tensor:
     """Volume mean over the interior of the viscous dissipation tau_ij
     du_i/dx_j, with the solver's ingredients: u = rho u / rho,
@@ -305,21 +365,26 @@ def dissipation_rate(U: torch.Tensor, params, grid) -> torch.Tensor:
     (field_structures.h:96-106), tau_ij = mu (du_i/dx_j + du_j/dx_i) -
     2/3 mu div(u) delta_ij (ViscousFlux.cpp:151-208) and 6th-order central
     gradients on the periodic box."""
+    return _dissipation_field(interior(U), params, grid).mean()
+
+
+def dissipation_rate_grad(U: torch.Tensor, params, grid, slab: int = 32) -> torch.Tensor:
+    """d dissipation_rate / dU (ghosted shape, zero ghosts), formed slab by
+    slab along z (each slab with its three periodic halo planes), so that the
+    autograd intermediates of a large box need only a slab's memory."""
     I = interior(U)
-    rho = I[0]
-    vel = [I[1] / rho, I[2] / rho, I[3] / rho]
-    p = (params.gamma - 1.0) * (I[4] - 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
-    T = p / (rho * params.Rgas)
-    mu = params.mu * T ** 1.5 * (1.0 + params.S_ref) / (T + params.S_ref)
-    h = (grid.dx, grid.dy, grid.dz)
-    axis = (2, 1, 0)  # x, y, z of a (nz, ny, nx) array
-    g = [[d6(vel[i], axis[j], h[j]) for j in range(3)] for i in range(3)]
-    div = g[0][0] + g[1][1] + g[2][2]
-    eps = -2.0 / 3.0 * mu * div * div
-    for i in range(3):
-        for j in range(3):
-            eps = eps + mu * (g[i][j] + g[j][i]) * g[i][j]
-    return eps.mean()
+    nz = I.shape[1]
+    ncell = I[0].numel()
+    G = torch.zeros_like(U)
+    GI = interior(G)
+    for z0 in range(0, nz, slab):
+        z1 = min(z0 + slab, nz)
+        idx = torch.arange(z0 - 3, z1 + 3, device=U.device) % nz
+        with torch.enable_grad():
+            X = I.index_select(1, idx).detach().requires_grad_(True)
+            (g,) = torch.autograd.grad(_dissipation_field(X, params, grid, zh=3).sum(), X)
+        GI.index_add_(1, idx, g)
+    return G / ncell
 
 
 def kinetic_energy_density(U: torch.Tensor) -> torch.Tensor:

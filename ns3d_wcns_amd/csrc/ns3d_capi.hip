@@ -572,6 +572,30 @@ int check_adjoint(ns3d_ctx *c) {
     return NS3D_OK;
 }
 
+// The reverse of one RK3 step, the context holding U^(2) (primitives and
+// ghosts current) and U^n recorded; U1: U^(1).  adj: Ub^3 on entry, Ub^n on
+// return; the context is left at U^n.
+int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1) {
+    int rc;
+    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3 (the adjoint's
+    // ghosts are filled in place; its ghosts are zeroed on return)
+    if ((rc = fill_ghosts(c, adj, 5)) || (rc = adjoint_rhs_impl(c, adj, c->pr))) return rc;
+    if ((rc = lincomb(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr)) ||
+        (rc = lincomb(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr)))
+        return rc;
+    // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
+    if ((rc = copy_fields(c, conserved(c), U1, 5)) || (rc = refresh_state(c))) return rc;
+    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
+    if ((rc = lincomb(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr)) ||
+        (rc = lincomb(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr)))
+        return rc;
+    // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P
+    if ((rc = copy_fields(c, conserved(c), recorded(c), 5)) || (rc = refresh_state(c))) return rc;
+    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
+    if ((rc = zero_fields(c, adj, 5))) return rc;
+    return lincomb(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
+}
+
 }  // namespace
 
 extern "C" {
@@ -653,23 +677,30 @@ int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
     if ((rc = fused_stage(c, 1, nullptr, dt)) || (rc = copy_fields(c, c->s1, conserved(c), 5)) ||
         (rc = fused_stage(c, 2, nullptr, dt)))
         return rc;
-    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3 (the adjoint's
-    // ghosts are filled in place; its ghosts are zeroed on return)
-    if ((rc = fill_ghosts(c, adj, 5)) || (rc = adjoint_rhs_impl(c, adj, c->pr))) return rc;
-    if ((rc = lincomb(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr)) ||
-        (rc = lincomb(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr)))
+    return adjoint_reverse(c, dt, adj, c->s1);
+}
+
+int ns3d_step_stages(ns3d_ctx *c, double dt, double *const *U1, double *const *U2) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)U1, 5)) || (rc = check_ptrs((const void *const *)U2, 5))) return rc;
+    if ((rc = record0(c)) || (rc = fused_stage(c, 1, nullptr, dt)) || (rc = copy_fields(c, U1, conserved(c), 5)) ||
+        (rc = fused_stage(c, 2, nullptr, dt)) || (rc = copy_fields(c, U2, conserved(c), 5)))
         return rc;
-    // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
-    if ((rc = copy_fields(c, conserved(c), c->s1, 5)) || (rc = refresh_state(c))) return rc;
-    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
-    if ((rc = lincomb(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr)) ||
-        (rc = lincomb(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr)))
+    return fused_stage(c, 3, nullptr, dt);
+}
+
+int ns3d_adjoint_step_stages(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1,
+                             const double *const *U2) {
+    NS3D_CTX(c);
+    int rc;
+    if ((rc = check_ptrs((const void *const *)adj, 5)) || (rc = check_ptrs((const void *const *)U1, 5)) ||
+        (rc = check_ptrs((const void *const *)U2, 5)))
         return rc;
-    // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P
-    if ((rc = copy_fields(c, conserved(c), recorded(c), 5)) || (rc = refresh_state(c))) return rc;
-    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
-    if ((rc = zero_fields(c, adj, 5))) return rc;
-    return lincomb(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
+    if ((rc = check_adjoint(c)) || (rc = ensure_adj(c)) || (rc = record0(c))) return rc;
+    // U^(2) from the caller: primitives and ghosts formed as after a stage
+    if ((rc = copy_fields(c, conserved(c), U2, 5)) || (rc = refresh_state(c))) return rc;
+    return adjoint_reverse(c, dt, adj, U1);
 }
 
 int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params, int precision) {

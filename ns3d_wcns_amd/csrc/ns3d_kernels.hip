@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
             const long o = b.oidx(i, j, k);
             R *const *V = reinterpret_cast<R *const *>(a.V);
 #pragma unroll
-            for (int n = 0; n < NVISC; ++n) V[n][o] = Vc[n];
+            for (int n = 0; n < NVISC; ++n) __builtin_nontemporal_store(Vc[n], &V[n][o]);  // read once, next kernel
         }
 #pragma unroll
         for (int m = 0; m < 6; ++m)

@@ -85,11 +85,14 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, const 
     a.Unew[2][g] = rr * v;
     a.Unew[3][g] = rr * w;
     a.Unew[4][g] = rr * (p / ((a.gamma - 1.0) * rr) + kin);
-    a.Q[0][g] = u;
-    a.Q[1][g] = v;
-    a.Q[2][g] = w;
-    a.Q[3][g] = p;
-    a.Q[4][g] = p / (rr * a.Rgas);
+    // the primitives are next read by the following stage's viscous kernel:
+    // streamed past the caches (nontemporal), as are the partial sums and U0
+    // below and the viscous fluxes, each read once by one later kernel
+    __builtin_nontemporal_store(u, &a.Q[0][g]);
+    __builtin_nontemporal_store(v, &a.Q[1][g]);
+    __builtin_nontemporal_store(w, &a.Q[2][g]);
+    __builtin_nontemporal_store(p, &a.Q[3][g]);
+    __builtin_nontemporal_store(p / (rr * a.Rgas), &a.Q[4][g]);
 }
 
 // ------------------------------------------------------------------ x pencil
@@ -121,8 +124,9 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
         if (valid) {
 #pragma unroll
             for (int n = 0; n < 5; ++n) {
-                sp[n] = a.Rz[n][orow + t] + a.Ry[n][orow + t];
-                zp[n] = need_u0 ? a.U0[n][row + NG + t] : 0.0;
+                sp[n] = __builtin_nontemporal_load(&a.Rz[n][orow + t]) +
+                        __builtin_nontemporal_load(&a.Ry[n][orow + t]);
+                zp[n] = need_u0 ? __builtin_nontemporal_load(&a.U0[n][row + NG + t]) : 0.0;
             }
         }
     }
@@ -336,7 +340,7 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
             const long o = obase + (long)c * ost;
             double *const *dst = DIR == 2 ? a.Rz : a.Ry;
 #pragma unroll
-            for (int n = 0; n < 5; ++n) dst[n][o] = acc[0][n] * se;
+            for (int n = 0; n < 5; ++n) __builtin_nontemporal_store(acc[0][n] * se, &dst[n][o]);
         }
 #pragma unroll
         for (int q = 0; q < 6; ++q)

@@ -232,6 +232,16 @@ class DeviceField:
         U^n with fixed dt; the state stays U^n."""
         check(self._lib.ns3d_adjoint_step(self._h, dt, _lib.ptr5(adj)))
 
+    def step_stages(self, dt: float, U1, U2):
+        """step(dt), storing the RK3 stage states U^(1), U^(2) (five device
+        pointers each, ghosted layout)."""
+        check(self._lib.ns3d_step_stages(self._h, dt, _lib.ptr5(U1), _lib.ptr5(U2)))
+
+    def adjoint_step_stages(self, dt: float, adj, U1, U2):
+        """adjoint_step with the stage states stored by step_stages (no
+        recomputation of stages 1 and 2)."""
+        check(self._lib.ns3d_adjoint_step_stages(self._h, dt, _lib.ptr5(adj), _lib.ptr5(U1), _lib.ptr5(U2)))
+
     def step(self, dt: float):
         """recordConservedTo0 + runge_kutta_3 with a fixed dt."""
         self.recordConservedTo0()

@@ -10,7 +10,9 @@ device through the C-ABI, and the checkpointed autograd rollout.
 - gradient of J = mean 0.5 rho |u|^2 at the final time with respect to the
   initial conserved fields vs central differences (eps = 1e-6) on 8 seeded
   unit-norm directions: 1e-6 relative;
-- checkpoint policies give the same gradient to 1e-12;
+- checkpoint policies give the same gradient to 1e-12, and so do the
+  adjoint steps with stored stage states (step_stages / adjoint_step_stages)
+  and with recomputed ones;
 - time-integrated viscous dissipation J = sum_n dt_n eps(U_{n+1}): the
   checkpointed forward value equals a plain forward's, the gradient matches
   central differences (eps = 1e-5) on 4 seeded directions to 1e-6 (1e-7 of
@@ -195,6 +197,56 @@ def test_checkpoint_policies_agree(gpu):
     f.close()
 
 
+def test_adjoint_with_stored_stages(gpu):
+    """step_stages + adjoint_step_stages (stage states kept by the caller)
+    against adjoint_step (stages recomputed inside): the stage states are
+    the same computation, so the adjoints agree to rounding (1e-13), and
+    the state after step_stages is the state after step."""
+    import torch
+
+    P = params(0.17)
+    f, U = field_and_state(SMALL, P, seed=6)
+    dt = 0.5 * f.compute_timestep()
+    lam = rand_interior(U.shape, 9)
+    U1, U2 = torch.empty_like(U), torch.empty_like(U)
+    f.set_state(ptrs(U))
+    f.step_stages(dt, ptrs(U1), ptrs(U2))
+    after = torch.empty_like(U)
+    f.get_state(ptrs(after))
+    f.set_state(ptrs(U))
+    f.step(dt)
+    ref = torch.empty_like(U)
+    f.get_state(ptrs(ref))
+    assert torch.equal(after, ref)
+    a1, a2 = lam.clone(), lam.clone()
+    f.set_state(ptrs(U))
+    f.adjoint_step(dt, ptrs(a1))
+    f.set_state(ptrs(U))
+    f.adjoint_step_stages(dt, ptrs(a2), ptrs(U1), ptrs(U2))
+    torch.cuda.synchronize()
+    err = float((a2 - a1).abs().max() / a1.abs().max())
+    print(f"stored-stage adjoint vs recomputed: {err:.2e}")
+    assert err < 1e-13, err
+    for policy, keep in (("uniform:4", False), ("uniform:4", True)):
+        stats = {}
+        _, g = rollout_grad_keep(f, U, [dt] * 6, policy, keep, stats)
+        if keep:
+            assert stats["recomputed_steps"] == 6
+            err = float((g - g0).abs().max() / g0.abs().max())
+            assert err < 1e-12, err
+        else:
+            g0 = g
+    f.close()
+
+
+def rollout_grad_keep(f, U, dts, policy, keep, stats):
+    from ns3d_wcns_amd.adjoint import mean_kinetic_energy, rollout
+
+    U0 = U.clone().requires_grad_(True)
+    mean_kinetic_energy(rollout(U0, f, dts, policy, stats, keep_stages=keep)).backward()
+    return None, U0.grad
+
+
 def test_adjoint_rejects_unsupported_scheme(gpu):
     import torch
 
@@ -269,6 +321,24 @@ def test_time_integrated_dissipation_gradient(gpu):
         assert cs < 1e-7, (k, cs)
         if abs(ad) >= 1e-2 * gnorm:
             assert rel < 1e-6, (k, rel)
+    f.close()
+
+
+def test_dissipation_gradient_by_slabs(gpu):
+    """The slab-wise gradient of the dissipation rate (z-slabs with periodic
+    halo planes, slabs of 5 and 32 on a 16-plane box, the latter wrapping
+    one slab around the box) equals torch autograd of the whole-box rate."""
+    import torch
+    from ns3d_wcns_amd.adjoint import dissipation_rate, dissipation_rate_grad
+
+    f, U, _, _ = integral_setup(steps=2)
+    X = U.clone().requires_grad_(True)
+    (g_ref,) = torch.autograd.grad(dissipation_rate(X, f.params, f.grid), X)
+    for slab in (5, 32):
+        g = dissipation_rate_grad(U, f.params, f.grid, slab=slab)
+        err = float((g - g_ref).abs().max() / g_ref.abs().max())
+        print(f"slab {slab}: {err:.2e}")
+        assert err < 1e-13, (slab, err)
     f.close()
 
 
