@@ -10,10 +10,12 @@
 //   k_tl_assemble tangent RHS: half-node differences + viscous divergence.
 // Adjoint (hand-written reverse mode, ns3d_adjoint.h; component-wise WCNS5 /
 // WENO5-JS with HLLC / Rusanov):
-//   k_adj_visc_a  per cell: face adjoints of the three directions (the
-//                 transposed half-node difference of lambda), viscous flux
-//                 adjoints from the transposed divergence, reverse of
-//                 stresses / heat flux -> gradient and (u, v, w, T) adjoints;
+//   k_adj_lam_march  z-marching plane tiles of lambda: face adjoints of the
+//                 three directions (the transposed half-node difference) and
+//                 viscous flux adjoints (the transposed divergence);
+//   k_adj_q_march  z-marching plane tiles of u, v, w, T: their gradients and
+//                 the reverse of stresses / heat flux -> gradient and
+//                 (u, v, w, T) adjoints;
 //   per direction, marching along the direction (lanes across it):
 //   k_adj_riemann both reconstructions of each cell (shared smoothness
 //                 indicators) and the reverse of the Riemann flux of each
@@ -441,54 +443,183 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a) {
     for (int n = 0; n < 5; ++n) a.out[n][g] = acc[n];
 }
 
-__global__ void __launch_bounds__(256) k_adj_visc_a(AdjArgs a) {
-    int i, j, k;
-    if (!cell_of(a.b, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    const Box &b = a.b;
-    const long c = b.gidx(i, j, k);
-    const long o = b.oidx(i, j, k);
-    const double *Qv[4] = {a.Q[0], a.Q[1], a.Q[2], a.Q[4]};
-    double g[4][3], lg[4][3];
+// The face and viscous adjoints of one adjoint RHS, as two z-marching
+// kernels over 64 x 4 column tiles (the tile order of k_visc_march): each
+// thread keeps planes k-3..k+3 of its own column in registers for the z
+// stencils; each plane's tile with a 3-cell x/y halo is staged in LDS for the
+// x and y stencils.  Cells beyond the box come from their periodic image, so
+// neither lambda nor the primitives need ghost layers.
+constexpr int AV_SEG = 16;  // z planes per block
+
+// Index of the periodic image of interior cell (x, y, z) for x, y in any
+// range and z in [-3, nz + 3).
+struct PlaneMap {
+    Box b;
+    long col, hc[2];
+    int hi[2], hj[2];
+    bool hv[2];
+    __device__ PlaneMap(const Box &bb, int i0, int j0, int tx, int ty) : b(bb) {
+        auto per = [](int x, int n) { return ((x % n) + n) % n; };
+        const int tid = ty * 64 + tx;
+        col = b.gidx(per(i0 + tx, b.nx), per(j0 + ty, b.ny), 0);
+        // halo cells of a plane: 6 rows x 64 above/below, then 4 rows x 6 left/right
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const long st = gstride(b, d);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double *f = Qv[q];
-            g[q][d] = d6c(f[c - 3 * st], f[c - 2 * st], f[c - st], f[c + st], f[c + 2 * st], f[c + 3 * st], a.i60[d]);
-        }
-#pragma unroll
-        for (int n = 0; n < 5; ++n) {
-            const double *l = a.X[n];
-            double lv[7];
-#pragma unroll
-            for (int m = 0; m < 7; ++m) lv[m] = l[c + (m - 3) * st];
-            // face adjoint of face c + 1/2: Fb = -(1/h) sum_m kW[m] lambda[c + 3 - m]
-            double sum = 0.0;
-#pragma unroll
-            for (int m = 0; m < 6; ++m) sum += kW[m] * lv[6 - m];
-            a.FB[d][n][o] = -sum * a.ih[d];
-            if (n > 0) lg[n - 1][d] = d6c(lv[0], lv[1], lv[2], lv[4], lv[5], lv[6], a.i60[d]);
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + 256 * h;
+            int x, y;
+            if (e < 6 * 64) {
+                const int r = e / 64;
+                y = r < 3 ? r - 3 : r + 1;
+                x = e % 64;
+            } else {
+                const int e2 = e - 6 * 64, r = e2 / 6, c6 = e2 % 6;
+                y = r;
+                x = c6 < 3 ? c6 - 3 : 64 + c6 - 3;
+            }
+            hi[h] = x;
+            hj[h] = y;
+            hv[h] = e < 6 * 64 + 6 * 4;
+            hc[h] = b.gidx(per(i0 + x, b.nx), per(j0 + y, b.ny), 0);
         }
     }
-    // transposed divergence (D6 is antisymmetric on the periodic box)
-    double Vb[NVISC];
-    Vb[VXX] = -lg[0][0];
-    Vb[VYY] = -lg[1][1];
-    Vb[VZZ] = -lg[2][2];
-    Vb[VXY] = -(lg[0][1] + lg[1][0]);
-    Vb[VXZ] = -(lg[0][2] + lg[2][0]);
-    Vb[VYZ] = -(lg[1][2] + lg[2][1]);
-    Vb[VEX] = -lg[3][0];
-    Vb[VEY] = -lg[3][1];
-    Vb[VEZ] = -lg[3][2];
-    double gb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, qb[4] = {0, 0, 0, 0};
-    visc_local_vjp(a.mu, a.S_ref, a.cp_pr, a.Q[0][c], a.Q[1][c], a.Q[2][c], a.Q[4][c], g, Vb, gb, qb);
+    __device__ long plane(int k) const { return (long)wrap(k, b.nz) * ((long)b.sx * b.sy); }
+};
+
+// Lambda part: face adjoints Fb = -(1/h) sum_m kW[m] lambda[c + 3 - m] of the
+// three directions (the transposed half-node difference) and the viscous-flux
+// adjoints Vb = -(transposed divergence of lambda), stored in SB[0..8] for the
+// second kernel (SB is rewritten later by the Riemann reverse).
+__global__ void __launch_bounds__(256) k_adj_lam_march(AdjArgs a) {
+    constexpr int HX = 70, HY = 10;
+    __shared__ double tile[5][HY][HX];
+    const Box &b = a.b;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4, i = i0 + tx, j = j0 + ty;
+    const bool own = i < b.nx && j < b.ny;
+    const int k0 = blockIdx.z * AV_SEG, k1 = min(k0 + AV_SEG, b.nz);
+    const PlaneMap pm(b, i0, j0, tx, ty);
+    double wl[5][7];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int m = 0; m < 6; ++m)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) a.Gb[3 * q + d][o] = gb[q][d];
-        a.qb[q][o] = qb[q];
+        for (int n = 0; n < 5; ++n) wl[n][m] = a.X[n][pm.col + pm.plane(k0 - 3 + m)];
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) wl[n][6] = a.X[n][pm.col + pm.plane(k + 3)];
+        double hv[2][5];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) hv[h][n] = pm.hv[h] ? a.X[n][pm.hc[h] + pm.plane(k)] : 0.0;
+        __syncthreads();  // the previous plane's tile has been read
+#pragma unroll
+        for (int n = 0; n < 5; ++n) tile[n][ty + 3][tx + 3] = wl[n][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (pm.hv[h])
+#pragma unroll
+                for (int n = 0; n < 5; ++n) tile[n][pm.hj[h] + 3][pm.hi[h] + 3] = hv[h][n];
+        __syncthreads();
+        if (own) {
+            const long o = b.oidx(i, j, k);
+            double lg[4][3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int n = 0; n < 5; ++n) {
+                    double lv[7];
+#pragma unroll
+                    for (int m = 0; m < 7; ++m)
+                        lv[m] = d == 0 ? tile[n][ty + 3][tx + m] : d == 1 ? tile[n][ty + m][tx + 3] : wl[n][m];
+                    double sum = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) sum += kW[m] * lv[6 - m];
+                    __builtin_nontemporal_store(-sum * a.ih[d], &a.FB[d][n][o]);
+                    if (n > 0) lg[n - 1][d] = d6c(lv[0], lv[1], lv[2], lv[4], lv[5], lv[6], a.i60[d]);
+                }
+            // transposed divergence (D6 is antisymmetric on the periodic box)
+            double Vb[NVISC];
+            Vb[VXX] = -lg[0][0];
+            Vb[VYY] = -lg[1][1];
+            Vb[VZZ] = -lg[2][2];
+            Vb[VXY] = -(lg[0][1] + lg[1][0]);
+            Vb[VXZ] = -(lg[0][2] + lg[2][0]);
+            Vb[VYZ] = -(lg[1][2] + lg[2][1]);
+            Vb[VEX] = -lg[3][0];
+            Vb[VEY] = -lg[3][1];
+            Vb[VEZ] = -lg[3][2];
+#pragma unroll
+            for (int n = 0; n < NVISC; ++n) __builtin_nontemporal_store(Vb[n], &a.SB[n][o]);
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) wl[n][m] = wl[n][m + 1];
+    }
+}
+
+// Primitive part: the gradients of u, v, w, T (6th-order central) and the
+// reverse of the local viscous-flux map with Vb from SB[0..8] -> gradient
+// adjoints Gb and (u, v, w, T) adjoints qb.
+__global__ void __launch_bounds__(256) k_adj_q_march(AdjArgs a) {
+    constexpr int HX = 70, HY = 10;
+    __shared__ double tile[4][HY][HX];
+    const Box &b = a.b;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4, i = i0 + tx, j = j0 + ty;
+    const bool own = i < b.nx && j < b.ny;
+    const int k0 = blockIdx.z * AV_SEG, k1 = min(k0 + AV_SEG, b.nz);
+    const PlaneMap pm(b, i0, j0, tx, ty);
+    const double *Qv[4] = {a.Q[0], a.Q[1], a.Q[2], a.Q[4]};
+    double wq[4][7];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wq[q][m] = Qv[q][pm.col + pm.plane(k0 - 3 + m)];
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wq[q][6] = Qv[q][pm.col + pm.plane(k + 3)];
+        double hv[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) hv[h][q] = pm.hv[h] ? Qv[q][pm.hc[h] + pm.plane(k)] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tile[q][ty + 3][tx + 3] = wq[q][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (pm.hv[h])
+#pragma unroll
+                for (int q = 0; q < 4; ++q) tile[q][pm.hj[h] + 3][pm.hi[h] + 3] = hv[h][q];
+        __syncthreads();
+        if (own) {
+            const long o = b.oidx(i, j, k);
+            double g[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                g[q][0] = d6c(tile[q][ty + 3][tx], tile[q][ty + 3][tx + 1], tile[q][ty + 3][tx + 2],
+                              tile[q][ty + 3][tx + 4], tile[q][ty + 3][tx + 5], tile[q][ty + 3][tx + 6], a.i60[0]);
+                g[q][1] = d6c(tile[q][ty][tx + 3], tile[q][ty + 1][tx + 3], tile[q][ty + 2][tx + 3],
+                              tile[q][ty + 4][tx + 3], tile[q][ty + 5][tx + 3], tile[q][ty + 6][tx + 3], a.i60[1]);
+                g[q][2] = d6c(wq[q][0], wq[q][1], wq[q][2], wq[q][4], wq[q][5], wq[q][6], a.i60[2]);
+            }
+            double Vb[NVISC];
+#pragma unroll
+            for (int n = 0; n < NVISC; ++n) Vb[n] = __builtin_nontemporal_load(&a.SB[n][o]);
+            double gb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, qb[4] = {0, 0, 0, 0};
+            visc_local_vjp(a.mu, a.S_ref, a.cp_pr, wq[0][3], wq[1][3], wq[2][3], wq[3][3], g, Vb, gb, qb);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) a.Gb[3 * q + d][o] = gb[q][d];
+                a.qb[q][o] = qb[q];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wq[q][m] = wq[q][m + 1];
     }
 }
 
@@ -586,7 +717,9 @@ void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s) {
     // the face and viscous gradient adjoints first, then the y and z
     // directions (y writes the interior of out, z accumulates), then the x
     // direction with the viscous part (ghosts of out are not written)
-    hipLaunchKernelGGL(k_adj_visc_a, cgrid(a.b), dim3(256), 0, s, a);
+    const dim3 gv((a.b.nx + 63) / 64, (a.b.ny + 3) / 4, (a.b.nz + AV_SEG - 1) / AV_SEG), bv(64, 4);
+    hipLaunchKernelGGL(k_adj_lam_march, gv, bv, 0, s, a);
+    hipLaunchKernelGGL(k_adj_q_march, gv, bv, 0, s, a);
     adj_dir<1>(a, interp, rs, s);
     adj_dir<2>(a, interp, rs, s);
     adj_x(a, interp, rs, s);

@@ -519,7 +519,7 @@ int tangent_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
 }
 
 // Adjoint RHS (interior of out) at the current state, of the adjoint X
-// (ghosts filled).
+// (interior read; stencils take periodic images).
 int adjoint_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
     AdjArgs a = adj_args(c, out);
     for (int n = 0; n < 5; ++n) a.X[n] = X[n];
@@ -577,21 +577,22 @@ int check_adjoint(ns3d_ctx *c) {
 // return; the context is left at U^n.
 int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1) {
     int rc;
-    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3 (the adjoint's
-    // ghosts are filled in place; its ghosts are zeroed on return)
-    if ((rc = fill_ghosts(c, adj, 5)) || (rc = adjoint_rhs_impl(c, adj, c->pr))) return rc;
+    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3 (the adjoint
+    // kernels read periodic images, not ghosts; the ghosts of adj are zeroed
+    // on return)
+    if ((rc = adjoint_rhs_impl(c, adj, c->pr))) return rc;
     if ((rc = lincomb(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr)) ||
         (rc = lincomb(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr)))
         return rc;
     // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
     if ((rc = copy_fields(c, conserved(c), U1, 5)) || (rc = refresh_state(c))) return rc;
-    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
+    if ((rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
     if ((rc = lincomb(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr)) ||
         (rc = lincomb(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr)))
         return rc;
     // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P
     if ((rc = copy_fields(c, conserved(c), recorded(c), 5)) || (rc = refresh_state(c))) return rc;
-    if ((rc = fill_ghosts(c, c->a1, 5)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
+    if ((rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
     if ((rc = zero_fields(c, adj, 5))) return rc;
     return lincomb(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
 }
