@@ -625,18 +625,57 @@ __global__ void __launch_bounds__(256) k_adj_q_march(AdjArgs a) {
 
 // out = alpha A + beta B + gamma P on the interior cells; a null operand
 // counts as zero, and out may alias any operand.
+__device__ __forceinline__ double lincomb_at(const LinComb &l, int n, long g) {
+    double r = 0.0;
+    if (l.A[n]) r += l.alpha * l.A[n][g];
+    if (l.B[n]) r += l.beta * l.B[n][g];
+    if (l.P[n]) r += l.gamma * l.P[n][g];
+    return r;
+}
+
+// Interior cell of a thread (32-bit index arithmetic: the box has fewer than
+// 2^31 cells).
+__device__ __forceinline__ bool cell_of32(const Box &b, int id, int &i, int &j, int &k) {
+    if (id >= b.nx * b.ny * b.nz) return false;
+    i = id % b.nx;
+    const int r = id / b.nx;
+    j = r % b.ny;
+    k = r / b.ny;
+    return true;
+}
+
 __global__ void __launch_bounds__(256) k_lincomb(LinComb l) {
     int i, j, k;
-    if (!cell_of(l.b, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!cell_of32(l.b, blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const long g = l.b.gidx(i, j, k);
 #pragma unroll
+    for (int n = 0; n < 5; ++n) l.out[n][g] = lincomb_at(l, n, g);
+}
+
+// Two combinations in one pass, both formed before either is stored (l2 may
+// read what l1 writes and the reverse).
+__global__ void __launch_bounds__(256) k_lincomb2(LinComb l1, LinComb l2) {
+    int i, j, k;
+    if (!cell_of32(l1.b, blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const long g = l1.b.gidx(i, j, k);
+#pragma unroll
     for (int n = 0; n < 5; ++n) {
-        double r = 0.0;
-        if (l.A[n]) r += l.alpha * l.A[n][g];
-        if (l.B[n]) r += l.beta * l.B[n][g];
-        if (l.P[n]) r += l.gamma * l.P[n][g];
-        l.out[n][g] = r;
+        const double r1 = lincomb_at(l1, n, g), r2 = lincomb_at(l2, n, g);
+        l1.out[n][g] = r1;
+        l2.out[n][g] = r2;
     }
+}
+
+// The combination on the interior and zero on the ghost layers, over the
+// whole ghosted box.
+__global__ void __launch_bounds__(256) k_lincomb_fill(LinComb l) {
+    const Box &b = l.b;
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= b.sx * b.sy * b.sz) return;
+    const int x = id % b.sx, r = id / b.sx, y = r % b.sy, z = r / b.sy;
+    const bool in = x >= NG && x < NG + b.nx && y >= NG && y < NG + b.ny && z >= NG && z < NG + b.nz;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) l.out[n][id] = in ? lincomb_at(l, n, id) : 0.0;
 }
 
 // ------------------------------------------------------------ launchers
@@ -727,6 +766,15 @@ void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s) {
 
 void launch_lincomb(const LinComb &l, hipStream_t s) {
     hipLaunchKernelGGL(k_lincomb, cgrid(l.b), dim3(256), 0, s, l);
+}
+
+void launch_lincomb2(const LinComb &l1, const LinComb &l2, hipStream_t s) {
+    hipLaunchKernelGGL(k_lincomb2, cgrid(l1.b), dim3(256), 0, s, l1, l2);
+}
+
+void launch_lincomb_fill(const LinComb &l, hipStream_t s) {
+    const long cells = (long)l.b.sx * l.b.sy * l.b.sz;
+    hipLaunchKernelGGL(k_lincomb_fill, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, l);
 }
 
 }  // namespace ns3d

@@ -535,8 +535,8 @@ int load_input(ns3d_ctx *c, const double *const *in) {
     return fill_ghosts(c, c->xg, 5);
 }
 
-int lincomb(ns3d_ctx *c, double *const *out, double alpha, double *const *A, double beta, double *const *B,
-            double gamma, double *const *P) {
+LinComb lc(ns3d_ctx *c, double *const *out, double alpha, double *const *A, double beta, double *const *B,
+           double gamma, double *const *P) {
     LinComb l{};
     for (int n = 0; n < 5; ++n) {
         l.out[n] = out[n];
@@ -548,7 +548,12 @@ int lincomb(ns3d_ctx *c, double *const *out, double alpha, double *const *A, dou
     l.beta = beta;
     l.gamma = gamma;
     l.b = c->b;
-    launch_lincomb(l, c->stream);
+    return l;
+}
+
+int lincomb(ns3d_ctx *c, double *const *out, double alpha, double *const *A, double beta, double *const *B,
+            double gamma, double *const *P) {
+    launch_lincomb(lc(c, out, alpha, A, beta, B, gamma, P), c->stream);
     NS3D_LAUNCHED("lincomb");
     return NS3D_OK;
 }
@@ -581,20 +586,21 @@ int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *co
     // kernels read periodic images, not ghosts; the ghosts of adj are zeroed
     // on return)
     if ((rc = adjoint_rhs_impl(c, adj, c->pr))) return rc;
-    if ((rc = lincomb(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr)) ||
-        (rc = lincomb(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr)))
-        return rc;
+    launch_lincomb2(lc(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr),
+                    lc(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr), c->stream);
+    NS3D_LAUNCHED("lincomb");
     // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
     if ((rc = copy_fields(c, conserved(c), U1, 5)) || (rc = refresh_state(c))) return rc;
     if ((rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
-    if ((rc = lincomb(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr)) ||
-        (rc = lincomb(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr)))
-        return rc;
+    launch_lincomb2(lc(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr),
+                    lc(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr), c->stream);
+    NS3D_LAUNCHED("lincomb");
     // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P
     if ((rc = copy_fields(c, conserved(c), recorded(c), 5)) || (rc = refresh_state(c))) return rc;
     if ((rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
-    if ((rc = zero_fields(c, adj, 5))) return rc;
-    return lincomb(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
+    launch_lincomb_fill(lc(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr), c->stream);
+    NS3D_LAUNCHED("lincomb");
+    return NS3D_OK;
 }
 
 }  // namespace

@@ -101,6 +101,8 @@ void launch_tl_prim(const AdjArgs &a, hipStream_t s);
 void launch_tl_rhs(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s);
 void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s);
 void launch_lincomb(const LinComb &l, hipStream_t s);
+void launch_lincomb2(const LinComb &l1, const LinComb &l2, hipStream_t s);  // both in one pass
+void launch_lincomb_fill(const LinComb &l, hipStream_t s);                  // interior; ghosts set to 0
 
 long ke_partial_count(const Box &b);
 void launch_visc(const ViscArgs &a, hipStream_t s);
