@@ -231,10 +231,17 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
     }
 }
 
+// FP64 component-wise WCNS5 / WENO5-JS: the marches form both
+// reconstructions of a cell at once (recon_pair).  The FP32 faces of the
+// mixed method keep the per-face form, which measured faster at three waves
+// per SIMD.
+template <int INTERP, bool CHAR, class R>
+constexpr bool kPairRecon = sizeof(R) == 8 && !CHAR && (INTERP == I_WENO5 || INTERP == I_WENO5_JS);
+
 // ------------------------------------------------------------ y / z pencils
 // Thread per (x, other, segment) column marching along DIR.  Iteration f
-// computes face f (cells f-2..f+3 in the register window) and loads the
-// viscous fluxes of cell f; seven running sums hold cells f-3..f+3, and cell
+// computes face f (its stencil, cells f-2..f+3, in the register window) and
+// loads the viscous fluxes of cell f; seven running sums hold cells f-3..f+3, and cell
 // f-3 is complete (faces f-6..f-1, viscous fluxes f-6..f) and is emitted
 // into the pencil's compact partial-RHS array.
 template <int DIR, int INTERP, int RS, bool CHAR, class R>
@@ -264,12 +271,13 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
     };
     // Cell-sum weights: face f -> cells f-2..f+3 (-kHalfW, slots 1..6),
     // viscous flux of cell f -> cells f-3..f+3 (central weights / 60, slots
-    // 0..6, none for the cell itself).  FP64 component-wise: pre-scaled by
-    // 1/h.  Otherwise unscaled compile-time constants, which occupy no vector
-    // registers (the kernel then fits two waves per SIMD characteristic-wise,
+    // 0..6, none for the cell itself).  FP64 component-wise MDCD: pre-scaled
+    // by 1/h.  Otherwise unscaled compile-time constants, which occupy no
+    // vector registers (the kernel then fits two waves per SIMD in FP64,
     // three with FP32 faces component-wise), and the cell sum is scaled when
     // emitted.
-    constexpr bool pre = sizeof(R) == 8 && !CHAR;
+    constexpr bool PAIR = kPairRecon<INTERP, CHAR, R>;
+    constexpr bool pre = sizeof(R) == 8 && !CHAR && !PAIR;
     const double sc = pre ? a.ih : 1.0, se = pre ? 1.0 : a.ih;
     double wf[7], wv[7];
     wf[0] = 0.0;
@@ -281,7 +289,15 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
 #pragma unroll
         for (int q = 0; q < 7; ++q) wv[q] = d[q] * s60;
     }
+    // Component-wise WCNS5 / WENO5-JS: the two reconstructions of a cell
+    // share their smoothness indicators (recon_pair), so iteration f forms
+    // both values of cell f + 1 from cells f-1..f+3: the right state of face
+    // f and the left state of face f + 1, carried to the next iteration in
+    // yl.  The window then holds five cells (W = 5), else the six cells
+    // f-2..f+3 of the face's stencil.
+    constexpr int W = PAIR ? 5 : 6;
     R w[5][6];
+    R yl[5];
     {
         R u[5];
 #pragma unroll
@@ -290,6 +306,18 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
 #pragma unroll
             for (int n = 0; n < 5; ++n) w[n][m] = u[n];
         }
+    }
+    if constexpr (PAIR) {
+        // cell c0 - 3 from cells c0-5..c0-1: the left state of face c0 - 3
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            R yr;
+            recon_pair<INTERP>(w[n], yl[n], yr);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) w[n][m] = w[n][m + 1];
     }
     R nxt[5];
     load(c0, nxt);
@@ -300,7 +328,7 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
         for (int n = 0; n < 5; ++n) acc[q][n] = 0.0;
     for (int f = c0 - 3; f <= c1 + 2; ++f) {
 #pragma unroll
-        for (int n = 0; n < 5; ++n) w[n][5] = nxt[n];
+        for (int n = 0; n < 5; ++n) w[n][W - 1] = nxt[n];
         if (f < c1 + 2) load(f + 4, nxt);  // prefetch the next window cell
         double vv[4];
         {
@@ -318,7 +346,16 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
                 }
             }
             R Fr[5];
-            face_flux<INTERP, RS, CHAR, XS>(cf, w, pc, Fr);
+            if constexpr (PAIR) {
+                R UR[5], yn[5];
+#pragma unroll
+                for (int n = 0; n < 5; ++n) recon_pair<INTERP>(w[n], yn[n], UR[n]);  // cell f + 1
+                riemann<RS, XS>(cf, yl, UR, Fr);
+#pragma unroll
+                for (int n = 0; n < 5; ++n) yl[n] = yn[n];
+            } else {
+                face_flux<INTERP, RS, CHAR, XS>(cf, w, pc, Fr);
+            }
             double F[5];
             F[0] = (double)Fr[0];
             F[P1] = (double)Fr[1];
@@ -349,21 +386,22 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
 #pragma unroll
         for (int n = 0; n < 5; ++n) acc[6][n] = 0.0;
 #pragma unroll
-        for (int m = 0; m < 5; ++m)
+        for (int m = 0; m < W - 1; ++m)
 #pragma unroll
             for (int n = 0; n < 5; ++n) w[n][m] = w[n][m + 1];
     }
 }
 
-// The compiler's own register budget: two waves per SIMD for FP64
+// The compiler's own register budget: two waves per SIMD for FP64 MDCD
 // component-wise and for FP32 faces characteristic-wise.
 template <int DIR, int INTERP, int RS, bool CHAR, class R>
 __global__ void __launch_bounds__(256) k_pencil_m(PencilArgs a, SchemeCoef<R> cf) {
     pencil_m<DIR, INTERP, RS, CHAR, R>(a, cf);
 }
 
-// FP64 characteristic-wise: held to two waves per SIMD (256 registers; a few
-// values wait in scratch across the march, outside its loop) instead of one.
+// FP64 characteristic-wise and FP64 paired reconstruction: held to two waves
+// per SIMD (256 registers, a few values in scratch) instead of the compiler's
+// one-wave allocation.
 template <int DIR, int INTERP, int RS, bool CHAR, class R>
 __global__ void __launch_bounds__(256, 2) k_pencil_m2(PencilArgs a, SchemeCoef<R> cf) {
     pencil_m<DIR, INTERP, RS, CHAR, R>(a, cf);
@@ -380,7 +418,7 @@ template <int DIR, int INTERP, int RS, bool CHAR, class R>
 inline void launch_pencil_m(dim3 g, const PencilArgs &a, const SchemeCoef<R> &cf, hipStream_t s) {
     if constexpr (sizeof(R) == 4 && !CHAR)
         hipLaunchKernelGGL((k_pencil_m3<DIR, INTERP, RS, CHAR, R>), g, dim3(64, 4), 0, s, a, cf);
-    else if constexpr (sizeof(R) == 8 && CHAR)
+    else if constexpr (sizeof(R) == 8 && (CHAR || kPairRecon<INTERP, CHAR, R>))
         hipLaunchKernelGGL((k_pencil_m2<DIR, INTERP, RS, CHAR, R>), g, dim3(64, 4), 0, s, a, cf);
     else
         hipLaunchKernelGGL((k_pencil_m<DIR, INTERP, RS, CHAR, R>), g, dim3(64, 4), 0, s, a, cf);
