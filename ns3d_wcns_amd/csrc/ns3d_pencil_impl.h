@@ -95,6 +95,18 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, const 
     __builtin_nontemporal_store(p / (rr * a.Rgas), &a.Q[4][g]);
 }
 
+// FP64 component-wise WCNS5 / WENO5-JS: the marches form both
+// reconstructions of a cell at once (recon_pair).  The FP32 faces of the
+// mixed method keep the per-face form, which measured faster at three waves
+// per SIMD.
+template <int INTERP, bool CHAR, class R>
+constexpr bool kPairRecon = sizeof(R) == 8 && !CHAR && (INTERP == I_WENO5 || INTERP == I_WENO5_JS);
+
+// The x pencil forms both reconstructions of a cell at once for
+// component-wise WCNS5 / WENO5-JS when one thread owns one face and one cell.
+template <int INTERP, bool CHAR, class R>
+constexpr bool kPairReconX = !CHAR && (INTERP == I_WENO5 || INTERP == I_WENO5_JS);
+
 // ------------------------------------------------------------------ x pencil
 // Last of the three: adds the x part to the partial RHS of the z and y
 // pencils and stores the RHS (stage 0) or updates the line in place (stage
@@ -231,12 +243,129 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
     }
 }
 
-// FP64 component-wise WCNS5 / WENO5-JS: the marches form both
-// reconstructions of a cell at once (recon_pair).  The FP32 faces of the
-// mixed method keep the per-face form, which measured faster at three waves
-// per SIMD.
-template <int INTERP, bool CHAR, class R>
-constexpr bool kPairRecon = sizeof(R) == 8 && !CHAR && (INTERP == I_WENO5 || INTERP == I_WENO5_JS);
+// LDS doubles per line of the paired x pencil: U and V on the line with three
+// periodically wrapped cells on each side (W = N + 6 slots, slot x = cell
+// x - 3), the faces -3..N+1 (N + 5 slots, slot s = face s - 3) and the right
+// states of faces -1..N-1 (N + 1 slots, slot t = face t - 1).
+__host__ __device__ inline long xpair_line_doubles(int N) { return 9L * (N + 2 * NG) + 5L * (N + 5) + 5L * (N + 1); }
+
+// The x pencil with one thread per face and cell (N threads per line, N a
+// multiple of 64, at most 512) for component-wise WCNS5 / WENO5-JS.  Thread t
+// forms both reconstructions of cell t from cells t-2..t+2 (recon_pair): the
+// left state of face t, kept in registers, and the right state of face t - 1,
+// handed over in LDS.  The face and viscous-flux arrays are padded with their
+// periodic images, so the differences index them without wrapping.  The rest
+// is k_pencil_x's ONE path: partials and U0 in registers before the face
+// work, then the sum, the stage update and the C2P/P2C round trip in place.
+template <int INTERP, int RS, class R>
+__global__ void __launch_bounds__(512) k_pencil_xp(PencilArgs a, SchemeCoef<R> cf) {
+    extern __shared__ double lds[];
+    const Box &b = a.b;
+    const int N = b.nx, W = N + 2 * NG, NF = N + 5, NX = N + 1;
+    const int Lb = blockDim.x / N;
+    const int ll = threadIdx.x / N, t = threadIdx.x % N;
+    const long line = (long)blockIdx.x * Lb + ll;
+    const bool valid = line < (long)b.ny * b.nz;
+    const int j = valid ? (int)(line % b.ny) : 0, k = valid ? (int)(line / b.ny) : 0;
+    double *Ul = lds + (long)ll * xpair_line_doubles(N);  // U[5][W], V[4][W], F[5][NF], X[5][NX]
+    double *Vl = Ul + 5 * W;
+    double *Fl = Vl + 4 * W;
+    double *Xl = Fl + 5 * NF;
+    const long row = ((long)(k + NG) * b.sy + (j + NG)) * b.sx;
+    const long orow = b.oidx(0, j, k);
+    const bool need_u0 = a.stage > 1 || (a.stage == 1 && !a.record);
+    double sp[5], zp[5];
+    if (valid) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            sp[n] = __builtin_nontemporal_load(&a.Rz[n][orow + t]) + __builtin_nontemporal_load(&a.Ry[n][orow + t]);
+            zp[n] = need_u0 ? __builtin_nontemporal_load(&a.U0[n][row + NG + t]) : 0.0;
+        }
+        // all loads in flight, then the LDS stores: slots t and t + N (< W)
+        double uv[5][2], vv[4][2];
+        const bool tail = t + N < W;
+        const int x0 = wrap(t - NG, N), x1 = wrap(t + N - NG, N);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            uv[n][0] = a.U[n][row + NG + x0];
+            uv[n][1] = tail ? a.U[n][row + NG + x1] : 0.0;
+        }
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            vv[n][0] = visc_at<R>(a.V[kViscSlot[0][n]], orow + x0);
+            vv[n][1] = tail ? visc_at<R>(a.V[kViscSlot[0][n]], orow + x1) : 0.0;
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            Ul[n * W + t] = uv[n][0];
+            if (tail) Ul[n * W + t + N] = uv[n][1];
+        }
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            Vl[n * W + t] = vv[n][0];
+            if (tail) Vl[n * W + t + N] = vv[n][1];
+        }
+    }
+    __syncthreads();
+    R yl[5];
+    if (valid) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            R f[5];
+            // E' = E - E_ref in FP64 before an FP32 cast (E_ref = 0 when plain)
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                f[m] = (kPert<R> && n == 4) ? (R)(Ul[n * W + t + 1 + m] - a.ref.E_ref) : (R)Ul[n * W + t + 1 + m];
+            R yr;
+            recon_pair<INTERP>(f, yl[n], yr);
+            Xl[n * NX + t] = (double)yr;
+            if (t == 0) Xl[n * NX + N] = (double)yr;
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        R UR[5], F[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) UR[n] = (R)Xl[n * NX + t + 1];
+        riemann<RS, 1>(cf, yl, UR, F);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            const double fv = (double)F[n];
+            Fl[n * NF + t + 3] = fv;
+            if (t < 2) Fl[n * NF + t + 3 + N] = fv;
+            if (t >= N - 3) Fl[n * NF + t + 3 - N] = fv;
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int c = t;
+    const long g = row + c + NG;
+    double r[5], u0[5], u[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        u[n] = Ul[n * W + c + NG];
+        u0[n] = need_u0 ? zp[n] : u[n];
+        const double *Fn = Fl + n * NF + c;  // faces c-3..c+2
+        r[n] = sp[n] - ((Fn[3] - Fn[2]) * (75.0 / 64.0) + (Fn[4] - Fn[1]) * (-25.0 / 384.0) +
+                        (Fn[5] - Fn[0]) * (3.0 / 640.0)) *
+                           a.ih;
+    }
+#pragma unroll
+    for (int n = 1; n < 5; ++n) {
+        const double *V = Vl + (n - 1) * W + c;  // cells c-3..c+3
+        r[n] += (V[6] - 9.0 * V[5] + 45.0 * V[4] - 45.0 * V[2] + 9.0 * V[1] - V[0]) * a.i60;
+    }
+    if (a.stage == 0) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) a.RHS[n][g] = r[n];
+    } else {
+        if (a.stage == 1 && a.record) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
+        }
+        stage_update(a, g, u0, u, r);
+    }
+}
 
 // ------------------------------------------------------------ y / z pencils
 // Thread per (x, other, segment) column marching along DIR.  Iteration f
@@ -456,6 +585,13 @@ inline void launch_pencils_t(int dir, const PencilArgs &a, const SchemeCoef<R> &
         ax.ih = a.ihd[0];
         ax.i60 = a.ihd[0] / 60.0;
         const dim3 gx((unsigned)((lines + lb - 1) / lb));
+        if constexpr (kPairReconX<INTERP, CHAR, R>) {
+            if (one) {
+                const size_t shp = (size_t)lb * xpair_line_doubles(b.nx) * sizeof(double);
+                hipLaunchKernelGGL((k_pencil_xp<INTERP, RS, R>), gx, dim3(bs), shp, s, ax, cf);
+                return;
+            }
+        }
         if (one) hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, true, R>), gx, dim3(bs), shm, s, ax, cf);
         else hipLaunchKernelGGL((k_pencil_x<INTERP, RS, CHAR, false, R>), gx, dim3(bs), shm, s, ax, cf);
     }
