@@ -43,20 +43,27 @@ __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
     for (int n = 0; n < NVISC; ++n) a.V[n][o] = V[n];
 }
 
-// The same viscous fluxes from a z-marching 64 x 4 column tile: each thread
+// The same viscous fluxes from a z-marching 64 x 8 column tile: each thread
 // keeps u, v, w, T of planes k-3..k+3 of its own column in registers (z
 // derivatives); each plane's tile with a 3-cell x/y halo is staged in LDS
 // (x and y derivatives).  Cells beyond the box are read from their periodic
 // image.
-constexpr int VZ_SEG = 16;
+// 64 x 8 tiles marching 32 planes: the plane tile with its halo is 70 x 14
+// cells per 512 outputs (1.91 loads per cell against 2.73 for 64 x 4), and
+// the 6-plane window prologue is paid once per 32 planes.  The FP64 kernel is
+// held to 128 registers, four waves per SIMD: two blocks per CU.
+constexpr int VZ_SEG = 32;
+constexpr int VT_Y = 8;      // tile rows (64 x VT_Y threads per block)
+constexpr int VT_WAVES = 4;  // minimum waves per SIMD asked of the register allocation
 
 // R = double: the plain method.  R = float: the mixed method's FP32 viscous
 // stencils; the temperature enters as T - T_ref (subtracted in FP64, T_ref =
 // 1, the nondimensional reference temperature) and the nine fluxes are
 // stored as FP32 in the first half of each FP64 buffer.
 template <class R>
-__global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
-    constexpr int TX = 64, TY = 4, HX = TX + 6, HY = TY + 6;
+__global__ void __launch_bounds__(64 * VT_Y, VT_WAVES) k_visc_march(ViscArgs a) {
+    constexpr int TX = 64, TY = VT_Y, HX = TX + 6, HY = TY + 6, NT = TX * TY;
+    constexpr int NHALO = 6 * TX + 6 * TY, NH = (NHALO + NT - 1) / NT;
     __shared__ R tile[4][HY][HX];
     const Box &b = a.b;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
@@ -80,17 +87,17 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
     for (int m = 0; m < 6; ++m)
 #pragma unroll
         for (int q = 0; q < 4; ++q) wq[q][m] = ld(q, col + (long)wrap(k0 - 3 + m, b.nz) * sxy);
-    // halo cells of a plane: 6 rows x 64 above/below, then 4 rows x 6 left/right
-    int hi[2], hj[2];
-    bool hv[2];
-    long hc[2];
+    // halo cells of a plane: 6 rows x 64 above/below, then TY rows x 6 left/right
+    int hi[NH], hj[NH];
+    bool hv[NH];
+    long hc[NH];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int e = tid + 256 * h;
+    for (int h = 0; h < NH; ++h) {
+        const int e = tid + NT * h;
         int x, y;
         if (e < 6 * TX) {
             const int r = e / TX;
-            y = r < 3 ? r - 3 : r + 1;
+            y = r < 3 ? r - 3 : TY + r - 3;
             x = e % TX;
         } else {
             const int e2 = e - 6 * TX, r = e2 / 6, c6 = e2 % 6;
@@ -99,15 +106,15 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
         }
         hi[h] = x;
         hj[h] = y;
-        hv[h] = e < 6 * TX + 6 * TY;
+        hv[h] = e < NHALO;
         hc[h] = b.gidx(per(i0 + x, b.nx), per(j0 + y, b.ny), 0);
     }
     for (int k = k0; k < k1; ++k) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) wq[q][6] = ld(q, col + (long)wrap(k + 3, b.nz) * sxy);
-        R hvv[2][4];
+        R hvv[NH][4];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < NH; ++h) {
             const long g = hc[h] + (long)k * sxy;
 #pragma unroll
             for (int q = 0; q < 4; ++q) hvv[h][q] = hv[h] ? ld(q, g) : R(0);
@@ -116,7 +123,7 @@ __global__ void __launch_bounds__(256) k_visc_march(ViscArgs a) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) tile[q][ty + 3][tx + 3] = wq[q][3];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < NH; ++h)
             if (hv[h])
 #pragma unroll
                 for (int q = 0; q < 4; ++q) tile[q][hj[h] + 3][hi[h] + 3] = hvv[h][q];
@@ -478,9 +485,9 @@ void launch_visc(const ViscArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_visc, tile_grid(a.b), cell_block(), 0, s, a);
 }
 void launch_visc_march(const ViscArgs &a, int precision, hipStream_t s) {
-    const dim3 g((a.b.nx + 63) / 64, (a.b.ny + 3) / 4, (a.b.nz + VZ_SEG - 1) / VZ_SEG);
-    if (precision == 1) hipLaunchKernelGGL(k_visc_march<float>, g, dim3(64, 4), 0, s, a);
-    else hipLaunchKernelGGL(k_visc_march<double>, g, dim3(64, 4), 0, s, a);
+    const dim3 g((a.b.nx + 63) / 64, (a.b.ny + VT_Y - 1) / VT_Y, (a.b.nz + VZ_SEG - 1) / VZ_SEG);
+    if (precision == 1) hipLaunchKernelGGL(k_visc_march<float>, g, dim3(64, VT_Y), 0, s, a);
+    else hipLaunchKernelGGL(k_visc_march<double>, g, dim3(64, VT_Y), 0, s, a);
 }
 void launch_visc_from_grad(const ViscArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_visc_from_grad, tile_grid(a.b), cell_block(), 0, s, a);
