@@ -48,8 +48,9 @@ struct FaceArgs {
 // E; ViscousFlux.cpp:187-206); the off-diagonal stresses appear twice there.
 enum { VXX = 0, VYY, VZZ, VXY, VXZ, VYZ, VEX, VEY, VEZ, NVISC };
 
-// Cells per marching segment of the y / z pencils.
-constexpr int PENCIL_SEG = 64;
+// Shortest marching segment of the y / z pencils (the launch picks the
+// segment count from the kernel's occupancy).
+constexpr int PENCIL_SEG_MIN = 32;
 
 // Arguments of the fused pencil kernels of one stage.
 struct PencilArgs {

@@ -8,8 +8,9 @@ device through the C-ABI, and the checkpointed autograd rollout.
   boxes of one and of several marching segments along every axis;
 - tangent RHS vs central differences of compute_rhs: 1e-6;
 - gradient of J = mean 0.5 rho |u|^2 at the final time with respect to the
-  initial conserved fields vs central differences (eps = 1e-6) on 8 seeded
-  unit-norm directions: 1e-6 relative;
+  initial conserved fields vs central differences (eps = 1e-5) on 8 seeded
+  unit-norm directions: 1e-6 relative (1e-7 of |grad J| for directions
+  nearly orthogonal to it);
 - checkpoint policies give the same gradient to 1e-12, and so do the
   adjoint steps with stored stage states (step_stages / adjoint_step_stages)
   and with recomputed ones;
@@ -164,18 +165,26 @@ def test_gradient_matches_central_differences(gpu):
         dts.append(f.compute_timestep())
         f.step(dts[-1])
     J0, grad = rollout_grad(f, U, dts, "uniform:5")
-    eps = 1e-6
-    worst = 0.0
+    # The central difference carries a rounding floor of about 1e-16 J / eps
+    # in absolute terms, which at eps = 1e-6 reaches 1e-6 relative on a
+    # direction whose derivative is ten times below the gradient's norm, and
+    # moves with ulp-level changes of the forward.  eps = 1e-5 keeps that
+    # floor below the bar and the O(eps^2) truncation far below it; every
+    # direction is judged against |grad J| |d| (the Cauchy-Schwarz scale),
+    # and against its own derivative unless that is below 1 % of |grad J|.
+    eps = 1e-5
+    gnorm = float(torch.linalg.vector_norm(grad[1:]))
     for k in range(8):
         d = rand_interior(U.shape, 100 + k)
         d[0] = 0.0  # directions in (rho u, rho v, rho w, E)
         d /= torch.linalg.vector_norm(d)
         fd = float(central_difference(f, U, d, dts, eps))
         ad = float((grad * d).sum())
-        rel = abs(fd - ad) / max(abs(ad), 1e-300)
-        worst = max(worst, rel)
-        print(f"direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e}")
-    assert worst < 1e-6, worst
+        rel, cs = abs(fd - ad) / max(abs(ad), 1e-300), abs(fd - ad) / gnorm
+        print(f"direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e} over |grad| {cs:.2e}")
+        assert cs < 1e-7, (k, cs)
+        if abs(ad) >= 1e-2 * gnorm:
+            assert rel < 1e-6, (k, rel)
     f.close()
 
 

@@ -169,6 +169,30 @@ def test_rk3_step(gpu, rs):
     dev.close()
 
 
+# x lines of 64 cells (one face per thread, four lines per block) and y / z
+# columns long enough for two marching segments each
+MULTI = (64, 72, 66)
+
+
+@pytest.mark.parametrize("interp,char", [(Interpolation.WENO5, False), (Interpolation.WENO5_JS, False),
+                                         (Interpolation.MDCD_HYBRID, False), (Interpolation.WENO5, True)],
+                         ids=["WENO5", "WENO5_JS", "MDCD_HYBRID", "WENO5-char"])
+def test_rk3_step_several_segments(gpu, interp, char):
+    prim, ma = state(MULTI, 0.3, noise=0.02, seed=21)
+    dev, ref = pair(MULTI, params(ma, interp=interp, char=char), prim)
+    dt = ref.compute_timestep()
+    ref.record_conserved_to0()
+    ref.runge_kutta_3(dt)
+    dev.recordConservedTo0()
+    dev.runge_kutta_3(dt)
+    for name in CONS:
+        a, b = dev.interior(dev.download(name)), ref.interior(ref.view(name))
+        b0 = ref.interior(ref.view(name + "0"))
+        assert np.max(np.abs(a - b)) / np.max(np.abs(b - b0)) < 1e-11, name
+        assert rel_err(a, b) < TOL, name
+    dev.close()
+
+
 def test_time_advance_trajectory(gpu):
     """10 steps of time_advance: dt sequence and kinetic energy."""
     prim, ma = state(16, 0.3, seed=21)
