@@ -190,8 +190,7 @@ __global__ void __launch_bounds__(256) k_tl_assemble(AdjArgs a) {
 
 // ------------------------------------------------------------ adjoint
 // Column marching along DIR: lanes run along LA (x, or y for DIR = x), block
-// rows along OA, segments of ADJ_SEG cells along DIR in grid z.
-constexpr int ADJ_SEG = 16;
+// rows along OA, segments of a.seg cells along DIR in grid z.
 
 template <int DIR>
 struct March {
@@ -200,7 +199,7 @@ struct March {
     int N, c0, c1;
     long gbase, gst, obase, ost;
     bool ok;
-    __device__ March(const Box &b) {
+    __device__ March(const Box &b, int seg) {
         const int n[3] = {b.nx, b.ny, b.nz};
         int ijk[3];
         ijk[LA] = blockIdx.x * 64 + threadIdx.x;
@@ -209,8 +208,8 @@ struct March {
         ok = ijk[LA] < n[LA] && ijk[OA] < n[OA];
         if (!ok) ijk[LA] = ijk[OA] = 0;
         N = n[DIR];
-        c0 = blockIdx.z * ADJ_SEG;
-        c1 = min(c0 + ADJ_SEG, N);
+        c0 = blockIdx.z * seg;
+        c1 = min(c0 + seg, N);
         gbase = b.gidx(ijk[0], ijk[1], ijk[2]);
         obase = b.oidx(ijk[0], ijk[1], ijk[2]);
         gst = DIR == 0 ? 1L : (DIR == 1 ? (long)b.sx : (long)b.sx * b.sy);
@@ -220,11 +219,14 @@ struct March {
     __device__ long o(int c) const { return obase + (long)wrap(c, N) * ost; }
 };
 
-template <int DIR>
-dim3 march_grid(const Box &b) {
-    const int n[3] = {b.nx, b.ny, b.nz};
+// Launch of kernel K marching along DIR, with K's segment length.
+template <int DIR, auto K>
+void launch_adj_march(AdjArgs a, hipStream_t s) {
+    const int n[3] = {a.b.nx, a.b.ny, a.b.nz};
     constexpr int LA = March<DIR>::LA, OA = March<DIR>::OA;
-    return dim3((n[LA] + 63) / 64, (n[OA] + 3) / 4, (n[DIR] + ADJ_SEG - 1) / ADJ_SEG);
+    const int gx = (n[LA] + 63) / 64, gy = (n[OA] + 3) / 4;
+    a.seg = march_seg<K>((long)gx * gy, n[DIR], ADJ_SEG_MIN);
+    hipLaunchKernelGGL(K, dim3(gx, gy, (n[DIR] + a.seg - 1) / a.seg), dim3(64, 4), 0, s, a);
 }
 
 // Per face of DIR (marching): both reconstructions of every cell (shared
@@ -233,7 +235,7 @@ dim3 march_grid(const Box &b) {
 // the face adjoint FB[DIR] -> left / right state adjoints SB[0..4], SB[5..9].
 template <int DIR, int INTERP, int RS>
 __global__ void __launch_bounds__(256) k_adj_riemann(AdjArgs a) {
-    const March<DIR> M(a.b);
+    const March<DIR> M(a.b, a.seg);
     if (!M.ok) return;
     constexpr bool JS = INTERP == I_WENO5_JS;
     constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__(256) k_adj_riemann(AdjArgs a) {
 template <int DIR, bool JS, bool FIRST>
 __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
     __shared__ double ring[5][5][256];  // [cell mod 5][var][thread]
-    const March<DIR> M(a.b);
+    const March<DIR> M(a.b, a.seg);
     const int tid = threadIdx.y * 64 + threadIdx.x;
     if (!M.ok) return;  // no barrier below: each thread owns its ring column
     double w[5][5];
@@ -308,7 +310,7 @@ __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
             for (int n = 0; n < 5; ++n) w[n][4] = a.U[n][g];
         }
         const long oc = M.o(c), om = M.o(c - 1);
-        const int base = c + 5 * ADJ_SEG;  // non-negative for the mod
+        const int base = c + 5;  // non-negative for the mod (c >= c0 - 2)
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
             double fb[5] = {0, 0, 0, 0, 0};
@@ -449,7 +451,6 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a) {
 // stencils; each plane's tile with a 3-cell x/y halo is staged in LDS for the
 // x and y stencils.  Cells beyond the box come from their periodic image, so
 // neither lambda nor the primitives need ghost layers.
-constexpr int AV_SEG = 16;  // z planes per block
 
 // Index of the periodic image of interior cell (x, y, z) for x, y in any
 // range and z in [-3, nz + 3).
@@ -496,7 +497,7 @@ __global__ void __launch_bounds__(256) k_adj_lam_march(AdjArgs a) {
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4, i = i0 + tx, j = j0 + ty;
     const bool own = i < b.nx && j < b.ny;
-    const int k0 = blockIdx.z * AV_SEG, k1 = min(k0 + AV_SEG, b.nz);
+    const int k0 = blockIdx.z * a.seg, k1 = min(k0 + a.seg, b.nz);
     const PlaneMap pm(b, i0, j0, tx, ty);
     double wl[5][7];
 #pragma unroll
@@ -568,7 +569,7 @@ __global__ void __launch_bounds__(256) k_adj_q_march(AdjArgs a) {
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4, i = i0 + tx, j = j0 + ty;
     const bool own = i < b.nx && j < b.ny;
-    const int k0 = blockIdx.z * AV_SEG, k1 = min(k0 + AV_SEG, b.nz);
+    const int k0 = blockIdx.z * a.seg, k1 = min(k0 + a.seg, b.nz);
     const PlaneMap pm(b, i0, j0, tx, ty);
     const double *Qv[4] = {a.Q[0], a.Q[1], a.Q[2], a.Q[4]};
     double wq[4][7];
@@ -713,13 +714,12 @@ void tl_face_i(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s) {
 template <int DIR>
 void adj_dir(const AdjArgs &a, int interp, int rs, hipStream_t s) {
     const bool js = interp == I_WENO5_JS, rus = rs == R_RUSANOV;
-    const dim3 g = march_grid<DIR>(a.b), bl(64, 4);
-    if (js && rus) hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5_JS, R_RUSANOV>), g, bl, 0, s, a);
-    else if (js) hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5_JS, R_HLLC>), g, bl, 0, s, a);
-    else if (rus) hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5, R_RUSANOV>), g, bl, 0, s, a);
-    else hipLaunchKernelGGL((k_adj_riemann<DIR, I_WENO5, R_HLLC>), g, bl, 0, s, a);
-    if (js) hipLaunchKernelGGL((k_adj_weno<DIR, true, DIR == 1>), g, bl, 0, s, a);
-    else hipLaunchKernelGGL((k_adj_weno<DIR, false, DIR == 1>), g, bl, 0, s, a);
+    if (js && rus) launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5_JS, R_RUSANOV>>(a, s);
+    else if (js) launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5_JS, R_HLLC>>(a, s);
+    else if (rus) launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5, R_RUSANOV>>(a, s);
+    else launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5, R_HLLC>>(a, s);
+    if (js) launch_adj_march<DIR, k_adj_weno<DIR, true, DIR == 1>>(a, s);
+    else launch_adj_march<DIR, k_adj_weno<DIR, false, DIR == 1>>(a, s);
 }
 
 void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s) {
@@ -756,9 +756,8 @@ void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s) {
     // the face and viscous gradient adjoints first, then the y and z
     // directions (y writes the interior of out, z accumulates), then the x
     // direction with the viscous part (ghosts of out are not written)
-    const dim3 gv((a.b.nx + 63) / 64, (a.b.ny + 3) / 4, (a.b.nz + AV_SEG - 1) / AV_SEG), bv(64, 4);
-    hipLaunchKernelGGL(k_adj_lam_march, gv, bv, 0, s, a);
-    hipLaunchKernelGGL(k_adj_q_march, gv, bv, 0, s, a);
+    launch_adj_march<2, k_adj_lam_march>(a, s);
+    launch_adj_march<2, k_adj_q_march>(a, s);
     adj_dir<1>(a, interp, rs, s);
     adj_dir<2>(a, interp, rs, s);
     adj_x(a, interp, rs, s);

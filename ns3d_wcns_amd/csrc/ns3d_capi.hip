@@ -325,11 +325,13 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
 
 // record: stage 1 also stores U0 (interior cells) from the state it reads.
 // The intermediate stage states are not handed out, so only the last stage
-// refills the ghost layers.
-int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false) {
+// refills the ghost layers, and only with ghosts set: the device time loops
+// run their steps without (no kernel of a step reads a ghost layer) and
+// refill once before they return.
+int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool ghosts = true) {
     int rc;
     for (int stage = 1; stage <= 3; ++stage)
-        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, stage == 3))) return rc;
+        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, ghosts && stage == 3))) return rc;
     return NS3D_OK;
 }
 
@@ -1062,11 +1064,12 @@ int ns3d_time_advance(ns3d_ctx *c, int max_steps, double dt_fixed, double total_
             break;
         }
         t += dt;
-        if ((rc = rk3(c, nullptr, dt))) return rc;
+        if ((rc = rk3(c, nullptr, dt, false, false))) return rc;
         if (dt_out) dt_out[taken] = dt;
         ++taken;
         if (t >= total_time) break;
     }
+    if (taken > 0 && (rc = halo(c, true))) return rc;
     NS3D_HIP(hipStreamSynchronize(c->stream));
     if (steps_taken) *steps_taken = taken;
     return NS3D_OK;
@@ -1077,9 +1080,10 @@ int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
     int rc;
     for (int s = 0; s < nsteps; ++s) {
         double *d_dt = d_dt_hist ? d_dt_hist + s : c->scal;
-        // recordConservedTo0 fused into the first stage
+        // recordConservedTo0 fused into the first stage; ghost layers
+        // refilled after the last step only
         if ((rc = cfl(c, d_dt, false))) return rc;
-        if ((rc = rk3(c, d_dt, 0.0, true))) return rc;
+        if ((rc = rk3(c, d_dt, 0.0, true, s == nsteps - 1))) return rc;
     }
     return NS3D_OK;
 }

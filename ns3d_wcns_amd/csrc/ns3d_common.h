@@ -48,9 +48,47 @@ struct FaceArgs {
 // E; ViscousFlux.cpp:187-206); the off-diagonal stresses appear twice there.
 enum { VXX = 0, VYY, VZZ, VXY, VXZ, VYZ, VEX, VEY, VEZ, NVISC };
 
-// Shortest marching segment of the y / z pencils (the launch picks the
-// segment count from the kernel's occupancy).
+// Shortest marching segment of the forward pencils (the launch picks the
+// segment count from the kernel's occupancy, march_seg); the adjoint's
+// marches, with fewer cells of lead-in, go down to ADJ_SEG_MIN.
 constexpr int PENCIL_SEG_MIN = 32;
+constexpr int ADJ_SEG_MIN = 16;
+
+// Segments per column of a marching launch: the blocks the device holds at
+// once (blocks per CU at the kernel's occupancy x CUs) over the blocks of one
+// segment layer, rounded, so that the launch fills the resident slots a whole
+// number of times with the fewest segments (each segment repeats its
+// stencil's lead-in); segments of at least min_len cells.  E.g. 256^3: a
+// layer is 1024 blocks; two waves per SIMD hold 2048 blocks -> 2 segments of
+// 128, three hold 3072 -> 3 segments of 86.
+inline int march_segments(int blocks_per_cu, long layer_blocks, int N, int min_len) {
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            n = 0;
+        return n;
+    }();
+    const int most = N / min_len > 1 ? N / min_len : 1;
+    if (cus <= 0 || blocks_per_cu <= 0 || layer_blocks <= 0) return (N + 63) / 64 < most ? (N + 63) / 64 : most;
+    long S = ((long)blocks_per_cu * cus + layer_blocks / 2) / layer_blocks;
+    if (S < 1) S = 1;
+    return S < most ? (int)S : most;
+}
+
+// Cells per segment for kernel K (256-thread blocks) marching columns of
+// length N, layer_blocks blocks per segment layer; K's occupancy is queried
+// once.
+template <auto K>
+inline int march_seg(long layer_blocks, int N, int min_len = PENCIL_SEG_MIN) {
+    static const int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, K, 256, 0) != hipSuccess) n = 0;
+        return n;
+    }();
+    const int S = march_segments(per_cu, layer_blocks, N, min_len);
+    return (N + S - 1) / S;
+}
 
 // Arguments of the fused pencil kernels of one stage.
 struct PencilArgs {

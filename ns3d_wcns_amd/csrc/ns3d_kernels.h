@@ -85,6 +85,7 @@ struct AdjArgs {
     SchemeCoef<double> c;
     double ih[3], i60[3];
     double gamma, Rgas, mu, S_ref, cp_pr;
+    int seg;  // cells (planes) per segment of the marching launch at hand
 };
 
 // out = alpha A + beta B + gamma P over the interior (null operands are 0).

@@ -543,48 +543,19 @@ __global__ void __launch_bounds__(256, 3) k_pencil_m3(PencilArgs a, SchemeCoef<R
     pencil_m<DIR, INTERP, RS, CHAR, R>(a, cf);
 }
 
-// Segments per column of a marching launch: the blocks the device holds at
-// once (blocks per CU at the kernel's occupancy x CUs) over the blocks of one
-// segment layer, rounded, so that the launch fills the resident slots a whole
-// number of times with the fewest segments (each segment recomputes five
-// faces); at most one segment per PENCIL_SEG_MIN cells.  E.g. 256^3: a layer
-// is 1024 blocks; two waves per SIMD (FP64) hold 2048 blocks -> 2 segments of
-// 128, three (FP32 faces) hold 3072 -> 3 segments of 86.
-inline int march_segments(int blocks_per_cu, long layer_blocks, int N) {
-    static const int cus = [] {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            n = 0;
-        return n;
-    }();
-    const int most = N / PENCIL_SEG_MIN > 1 ? N / PENCIL_SEG_MIN : 1;
-    if (cus <= 0 || blocks_per_cu <= 0 || layer_blocks <= 0) return (N + 63) / 64 < most ? (N + 63) / 64 : most;
-    long S = ((long)blocks_per_cu * cus + layer_blocks / 2) / layer_blocks;
-    if (S < 1) S = 1;
-    return S < most ? (int)S : most;
-}
-
-template <class K>
-inline int resident_blocks(K kernel) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0) != hipSuccess) n = 0;
-    return n;
-}
-
 // Launch of the DIR march over gx x gy columns of length N.
+template <auto K, class R>
+inline void launch_march(int gx, int gy, int N, PencilArgs a, const SchemeCoef<R> &cf, hipStream_t s) {
+    a.seg = march_seg<K>((long)gx * gy, N);
+    hipLaunchKernelGGL(K, dim3(gx, gy, (N + a.seg - 1) / a.seg), dim3(64, 4), 0, s, a, cf);
+}
+
 template <int DIR, int INTERP, int RS, bool CHAR, class R>
-inline void launch_pencil_m(int gx, int gy, int N, PencilArgs a, const SchemeCoef<R> &cf, hipStream_t s) {
-    auto go = [&](auto kernel) {
-        static const int per_cu = resident_blocks(kernel);
-        const int S = march_segments(per_cu, (long)gx * gy, N);
-        a.seg = (N + S - 1) / S;
-        const dim3 g(gx, gy, (N + a.seg - 1) / a.seg);
-        hipLaunchKernelGGL(kernel, g, dim3(64, 4), 0, s, a, cf);
-    };
-    if constexpr (sizeof(R) == 4 && !CHAR) go(k_pencil_m3<DIR, INTERP, RS, CHAR, R>);
-    else if constexpr (sizeof(R) == 8 && (CHAR || kPairRecon<INTERP, CHAR, R>)) go(k_pencil_m2<DIR, INTERP, RS, CHAR, R>);
-    else go(k_pencil_m<DIR, INTERP, RS, CHAR, R>);
+inline void launch_pencil_m(int gx, int gy, int N, const PencilArgs &a, const SchemeCoef<R> &cf, hipStream_t s) {
+    if constexpr (sizeof(R) == 4 && !CHAR) launch_march<k_pencil_m3<DIR, INTERP, RS, CHAR, R>>(gx, gy, N, a, cf, s);
+    else if constexpr (sizeof(R) == 8 && (CHAR || kPairRecon<INTERP, CHAR, R>))
+        launch_march<k_pencil_m2<DIR, INTERP, RS, CHAR, R>>(gx, gy, N, a, cf, s);
+    else launch_march<k_pencil_m<DIR, INTERP, RS, CHAR, R>>(gx, gy, N, a, cf, s);
 }
 
 // ------------------------------------------------------------------ launch
