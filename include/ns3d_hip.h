@@ -267,7 +267,8 @@ int ns3d_adjoint_step(ns3d_ctx *ctx, double dt, double *const *d_adj);
 /* The same step pair with the RK3 stage states kept by the caller (five
  * ghosted device fields each), so that the adjoint need not recompute them:
  *   step_stages:         recordConservedTo0 + one runge_kutta_3 step with dt,
- *                        storing U^(1) and U^(2);
+ *                        storing U^(1) and U^(2) (interior cells; their ghost
+ *                        layers are not written);
  *   adjoint_step_stages: adjoint_step of the step from the current state U^n
  *                        whose stages step_stages stored; the state stays U^n. */
 int ns3d_step_stages(ns3d_ctx *ctx, double dt, double *const *d_U1, double *const *d_U2);

@@ -270,11 +270,22 @@ int halo(ns3d_ctx *c, bool all) {
     return NS3D_OK;
 }
 
+// State arrays of a fused stage other than the context's own: the stage reads
+// Uin (its primitives in the context), writes Uout and takes U0 as the
+// recorded state (stage 1 without record, stages 2 and 3).
+struct StageIO {
+    const double *const *Uin;
+    double *const *Uout;
+    double *const *U0;
+};
+
 // One fused stage (stage 0: RHS only): viscous fluxes, then the z, y and x
-// pencils; stages 1..3 update the state in place.  The kernels read no ghost
-// layer (periodic images instead); with ghosts the layers of the new state are
-// refilled for the callers that read them.
-int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false, bool ghosts = true) {
+// pencils; stages 1..3 update the state in place, or from io->Uin into
+// io->Uout.  The kernels read no ghost layer (periodic images instead); with
+// ghosts the layers of the new state (the context's) are refilled for the
+// callers that read them.
+int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false, bool ghosts = true,
+                const StageIO *io = nullptr) {
     {
         Timed t(c, NS3D_K_VISCOUS);
         launch_visc_march(visc_args(c, false, true), c->precision, c->stream);
@@ -283,10 +294,10 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     c->visc_fp32 = c->precision == NS3D_PRECISION_MIXED;
     PencilArgs a{};
     for (int n = 0; n < 5; ++n) {
-        a.U[n] = conserved(c)[n];
+        a.U[n] = io ? io->Uin[n] : conserved(c)[n];
         a.RHS[n] = rhs(c)[n];
-        a.U0[n] = recorded(c)[n];
-        a.Unew[n] = conserved(c)[n];  // updated in place by the x pencil
+        a.U0[n] = io ? io->U0[n] : recorded(c)[n];
+        a.Unew[n] = io ? io->Uout[n] : conserved(c)[n];  // in place by default (the x pencil owns its line)
         a.Q[n] = prims(c)[n];
         a.Rz[n] = c->rz[n];
         a.Ry[n] = c->ry[n];
@@ -320,6 +331,7 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     }
     NS3D_LAUNCHED("pencils");
     if (stage == 0 || !ghosts) return NS3D_OK;
+    if (io && io->Uout[0] != conserved(c)[0]) return fail(NS3D_E_ARG, "internal: ghost refill of a foreign state");
     return halo(c, true);
 }
 
@@ -444,6 +456,17 @@ int ensure_adj(ns3d_ctx *c) {
     return NS3D_OK;
 }
 
+// The context's primitives from the state U (interior cells are the ones
+// read; U is not written).
+int prims_from(ns3d_ctx *c, const double *const *U) {
+    ConvArgs a = conv_args(c);
+    for (int n = 0; n < 5; ++n) a.U[n] = const_cast<double *>(U[n]);
+    a.keep = 1;
+    launch_c2p(a, c->stream);
+    NS3D_LAUNCHED("c2p");
+    return NS3D_OK;
+}
+
 // Primitives and ghost layers consistent with the conserved state.
 int refresh_state(ns3d_ctx *c) {
     launch_c2p(conv_args(c), c->stream);
@@ -522,9 +545,14 @@ int tangent_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
 
 // Adjoint RHS (interior of out) at the current state, of the adjoint X
 // (interior read; stencils take periodic images).
-int adjoint_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
+// The adjoint RHS about the state U (the context's own by default), whose
+// primitives the context holds.
+int adjoint_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out, const double *const *U = nullptr) {
     AdjArgs a = adj_args(c, out);
-    for (int n = 0; n < 5; ++n) a.X[n] = X[n];
+    for (int n = 0; n < 5; ++n) {
+        a.X[n] = X[n];
+        if (U) a.U[n] = U[n];
+    }
     launch_adj_rhs(a, c->P.interpolation, c->P.riemann_solver, c->stream);
     NS3D_LAUNCHED("adjoint rhs");
     return NS3D_OK;
@@ -579,26 +607,28 @@ int check_adjoint(ns3d_ctx *c) {
     return NS3D_OK;
 }
 
-// The reverse of one RK3 step, the context holding U^(2) (primitives and
-// ghosts current) and U^n recorded; U1: U^(1).  adj: Ub^3 on entry, Ub^n on
-// return; the context is left at U^n.
-int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1) {
+// The reverse of one RK3 step: the context holds U^n, untouched, and the
+// primitives of U^(2); U1, U2: U^(1), U^(2) (interior cells are read; every
+// adjoint kernel reads periodic images, not ghost layers).  adj: Ub^3 on
+// entry, Ub^n on return (ghosts zero); the context's primitives are left
+// those of U^n, so the context is at U^n as on entry.
+int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1,
+                    const double *const *U2) {
     int rc;
-    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3 (the adjoint
-    // kernels read periodic images, not ghosts; the ghosts of adj are zeroed
-    // on return)
-    if ((rc = adjoint_rhs_impl(c, adj, c->pr))) return rc;
+    // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3
+    if ((rc = adjoint_rhs_impl(c, adj, c->pr, U2))) return rc;
     launch_lincomb2(lc(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr),
                     lc(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr), c->stream);
     NS3D_LAUNCHED("lincomb");
     // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
-    if ((rc = copy_fields(c, conserved(c), U1, 5)) || (rc = refresh_state(c))) return rc;
-    if ((rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
+    if ((rc = prims_from(c, U1)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr, U1))) return rc;
     launch_lincomb2(lc(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr),
                     lc(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr), c->stream);
     NS3D_LAUNCHED("lincomb");
-    // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P
-    if ((rc = copy_fields(c, conserved(c), recorded(c), 5)) || (rc = refresh_state(c))) return rc;
+    // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P (the primitives of U^n over
+    // every cell, ghosts included, as the context had them on entry)
+    launch_c2p(conv_args(c), c->stream);
+    NS3D_LAUNCHED("c2p");
     if ((rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
     launch_lincomb_fill(lc(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr), c->stream);
     NS3D_LAUNCHED("lincomb");
@@ -681,22 +711,30 @@ int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
     NS3D_CTX(c);
     int rc;
     if ((rc = check_ptrs((const void *const *)adj, 5))) return rc;
-    if ((rc = check_adjoint(c)) || (rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = record0(c))) return rc;
-    // forward stages 1 and 2 (U^(1) kept, U^(2) left in the context)
-    if ((rc = fused_stage(c, 1, nullptr, dt)) || (rc = copy_fields(c, c->s1, conserved(c), 5)) ||
-        (rc = fused_stage(c, 2, nullptr, dt)))
+    if ((rc = check_adjoint(c)) || (rc = ensure_adj(c))) return rc;
+    launch_c2p(conv_args(c), c->stream);  // the primitives of U^n
+    NS3D_LAUNCHED("c2p");
+    // forward stages 1 and 2 from U^n, which stays in the context: U^(1) into
+    // s1, U^(2) into xg (free during a step's adjoint); the ghost layers of
+    // the stage states are not formed (nothing reads them)
+    const StageIO st1{conserved(c), c->s1, conserved(c)}, st2{c->s1, c->xg, conserved(c)};
+    if ((rc = fused_stage(c, 1, nullptr, dt, false, false, &st1)) ||
+        (rc = fused_stage(c, 2, nullptr, dt, false, false, &st2)))
         return rc;
-    return adjoint_reverse(c, dt, adj, c->s1);
+    return adjoint_reverse(c, dt, adj, c->s1, c->xg);
 }
 
 int ns3d_step_stages(ns3d_ctx *c, double dt, double *const *U1, double *const *U2) {
     NS3D_CTX(c);
     int rc;
     if ((rc = check_ptrs((const void *const *)U1, 5)) || (rc = check_ptrs((const void *const *)U2, 5))) return rc;
-    if ((rc = record0(c)) || (rc = fused_stage(c, 1, nullptr, dt)) || (rc = copy_fields(c, U1, conserved(c), 5)) ||
-        (rc = fused_stage(c, 2, nullptr, dt)) || (rc = copy_fields(c, U2, conserved(c), 5)))
+    // the stages written straight into the caller's arrays (interior cells),
+    // U0 recorded by the stage-1 x pencil
+    const StageIO st1{conserved(c), U1, recorded(c)}, st2{U1, U2, recorded(c)}, st3{U2, conserved(c), recorded(c)};
+    if ((rc = fused_stage(c, 1, nullptr, dt, true, false, &st1)) ||
+        (rc = fused_stage(c, 2, nullptr, dt, false, false, &st2)))
         return rc;
-    return fused_stage(c, 3, nullptr, dt);
+    return fused_stage(c, 3, nullptr, dt, false, true, &st3);
 }
 
 int ns3d_adjoint_step_stages(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1,
@@ -706,10 +744,10 @@ int ns3d_adjoint_step_stages(ns3d_ctx *c, double dt, double *const *adj, const d
     if ((rc = check_ptrs((const void *const *)adj, 5)) || (rc = check_ptrs((const void *const *)U1, 5)) ||
         (rc = check_ptrs((const void *const *)U2, 5)))
         return rc;
-    if ((rc = check_adjoint(c)) || (rc = ensure_adj(c)) || (rc = record0(c))) return rc;
-    // U^(2) from the caller: primitives and ghosts formed as after a stage
-    if ((rc = copy_fields(c, conserved(c), U2, 5)) || (rc = refresh_state(c))) return rc;
-    return adjoint_reverse(c, dt, adj, U1);
+    if ((rc = check_adjoint(c)) || (rc = ensure_adj(c))) return rc;
+    // the primitives of U^(2) from the caller's array; U^n stays in the context
+    if ((rc = prims_from(c, U2))) return rc;
+    return adjoint_reverse(c, dt, adj, U1, U2);
 }
 
 int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params, int precision) {

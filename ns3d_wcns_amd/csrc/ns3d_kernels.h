@@ -39,6 +39,7 @@ struct ConvArgs {
     double *Q[5];
     long n;
     double gamma, Rgas;
+    int keep;  // C2P: leave U untouched (no store of the clamped density)
 };
 
 struct HaloArgs {

@@ -266,7 +266,7 @@ __global__ void k_c2p(ConvArgs a) {
     double rr = a.U[0][id];
     if (rr <= 0.0) {
         rr = 1e-12;
-        a.U[0][id] = rr;
+        if (!a.keep) a.U[0][id] = rr;
     }
     const double u = a.U[1][id] / rr, v = a.U[2][id] / rr, w = a.U[3][id] / rr;
     const double kin = 0.5 * (u * u + v * v + w * w);
