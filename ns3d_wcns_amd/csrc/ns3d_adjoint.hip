@@ -339,6 +339,16 @@ __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
     }
 }
 
+// alpha A + beta B + gamma p at ghosted cell g, component n (null operands
+// count as zero; p enters where l.P is set), in this order.
+__device__ __forceinline__ double lin_at(const LinComb &l, int n, long g, double p) {
+    double r = 0.0;
+    if (l.A[n]) r += l.alpha * l.A[n][g];
+    if (l.B[n]) r += l.beta * l.B[n][g];
+    if (l.P[n]) r += l.gamma * p;
+    return r;
+}
+
 // The x direction, one block per x line (threads = cells, nx <= 512; 82 KB of
 // LDS at 512, within gfx950's 160 KB per workgroup): the
 // line and its 2-cell periodic halo in LDS; both reconstructions of every
@@ -347,9 +357,10 @@ __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
 // one thread); per variable the reverse of the reconstructions into LDS and
 // the gather of the five contributions of each cell; then the viscous part
 // (transposed gradient stencil and the reverse of the conserved ->
-// primitive map), added to out.
+// primitive map), added to out; with a tail, the cell's total goes into the
+// tail's combinations instead of out.
 template <bool JS, int RS>
-__global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a) {
+__global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a, LinTail tl) {
     extern __shared__ double lds[];
     const Box &b = a.b;
     const int N = b.nx, W = N + 4;
@@ -441,8 +452,23 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a) {
         acc[n] += a.out[n][g];
     }
     c2p_vjp(a.gamma, a.Rgas, U, qb, acc);
+    if (tl.n == 0) {
 #pragma unroll
-    for (int n = 0; n < 5; ++n) a.out[n][g] = acc[n];
+        for (int n = 0; n < 5; ++n) a.out[n][g] = acc[n];
+        return;
+    }
+    // the stage combinations (only this cell of their operands is read, and
+    // no kernel of this RHS reads what they write)
+    double v[2][5];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) v[m][n] = m < tl.n ? lin_at(tl.c[m], n, g, acc[n]) : 0.0;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+        if (m < tl.n)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) tl.c[m].out[n][g] = v[m][n];
 }
 
 // The face and viscous adjoints of one adjoint RHS, as two z-marching
@@ -627,11 +653,7 @@ __global__ void __launch_bounds__(256) k_adj_q_march(AdjArgs a) {
 // out = alpha A + beta B + gamma P on the interior cells; a null operand
 // counts as zero, and out may alias any operand.
 __device__ __forceinline__ double lincomb_at(const LinComb &l, int n, long g) {
-    double r = 0.0;
-    if (l.A[n]) r += l.alpha * l.A[n][g];
-    if (l.B[n]) r += l.beta * l.B[n][g];
-    if (l.P[n]) r += l.gamma * l.P[n][g];
-    return r;
+    return lin_at(l, n, g, l.P[n] ? l.P[n][g] : 0.0);
 }
 
 // Interior cell of a thread (32-bit index arithmetic: the box has fewer than
@@ -653,30 +675,16 @@ __global__ void __launch_bounds__(256) k_lincomb(LinComb l) {
     for (int n = 0; n < 5; ++n) l.out[n][g] = lincomb_at(l, n, g);
 }
 
-// Two combinations in one pass, both formed before either is stored (l2 may
-// read what l1 writes and the reverse).
-__global__ void __launch_bounds__(256) k_lincomb2(LinComb l1, LinComb l2) {
-    int i, j, k;
-    if (!cell_of32(l1.b, blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    const long g = l1.b.gidx(i, j, k);
-#pragma unroll
-    for (int n = 0; n < 5; ++n) {
-        const double r1 = lincomb_at(l1, n, g), r2 = lincomb_at(l2, n, g);
-        l1.out[n][g] = r1;
-        l2.out[n][g] = r2;
-    }
-}
-
-// The combination on the interior and zero on the ghost layers, over the
-// whole ghosted box.
-__global__ void __launch_bounds__(256) k_lincomb_fill(LinComb l) {
+// Zero on the ghost layers of l.out (the interior untouched), over the whole
+// ghosted box.
+__global__ void __launch_bounds__(256) k_zero_ghosts(LinComb l) {
     const Box &b = l.b;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= b.sx * b.sy * b.sz) return;
     const int x = id % b.sx, r = id / b.sx, y = r % b.sy, z = r / b.sy;
-    const bool in = x >= NG && x < NG + b.nx && y >= NG && y < NG + b.ny && z >= NG && z < NG + b.nz;
+    if (x >= NG && x < NG + b.nx && y >= NG && y < NG + b.ny && z >= NG && z < NG + b.nz) return;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) l.out[n][id] = in ? lincomb_at(l, n, id) : 0.0;
+    for (int n = 0; n < 5; ++n) l.out[n][id] = 0.0;
 }
 
 // ------------------------------------------------------------ launchers
@@ -722,15 +730,15 @@ void adj_dir(const AdjArgs &a, int interp, int rs, hipStream_t s) {
     else launch_adj_march<DIR, k_adj_weno<DIR, false, DIR == 1>>(a, s);
 }
 
-void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s) {
+void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s, const LinTail &t) {
     const bool js = interp == I_WENO5_JS, rus = rs == R_RUSANOV;
     const int N = a.b.nx;
     const dim3 g((unsigned)((long)a.b.ny * a.b.nz)), bl((unsigned)(((N + 63) / 64) * 64));
     const size_t shm = (size_t)(5 * (N + 4) + 15 * N) * sizeof(double);
-    if (js && rus) hipLaunchKernelGGL((k_adj_xline<true, R_RUSANOV>), g, bl, shm, s, a);
-    else if (js) hipLaunchKernelGGL((k_adj_xline<true, R_HLLC>), g, bl, shm, s, a);
-    else if (rus) hipLaunchKernelGGL((k_adj_xline<false, R_RUSANOV>), g, bl, shm, s, a);
-    else hipLaunchKernelGGL((k_adj_xline<false, R_HLLC>), g, bl, shm, s, a);
+    if (js && rus) hipLaunchKernelGGL((k_adj_xline<true, R_RUSANOV>), g, bl, shm, s, a, t);
+    else if (js) hipLaunchKernelGGL((k_adj_xline<true, R_HLLC>), g, bl, shm, s, a, t);
+    else if (rus) hipLaunchKernelGGL((k_adj_xline<false, R_RUSANOV>), g, bl, shm, s, a, t);
+    else hipLaunchKernelGGL((k_adj_xline<false, R_HLLC>), g, bl, shm, s, a, t);
 }
 }  // namespace
 
@@ -752,7 +760,7 @@ void launch_tl_rhs(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s
     hipLaunchKernelGGL(k_tl_assemble, cgrid(a.b), dim3(256), 0, s, a);
 }
 
-void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s) {
+void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s, const LinTail &tail) {
     // the face and viscous gradient adjoints first, then the y and z
     // directions (y writes the interior of out, z accumulates), then the x
     // direction with the viscous part (ghosts of out are not written)
@@ -760,20 +768,16 @@ void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s) {
     launch_adj_march<2, k_adj_q_march>(a, s);
     adj_dir<1>(a, interp, rs, s);
     adj_dir<2>(a, interp, rs, s);
-    adj_x(a, interp, rs, s);
+    adj_x(a, interp, rs, s, tail);
 }
 
 void launch_lincomb(const LinComb &l, hipStream_t s) {
     hipLaunchKernelGGL(k_lincomb, cgrid(l.b), dim3(256), 0, s, l);
 }
 
-void launch_lincomb2(const LinComb &l1, const LinComb &l2, hipStream_t s) {
-    hipLaunchKernelGGL(k_lincomb2, cgrid(l1.b), dim3(256), 0, s, l1, l2);
-}
-
-void launch_lincomb_fill(const LinComb &l, hipStream_t s) {
+void launch_zero_ghosts(const LinComb &l, hipStream_t s) {
     const long cells = (long)l.b.sx * l.b.sy * l.b.sz;
-    hipLaunchKernelGGL(k_lincomb_fill, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, l);
+    hipLaunchKernelGGL(k_zero_ghosts, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, l);
 }
 
 }  // namespace ns3d

@@ -547,13 +547,18 @@ int tangent_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
 // (interior read; stencils take periodic images).
 // The adjoint RHS about the state U (the context's own by default), whose
 // primitives the context holds.
-int adjoint_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out, const double *const *U = nullptr) {
+// With a tail, the RHS is not stored: its value at each cell goes into the
+// tail's combinations.
+int adjoint_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out, const double *const *U = nullptr,
+                     const LinTail *tail = nullptr) {
     AdjArgs a = adj_args(c, out);
     for (int n = 0; n < 5; ++n) {
         a.X[n] = X[n];
         if (U) a.U[n] = U[n];
     }
-    launch_adj_rhs(a, c->P.interpolation, c->P.riemann_solver, c->stream);
+    LinTail t{};
+    if (tail) t = *tail;
+    launch_adj_rhs(a, c->P.interpolation, c->P.riemann_solver, c->stream, t);
     NS3D_LAUNCHED("adjoint rhs");
     return NS3D_OK;
 }
@@ -615,23 +620,27 @@ int check_adjoint(ns3d_ctx *c) {
 int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1,
                     const double *const *U2) {
     int rc;
+    // Each adjoint RHS P hands its value to the stage combinations in its
+    // last kernel (the P operand marks where it enters).
+    LinTail t{};
     // stage 3 at U^(2): Ub2 = 2/3 (Ub3 + dt P), Ub0 = Ub3 / 3
-    if ((rc = adjoint_rhs_impl(c, adj, c->pr, U2))) return rc;
-    launch_lincomb2(lc(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr),
-                    lc(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr), c->stream);
-    NS3D_LAUNCHED("lincomb");
+    t.n = 2;
+    t.c[0] = lc(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr);
+    t.c[1] = lc(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr);
+    if ((rc = adjoint_rhs_impl(c, adj, c->pr, U2, &t))) return rc;
     // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
-    if ((rc = prims_from(c, U1)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr, U1))) return rc;
-    launch_lincomb2(lc(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr),
-                    lc(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr), c->stream);
-    NS3D_LAUNCHED("lincomb");
+    t.c[0] = lc(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr);
+    t.c[1] = lc(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr);
+    if ((rc = prims_from(c, U1)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr, U1, &t))) return rc;
     // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P (the primitives of U^n over
     // every cell, ghosts included, as the context had them on entry)
     launch_c2p(conv_args(c), c->stream);
     NS3D_LAUNCHED("c2p");
-    if ((rc = adjoint_rhs_impl(c, c->a1, c->pr))) return rc;
-    launch_lincomb_fill(lc(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr), c->stream);
-    NS3D_LAUNCHED("lincomb");
+    t.n = 1;
+    t.c[0] = lc(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
+    if ((rc = adjoint_rhs_impl(c, c->a1, c->pr, nullptr, &t))) return rc;
+    launch_zero_ghosts(lc(c, adj, 0.0, nullptr, 0.0, nullptr, 0.0, nullptr), c->stream);
+    NS3D_LAUNCHED("zero ghosts");
     return NS3D_OK;
 }
 

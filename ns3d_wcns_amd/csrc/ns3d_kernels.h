@@ -101,10 +101,16 @@ bool adjoint_supported(int interp, int rs, bool chr);
 int adjoint_max_nx();  // the x-line adjoint kernel holds one line per block
 void launch_tl_prim(const AdjArgs &a, hipStream_t s);
 void launch_tl_rhs(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s);
-void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s);
+// Combinations formed from the finished adjoint RHS value r of each cell in
+// place of storing r: c[m].out = alpha A + beta B (+ gamma r where c[m].P is
+// set), m < n, both formed before either is stored; n = 0 stores r into out.
+struct LinTail {
+    LinComb c[2];
+    int n;
+};
+void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s, const LinTail &tail);
 void launch_lincomb(const LinComb &l, hipStream_t s);
-void launch_lincomb2(const LinComb &l1, const LinComb &l2, hipStream_t s);  // both in one pass
-void launch_lincomb_fill(const LinComb &l, hipStream_t s);                  // interior; ghosts set to 0
+void launch_zero_ghosts(const LinComb &l, hipStream_t s);  // l.out's ghost layers set to 0
 
 long ke_partial_count(const Box &b);
 void launch_visc(const ViscArgs &a, hipStream_t s);
