@@ -349,10 +349,63 @@ __device__ __forceinline__ double lin_at(const LinComb &l, int n, long g, double
     return r;
 }
 
-// The x direction, one block per x line (threads = cells, nx <= 512; 82 KB of
-// LDS at 512, within gfx950's 160 KB per workgroup): the
-// line and its 2-cell periodic halo in LDS; both reconstructions of every
-// cell; the reverse of the Riemann flux of every face (state adjoints
+// The viscous part of the x-direction adjoint at cell (i, j, k): the
+// transposed gradient stencil and the reverse of the conserved -> primitive
+// map added to the cell's face part acc, and the total added to out, or
+// handed to the tail's combinations.
+__device__ __forceinline__ void xline_tail(const AdjArgs &a, const LinTail &tl, int i, int j, int k, double *acc) {
+    const Box &b = a.b;
+    const long g = b.gidx(i, j, k), o = b.oidx(i, j, k);
+    double qb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) qb[q] = a.qb[q][o];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        long vo[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) vo[m] = owrap(b, i, j, k, d, m - 3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double *G = a.Gb[3 * q + d];
+            double sum = 0.0;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) sum += kD[m] * G[vo[m]];
+            qb[q] -= sum * a.i60[d];
+        }
+    }
+    double U[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        U[n] = a.U[n][g];
+        acc[n] += a.out[n][g];
+    }
+    c2p_vjp(a.gamma, a.Rgas, U, qb, acc);
+    if (tl.n == 0) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) a.out[n][g] = acc[n];
+        return;
+    }
+    // the stage combinations (only this cell of their operands is read, and
+    // no kernel of this RHS reads what they write)
+    double v[2][5];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) v[m][n] = m < tl.n ? lin_at(tl.c[m], n, g, acc[n]) : 0.0;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+        if (m < tl.n)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) tl.c[m].out[n][g] = v[m][n];
+}
+
+// Longest x line held whole by one block of the x-direction adjoint; longer
+// lines go in segments (k_adj_xseg).
+constexpr int XSEG = 256;
+
+// The x direction for lines of up to XSEG cells, one block per x line
+// (threads = cells): the line and its 2-cell periodic halo in LDS; both
+// reconstructions of every cell; the reverse of the Riemann flux of every face (state adjoints
 // written over the states they replace: each slot is read and written by
 // one thread); per variable the reverse of the reconstructions into LDS and
 // the gather of the five contributions of each cell; then the viscous part
@@ -426,49 +479,90 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a, LinTail tl) {
         __syncthreads();  // Cc is rewritten by the next variable
     }
     if (!own) return;
-    const int i = t;
-    const long g = b.gidx(i, j, k);
-    double qb[4];
+    xline_tail(a, tl, t, j, k, acc);
+}
+
+// The x direction for lines longer than XSEG cells: one block of XSEG
+// threads per segment of S <= XSEG cells, which recomputes the reach of the
+// reconstructions on either side (periodic images): the state of cells
+// s0-5..s0+S+4, both reconstructions of cells s0-3..s0+S+2, the Riemann
+// reverse of faces (s0-3)+1/2..(s0+S+1)+1/2 and the reconstruction reverse of
+// cells s0-2..s0+S+1, whose contributions reach the segment.  Per cell the
+// arithmetic and its order are those of k_adj_xline.
+
+__host__ __device__ inline long xseg_doubles(int S) { return 5L * (S + 10) + 10L * (S + 6) + 5L * (S + 4); }
+
+template <bool JS, int RS>
+__global__ void __launch_bounds__(XSEG) k_adj_xseg(AdjArgs a, LinTail tl, int nseg, int S) {
+    extern __shared__ double lds[];
+    const Box &b = a.b;
+    const int N = b.nx, W = S + 10, R = S + 6, C = S + 4;
+    const int t = threadIdx.x;
+    const int seg = blockIdx.x % nseg;
+    const long line = blockIdx.x / nseg;
+    const int j = (int)(line % b.ny), k = (int)(line / b.ny);
+    const int s0 = seg * S, Sl = min(S, N - s0);  // this segment: cells s0..s0+Sl-1
+    double *Ul = lds;          // [5][W]: slot x = cell s0 - 5 + x
+    double *A = Ul + 5 * W;    // [10][R]: slot r = cell s0 - 3 + r
+    double *Cc = A + 10 * R;   // [5][C]: slot q = cell s0 - 2 + q
+    const long grow = b.gidx(0, j, k);
+    for (int x = t; x < Sl + 10; x += XSEG) {
+        const long g = grow + wrap(s0 - 5 + x, N);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) qb[q] = a.qb[q][o];
+        for (int n = 0; n < 5; ++n) Ul[n * W + x] = a.U[n][g];
+    }
+    __syncthreads();
+    for (int r = t; r < Sl + 6; r += XSEG) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        long vo[7];
-#pragma unroll
-        for (int m = 0; m < 7; ++m) vo[m] = owrap(b, i, j, k, d, m - 3);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double *G = a.Gb[3 * q + d];
-            double sum = 0.0;
-#pragma unroll
-            for (int m = 0; m < 7; ++m) sum += kD[m] * G[vo[m]];
-            qb[q] -= sum * a.i60[d];
+        for (int n = 0; n < 5; ++n) {
+            const double f[5] = {Ul[n * W + r], Ul[n * W + r + 1], Ul[n * W + r + 2], Ul[n * W + r + 3],
+                                 Ul[n * W + r + 4]};
+            weno_pair<JS>(f, A[n * R + r], A[(5 + n) * R + r]);
         }
     }
-    double U[5];
+    __syncthreads();
+    // face (s0-3+q)+1/2: left state of slot q, right state of slot q + 1; each
+    // slot is read and written by one face
+    for (int q = t; q < Sl + 5; q += XSEG) {
+        const long of = b.oidx(wrap(s0 - 3 + q, N), j, k);
+        double UL[5], UR[5], Fb[5], ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-        U[n] = a.U[n][g];
-        acc[n] += a.out[n][g];
+        for (int n = 0; n < 5; ++n) {
+            UL[n] = A[n * R + q];
+            UR[n] = A[(5 + n) * R + q + 1];
+            Fb[n] = a.FB[0][n][of];
+        }
+        if constexpr (RS == R_HLLC) hllc_vjp(a.c, UL, UR, Fb, ULb, URb);
+        else rusanov_vjp(a.c, UL, UR, Fb, ULb, URb);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            A[n * R + q] = ULb[n];
+            A[(5 + n) * R + q + 1] = URb[n];
+        }
     }
-    c2p_vjp(a.gamma, a.Rgas, U, qb, acc);
-    if (tl.n == 0) {
+    __syncthreads();
+    double acc[5];
 #pragma unroll
-        for (int n = 0; n < 5; ++n) a.out[n][g] = acc[n];
-        return;
+    for (int v = 0; v < 5; ++v) {
+        for (int q = t; q < Sl + 4; q += XSEG) {  // cell s0-2+q: state slots q+1..q+5, recon slot q+1
+            const double f[5] = {Ul[v * W + q + 1], Ul[v * W + q + 2], Ul[v * W + q + 3], Ul[v * W + q + 4],
+                                 Ul[v * W + q + 5]};
+            double fb[5] = {0, 0, 0, 0, 0};
+            weno_pair_vjp<JS>(f, A[v * R + q + 1], A[(5 + v) * R + q + 1], fb);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) Cc[m * C + q] = fb[m];
+        }
+        __syncthreads();
+        if (t < Sl) {  // cell s0+t gathers output m of cell s0+t+2-m (slot t+4-m)
+            double sum = 0.0;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) sum += Cc[m * C + t + 4 - m];
+            acc[v] = sum;
+        }
+        __syncthreads();  // Cc is rewritten by the next variable
     }
-    // the stage combinations (only this cell of their operands is read, and
-    // no kernel of this RHS reads what they write)
-    double v[2][5];
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-#pragma unroll
-        for (int n = 0; n < 5; ++n) v[m][n] = m < tl.n ? lin_at(tl.c[m], n, g, acc[n]) : 0.0;
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-        if (m < tl.n)
-#pragma unroll
-            for (int n = 0; n < 5; ++n) tl.c[m].out[n][g] = v[m][n];
+    if (t >= Sl) return;
+    xline_tail(a, tl, s0 + t, j, k, acc);
 }
 
 // The face and viscous adjoints of one adjoint RHS, as two z-marching
@@ -733,6 +827,16 @@ void adj_dir(const AdjArgs &a, int interp, int rs, hipStream_t s) {
 void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s, const LinTail &t) {
     const bool js = interp == I_WENO5_JS, rus = rs == R_RUSANOV;
     const int N = a.b.nx;
+    if (N > XSEG) {  // segments of at most XSEG cells, balanced over the line
+        const int nseg = (N + XSEG - 1) / XSEG, S = (N + nseg - 1) / nseg;
+        const dim3 g((unsigned)((long)a.b.ny * a.b.nz * nseg)), bl(XSEG);
+        const size_t shm = (size_t)xseg_doubles(S) * sizeof(double);
+        if (js && rus) hipLaunchKernelGGL((k_adj_xseg<true, R_RUSANOV>), g, bl, shm, s, a, t, nseg, S);
+        else if (js) hipLaunchKernelGGL((k_adj_xseg<true, R_HLLC>), g, bl, shm, s, a, t, nseg, S);
+        else if (rus) hipLaunchKernelGGL((k_adj_xseg<false, R_RUSANOV>), g, bl, shm, s, a, t, nseg, S);
+        else hipLaunchKernelGGL((k_adj_xseg<false, R_HLLC>), g, bl, shm, s, a, t, nseg, S);
+        return;
+    }
     const dim3 g((unsigned)((long)a.b.ny * a.b.nz)), bl((unsigned)(((N + 63) / 64) * 64));
     const size_t shm = (size_t)(5 * (N + 4) + 15 * N) * sizeof(double);
     if (js && rus) hipLaunchKernelGGL((k_adj_xline<true, R_RUSANOV>), g, bl, shm, s, a, t);
@@ -746,7 +850,6 @@ bool adjoint_supported(int interp, int rs, bool chr) {
     return !chr && (interp == I_WENO5 || interp == I_WENO5_JS) && (rs == R_HLLC || rs == R_RUSANOV);
 }
 
-int adjoint_max_nx() { return 512; }  // 5 (nx + 4) + 15 nx doubles of LDS per block
 
 void launch_tl_prim(const AdjArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_tl_prim, cgrid(a.b), dim3(256), 0, s, a);

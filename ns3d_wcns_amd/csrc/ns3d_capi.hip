@@ -601,9 +601,6 @@ int check_ptrs(const void *const *p, int n) {
 }
 
 int check_adjoint(ns3d_ctx *c) {
-    if (c->b.nx > adjoint_max_nx())
-        return fail(NS3D_E_UNSUPPORTED, "the adjoint holds one x line per block: nx <= %d (got %d)",
-                    adjoint_max_nx(), c->b.nx);
     if (!adjoint_supported(c->P.interpolation, c->P.riemann_solver, c->P.char_recon != 0))
         return fail(NS3D_E_UNSUPPORTED,
                     "the adjoint covers component-wise WCNS5 / WENO5-JS interpolation with HLLC or Rusanov flux "

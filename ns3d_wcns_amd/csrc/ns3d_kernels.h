@@ -98,7 +98,6 @@ struct LinComb {
 };
 
 bool adjoint_supported(int interp, int rs, bool chr);
-int adjoint_max_nx();  // the x-line adjoint kernel holds one line per block
 void launch_tl_prim(const AdjArgs &a, hipStream_t s);
 void launch_tl_rhs(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s);
 // Combinations formed from the finished adjoint RHS value r of each cell in

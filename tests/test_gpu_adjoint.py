@@ -5,7 +5,9 @@ device through the C-ABI, and the checkpointed autograd rollout.
   agree to 1e-13 of |(dR/dU) dU| |lam| (the rounding scale of the inner
   product; the tangent runs the forward numerics on dual numbers, the
   adjoint is hand-written reverse mode: independent implementations), on
-  boxes of one and of several marching segments along every axis;
+  boxes of one and of several marching segments along every axis, and x
+  lines held whole or in segments (512 cells: two, 517 cells: three, the
+  last shorter);
 - tangent RHS vs central differences of compute_rhs: 1e-6;
 - gradient of J = mean 0.5 rho |u|^2 at the final time with respect to the
   initial conserved fields vs central differences (eps = 1e-5) on 8 seeded
@@ -73,8 +75,8 @@ def dot_agree(tl, lam, dU, ad, what):
     assert abs(a - b) <= 1e-13 * scale, (a, b, scale)
 
 
-@pytest.mark.parametrize("n", [SMALL, (40, 24, 20), (512, 6, 8)],
-                         ids=["one-segment", "several-segments", "512-cell-x-line"])
+@pytest.mark.parametrize("n", [SMALL, (40, 24, 20), (512, 6, 8), (517, 6, 8)],
+                         ids=["one-segment", "several-segments", "512-cell-x-line", "ragged-x-segments"])
 @pytest.mark.parametrize("interp,rs", ADJ, ids=lambda x: x.name)
 def test_rhs_dot_product(gpu, interp, rs, n):
     import torch
