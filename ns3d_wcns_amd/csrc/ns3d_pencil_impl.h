@@ -59,6 +59,22 @@ __device__ __forceinline__ double visc_at(const double *V, long o) {
     else return V[o];
 }
 
+// Partial RHS of the z and y pencils at compact cell o: FP64 for the plain
+// method; for the mixed method FP32, in the first half of the same buffer.
+// The mixed partials are sums of differences of FP32 face fluxes, whose
+// rounding (about 1e-7 of the fluxes before cancellation) dominates the 6e-8
+// relative rounding of the stored sum.
+template <class R>
+__device__ __forceinline__ void put_partial(double *P, long o, double v) {
+    if constexpr (kPert<R>) __builtin_nontemporal_store((float)v, reinterpret_cast<float *>(P) + o);
+    else __builtin_nontemporal_store(v, P + o);
+}
+template <class R>
+__device__ __forceinline__ double get_partial(const double *P, long o) {
+    if constexpr (kPert<R>) return (double)__builtin_nontemporal_load(reinterpret_cast<const float *>(P) + o);
+    else return __builtin_nontemporal_load(P + o);
+}
+
 // Shu-Osher stage update of one cell from the complete RHS r, the recorded
 // state u0v and the cell's current state ucur, then the conserved -> primitive -> conserved round
 // trip; writes the new state and the primitives (range_kutta.cpp:53-91,
@@ -136,8 +152,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
         if (valid) {
 #pragma unroll
             for (int n = 0; n < 5; ++n) {
-                sp[n] = __builtin_nontemporal_load(&a.Rz[n][orow + t]) +
-                        __builtin_nontemporal_load(&a.Ry[n][orow + t]);
+                sp[n] = get_partial<R>(a.Rz[n], orow + t) + get_partial<R>(a.Ry[n], orow + t);
                 zp[n] = need_u0 ? __builtin_nontemporal_load(&a.U0[n][row + NG + t]) : 0.0;
             }
         }
@@ -216,7 +231,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
                 part = sp[n];
                 u0[n] = need_u0 ? zp[n] : u[n];
             } else {
-                part = a.Rz[n][orow + c] + a.Ry[n][orow + c];
+                part = get_partial<R>(a.Rz[n], orow + c) + get_partial<R>(a.Ry[n], orow + c);
                 u0[n] = need_u0 ? a.U0[n][g] : u[n];
             }
             const double *Fn = Fl + n * N;
@@ -278,7 +293,7 @@ __global__ void __launch_bounds__(512) k_pencil_xp(PencilArgs a, SchemeCoef<R> c
     if (valid) {
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
-            sp[n] = __builtin_nontemporal_load(&a.Rz[n][orow + t]) + __builtin_nontemporal_load(&a.Ry[n][orow + t]);
+            sp[n] = get_partial<R>(a.Rz[n], orow + t) + get_partial<R>(a.Ry[n], orow + t);
             zp[n] = need_u0 ? __builtin_nontemporal_load(&a.U0[n][row + NG + t]) : 0.0;
         }
         // all loads in flight, then the LDS stores: slots t and t + N (< W)
@@ -506,7 +521,7 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
             const long o = obase + (long)c * ost;
             double *const *dst = DIR == 2 ? a.Rz : a.Ry;
 #pragma unroll
-            for (int n = 0; n < 5; ++n) __builtin_nontemporal_store(acc[0][n] * se, &dst[n][o]);
+            for (int n = 0; n < 5; ++n) put_partial<R>(dst[n], o, acc[0][n] * se);
         }
 #pragma unroll
         for (int q = 0; q < 6; ++q)
