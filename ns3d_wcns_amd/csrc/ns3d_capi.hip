@@ -354,13 +354,11 @@ int record0(ns3d_ctx *c) {
     return NS3D_OK;
 }
 
-int cfl(ns3d_ctx *c, double *d_out, bool record) {
+// dt = cfl * min(1e9, dx / max(|u| + a)) of the conserved state into the
+// device scalar d_out (compute_timestep, timestep.cpp:5-23).
+int cfl(ns3d_ctx *c, double *d_out) {
     CflArgs a{};
-    for (int n = 0; n < 5; ++n) {
-        a.U[n] = conserved(c)[n];
-        a.U0[n] = recorded(c)[n];
-    }
-    a.record = record ? 1 : 0;
+    for (int n = 0; n < 5; ++n) a.U[n] = conserved(c)[n];
     a.partial = c->partial;
     a.dt_out = d_out;
     a.b = c->b;
@@ -1073,7 +1071,7 @@ int ns3d_runge_kutta_3_dev(ns3d_ctx *c, const double *d_dt) {
 int ns3d_compute_timestep(ns3d_ctx *c, double *dt_out) {
     NS3D_CTX(c);
     if (!dt_out) return fail(NS3D_E_ARG, "null dt_out");
-    int rc = cfl(c, c->scal, false);
+    int rc = cfl(c, c->scal);
     if (rc) return rc;
     NS3D_HIP(hipMemcpyAsync(dt_out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     NS3D_HIP(hipStreamSynchronize(c->stream));
@@ -1083,7 +1081,7 @@ int ns3d_compute_timestep(ns3d_ctx *c, double *dt_out) {
 int ns3d_compute_timestep_dev(ns3d_ctx *c, double *d_dt) {
     NS3D_CTX(c);
     if (!d_dt) return fail(NS3D_E_ARG, "null device dt");
-    return cfl(c, d_dt, false);
+    return cfl(c, d_dt);
 }
 
 int ns3d_time_advance(ns3d_ctx *c, int max_steps, double dt_fixed, double total_time, double output_timefreq,
@@ -1126,7 +1124,7 @@ int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
         double *d_dt = d_dt_hist ? d_dt_hist + s : c->scal;
         // recordConservedTo0 fused into the first stage; ghost layers
         // refilled after the last step only
-        if ((rc = cfl(c, d_dt, false))) return rc;
+        if ((rc = cfl(c, d_dt))) return rc;
         if ((rc = rk3(c, d_dt, 0.0, true, s == nsteps - 1))) return rc;
     }
     return NS3D_OK;

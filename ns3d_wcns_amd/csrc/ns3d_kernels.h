@@ -53,8 +53,6 @@ constexpr long REDUCE_BLOCKS = 8192;
 
 struct CflArgs {
     const double *U[5];
-    double *U0[5];  // written when record != 0
-    int record;
     double *partial;
     double *dt_out;
     Box b;

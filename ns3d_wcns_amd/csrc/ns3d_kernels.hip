@@ -362,32 +362,34 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// One pass over the rows of the ghosted box: U0 <- U on every cell when
-// a.record is set (Field3D::recordConservedTo0, field_structures.h:491-497),
-// and the block's maximum CFL speed over the interior cells.
+// The block's maximum CFL speed over the interior cells, flat over the box
+// with four cells of each thread in flight (a maximum does not depend on the
+// order of the cells).
 __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
     const Box &b = a.b;
-    const long rows = (long)b.sy * b.sz;
+    const long n = (long)b.nx * b.ny * b.nz, stride = (long)gridDim.x * 256;
     double s = 0.0;
-    for (long r = blockIdx.x; r < rows; r += gridDim.x) {
-        const int j = (int)(r % b.sy), k = (int)(r / b.sy);
-        const bool row_in = j >= NG && j < NG + b.ny && k >= NG && k < NG + b.nz;
-        const long base = r * b.sx;
-        for (int i = threadIdx.x; i < b.sx; i += 256) {
-            const long c = base + i;
-            const double rho = a.U[0][c], rhou = a.U[1][c], E = a.U[4][c];
-            if (a.record) {
-                a.U0[0][c] = rho;
-                a.U0[1][c] = rhou;
-                a.U0[2][c] = a.U[2][c];
-                a.U0[3][c] = a.U[3][c];
-                a.U0[4][c] = E;
-            }
-            if (row_in && i >= NG && i < NG + b.nx) {
-                const double v = cfl_speed(rho, rhou, E, a.gamma);
-                if (v == v) s = fmax(s, v);  // skip NaN
+    for (long id0 = (long)blockIdx.x * 256 + threadIdx.x; id0 < n; id0 += 4 * stride) {
+        double rho[4], rhou[4], E[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long id = id0 + u * stride;
+            rho[u] = rhou[u] = E[u] = 0.0;
+            if (id < n) {
+                const int i = (int)(id % b.nx);
+                const long r = id / b.nx;
+                const long c = b.gidx(i, (int)(r % b.ny), (int)(r / b.ny));
+                rho[u] = a.U[0][c];
+                rhou[u] = a.U[1][c];
+                E[u] = a.U[4][c];
             }
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (id0 + u * stride < n) {
+                const double v = cfl_speed(rho[u], rhou[u], E[u], a.gamma);
+                if (v == v) s = fmax(s, v);  // skip NaN
+            }
     }
     s = wave_max(s);
     __shared__ double part[4];
@@ -509,7 +511,7 @@ void launch_halo(const HaloArgs &h, hipStream_t s) {
     hipLaunchKernelGGL(k_halo, dim3((unsigned)((m + 255) / 256), 3), dim3(256), 0, s, h);
 }
 void launch_cfl(const CflArgs &a, hipStream_t s) {
-    const unsigned g = reduce_grid((long)a.b.sy * a.b.sz);
+    const unsigned g = reduce_grid(((long)a.b.nx * a.b.ny * a.b.nz + 1023) / 1024);
     hipLaunchKernelGGL(k_cfl, dim3(g), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, (long)g);
 }
