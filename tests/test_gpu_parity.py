@@ -202,6 +202,26 @@ def test_rk3_step_several_segments(gpu, interp, char):
     dev.close()
 
 
+@pytest.mark.parametrize("n", [(512, 8, 6), (600, 6, 8)], ids=["x-line-512", "x-line-600"])
+def test_rk3_step_long_x_lines(gpu, n):
+    """x lines of 512 cells (one block of 512 threads per line, paired
+    reconstruction) and of 600 (the general x pencil, 256 threads looping over
+    the line)."""
+    prim, ma = state(n, 0.3, noise=0.02, seed=24)
+    dev, ref = pair(n, params(ma), prim)
+    dt = ref.compute_timestep()
+    ref.record_conserved_to0()
+    ref.runge_kutta_3(dt)
+    dev.recordConservedTo0()
+    dev.runge_kutta_3(dt)
+    for name in CONS:
+        a, b = dev.interior(dev.download(name)), ref.interior(ref.view(name))
+        b0 = ref.interior(ref.view(name + "0"))
+        assert np.max(np.abs(a - b)) / np.max(np.abs(b - b0)) < 1e-11, name
+        assert rel_err(a, b) < TOL, name
+    dev.close()
+
+
 def test_time_advance_trajectory(gpu):
     """10 steps of time_advance: dt sequence and kinetic energy."""
     prim, ma = state(16, 0.3, seed=21)
