@@ -103,11 +103,14 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, const 
     a.Unew[4][g] = rr * (p / ((a.gamma - 1.0) * rr) + kin);
     // the primitives are next read by the following stage's viscous kernel:
     // streamed past the caches (nontemporal), as are the partial sums and U0
-    // below and the viscous fluxes, each read once by one later kernel
+    // below and the viscous fluxes, each read once by one later kernel.  The
+    // pressure is read by no kernel of the next stage; it is stored with the
+    // step's last stage, whose primitives are handed out (an adjoint step,
+    // which stops after stage 2, recomputes its primitives by C2P).
     __builtin_nontemporal_store(u, &a.Q[0][g]);
     __builtin_nontemporal_store(v, &a.Q[1][g]);
     __builtin_nontemporal_store(w, &a.Q[2][g]);
-    __builtin_nontemporal_store(p, &a.Q[3][g]);
+    if (a.stage == 3) __builtin_nontemporal_store(p, &a.Q[3][g]);
     __builtin_nontemporal_store(p / (rr * a.Rgas), &a.Q[4][g]);
 }
 
