@@ -172,12 +172,15 @@ int ns3d_compute_timestep_dev(ns3d_ctx *ctx, double *d_dt);
  * record U0, dt from the CFL (or dt_fixed when > 0), clip dt to
  * output_timefreq / total_time, stop when dt <= 1e-12, runge_kutta_3.
  * dt_out (may be NULL) receives each step's dt; *steps_taken the count.
- * Synchronises once per step to run the reference's host-side dt logic. */
+ * Synchronises once per step to run the reference's host-side dt logic.
+ * The ghost layers of the state and primitives are refilled once, before
+ * the call returns (no kernel of a step reads them). */
 int ns3d_time_advance(ns3d_ctx *ctx, int max_steps, double dt_fixed, double total_time,
                       double output_timefreq, double *dt_out, int *steps_taken);
 /* nsteps of (record U0, CFL dt, RK3) entirely on the device; d_dt_hist (device,
  * nsteps doubles, may be NULL) receives each step's dt.  No clipping: the
- * caller's total time and output interval must lie beyond the run. */
+ * caller's total time and output interval must lie beyond the run.  Ghost
+ * layers as for time_advance. */
 int ns3d_advance_steps(ns3d_ctx *ctx, int nsteps, double *d_dt_hist);
 
 /* ---- observables (postprocess.cpp:103-128) ---- */
