@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(XSEG) k_adj_xseg(AdjArgs a, LinTail tl, int ns
 }
 
 // The face and viscous adjoints of one adjoint RHS, as two z-marching
-// kernels over 64 x 4 column tiles (the tile order of k_visc_march): each
+// kernels over 64 x 4 column tiles: each
 // thread keeps planes k-3..k+3 of its own column in registers for the z
 // stencils; each plane's tile with a 3-cell x/y halo is staged in LDS for the
 // x and y stencils.  Cells beyond the box come from their periodic image, so

@@ -9,9 +9,11 @@
 //             three periodically wrapped cells on each side is staged in LDS,
 //             one thread per face, exactly N faces per line.
 //   y / z pencils: one thread per (x, other, segment) column marching along
-//             the line with a register window of the six stencil cells and six
-//             running cell sums; lanes run along x, so every load is one
-//             coalesced row.  Segments of L cells compute L + 5 faces.
+//             the line with a register window of the stencil cells (five
+//             with the paired reconstruction, else six) and seven running
+//             cell sums; lanes run along x, so every load is one coalesced
+//             row.  Segments of L cells compute L + 5 faces; the segment
+//             count follows the kernel's occupancy (march_seg).
 // None of them reads a ghost layer: stencil cells beyond the box are taken
 // from the periodic image, so the state's ghost layers need to be current
 // only where the state is handed out.
