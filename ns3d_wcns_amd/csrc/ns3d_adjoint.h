@@ -43,8 +43,10 @@ NS3D_HD double weno_vjp(const double *f, double yb, double *fb) {
                          pc[2][0] * f2 + pc[2][1] * f3 + pc[2][2] * f4};
     const double N = e[0] * p[0] + e[1] * p[1] + e[2] * p[2];
     const double D = K * (e[0] + e[1] + e[2]);
-    const double y = N / D;
-    const double Nb = yb / D, Db = -yb * y / D;
+    // y = N / D: Nb = yb / D and Db = -Nb y from one reciprocal
+    const double iD = 1.0 / D;
+    const double y = N * iD;
+    const double Nb = yb * iD, Db = -Nb * y;
     double eb[3], pb[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -104,11 +106,14 @@ NS3D_HD void weno_pair_vjp(const double *f, double ybL, double ybR, double *fb) 
     // mirrored: beta'_0 = beta_2, beta'_2 = beta_0
     const double eR[3] = {d0 * q[1] * q[0], d1 * q[2] * q[0], d2 * q[2] * q[1]};
     const double pR[3] = {c0 * f4 + c1 * f3 + c2 * f2, -f3 + m1 * f2 + m2 * f1, m2 * f2 + m1 * f1 - f0};
+    // y = N / D, so Nb = yb / D and Db = -yb y / D = -Nb y: one reciprocal
+    // per side serves the value and both adjoints
     const double DL = K * (eL[0] + eL[1] + eL[2]), DR = K * (eR[0] + eR[1] + eR[2]);
-    const double yL = (eL[0] * pL[0] + eL[1] * pL[1] + eL[2] * pL[2]) / DL;
-    const double yR = (eR[0] * pR[0] + eR[1] * pR[1] + eR[2] * pR[2]) / DR;
-    const double NbL = ybL / DL, DbL = -ybL * yL / DL;
-    const double NbR = ybR / DR, DbR = -ybR * yR / DR;
+    const double iDL = 1.0 / DL, iDR = 1.0 / DR;
+    const double yL = (eL[0] * pL[0] + eL[1] * pL[1] + eL[2] * pL[2]) * iDL;
+    const double yR = (eR[0] * pR[0] + eR[1] * pR[1] + eR[2] * pR[2]) * iDR;
+    const double NbL = ybL * iDL, DbL = -NbL * yL;
+    const double NbR = ybR * iDR, DbR = -NbR * yR;
     double ebL[3], ebR[3], pbL[3], pbR[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
