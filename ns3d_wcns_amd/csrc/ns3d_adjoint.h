@@ -201,10 +201,11 @@ NS3D_HD void hllc_vjp(const SchemeCoef<double> &c, const double *UL, const doubl
         const double num = Rr.rho * Rr.un * (SR - Rr.un) - L.rho * L.un * (SL - L.un) + L.p - Rr.p;
         const double den = Rr.rho * (SR - Rr.un) - L.rho * (SL - L.un);
         const bool roeavg = !(::fabs(den) > 1e-12);
-        double SM;
+        double SM, iden = 0.0;
         double sqL = 0.0, sqR = 0.0;
         if (!roeavg) {
-            SM = num / den;
+            iden = 1.0 / den;  // SM = num / den; its adjoints below reuse 1 / den
+            SM = num * iden;
         } else {
             sqL = ::sqrt(L.rho);
             sqR = ::sqrt(Rr.rho);
@@ -218,8 +219,9 @@ NS3D_HD void hllc_vjp(const SchemeCoef<double> &c, const double *UL, const doubl
         double *Ubd = left ? ULb : URb;
         const double dS = S - s.un;
         const double iq = 1.0 / (S - SM);
+        const double idS = 1.0 / dS, irdS = s.ir * idS;  // 1 / dS, 1 / (rho dS)
         const double rs = s.rho * dS * iq;
-        const double Y = s.p / (s.rho * dS);
+        const double Y = s.p * irdS;
         const double Z = SM + Y;
         const double Es = s.E * s.ir + (SM - s.un) * Z;
         // F = Fs + S (Us - U)
@@ -243,9 +245,9 @@ NS3D_HD void hllc_vjp(const SchemeCoef<double> &c, const double *UL, const doubl
         sb.un -= Esb * Z;
         const double Zb = Esb * (SM - s.un);
         SMb += Zb;
-        sb.p += Zb / (s.rho * dS);
-        sb.rho -= Zb * Y / s.rho;
-        double dSb = -Zb * Y / dS;
+        sb.p += Zb * irdS;
+        sb.rho -= Zb * Y * s.ir;
+        double dSb = -Zb * Y * idS;
         // rs = rho dS iq, iq = 1 / (S - SM), dS = S - un
         sb.rho += rsb * dS * iq;
         dSb += rsb * s.rho * iq;
@@ -259,7 +261,7 @@ NS3D_HD void hllc_vjp(const SchemeCoef<double> &c, const double *UL, const doubl
         else SRb += Sb;
         // contact speed
         if (!roeavg) {
-            const double numb = SMb / den, denb = -SMb * SM / den;
+            const double numb = SMb * iden, denb = -numb * SM;
             Rb.rho += numb * Rr.un * (SR - Rr.un) + denb * (SR - Rr.un);
             Rb.un += numb * Rr.rho * (SR - 2.0 * Rr.un) - denb * Rr.rho;
             SRb += numb * Rr.rho * Rr.un + denb * Rr.rho;
