@@ -468,17 +468,22 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
 #pragma unroll
             for (int n = 0; n < 5; ++n) w[n][m] = w[n][m + 1];
     }
-    R nxt[5];
-    load(c0, nxt);
     double acc[7][5];
 #pragma unroll
     for (int q = 0; q < 7; ++q)
 #pragma unroll
         for (int n = 0; n < 5; ++n) acc[q][n] = 0.0;
     for (int f = c0 - 3; f <= c1 + 2; ++f) {
+        // The window's new cell is loaded in the iteration that uses it: a
+        // prefetch one iteration ahead kept the next cell's five values live
+        // through the face work, which spilled the FP64 marches at two waves
+        // per SIMD; the other resident waves cover the load's latency.
+        {
+            R nxt[5];
+            load(f + 3, nxt);
 #pragma unroll
-        for (int n = 0; n < 5; ++n) w[n][W - 1] = nxt[n];
-        if (f < c1 + 2) load(f + 4, nxt);  // prefetch the next window cell
+            for (int n = 0; n < 5; ++n) w[n][W - 1] = nxt[n];
+        }
         double vv[4];
         {
             const long o = obase + (long)wrap(f, N) * ost;
