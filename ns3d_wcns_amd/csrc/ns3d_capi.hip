@@ -53,6 +53,10 @@ struct ns3d_ctx {
     double *scal;  // [0] dt, [1] reduction result
     bool timing;
     bool visc_fp32;  // visc[] holds the mixed method's FP32 fluxes (last writer: a mixed fused stage)
+    // the primitives of the state are the compact FP32 u, v, w, T - 1 of
+    // prims32(): only between the stages of one rk3 / step_stages call of
+    // the mixed method (stage 3 stores the FP64 primitives)
+    bool prims_fp32;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;  // (class, (start, stop))
 };
 
@@ -170,6 +174,13 @@ double **conserved(ns3d_ctx *c) { return &c->cell[NS3D_RHO]; }
 double **recorded(ns3d_ctx *c) { return &c->cell[NS3D_RHO0]; }
 double **prims(ns3d_ctx *c) { return &c->cell[NS3D_U]; }
 double **rhs(ns3d_ctx *c) { return &c->cell[NS3D_RHS_RHO]; }
+// Compact FP32 u, v, w, T - 1 handed between the stages of the mixed method:
+// the second half of the z partial buffers, whose first half holds that
+// method's FP32 partial sums (nint floats; a buffer holds field_stride(nint)
+// doubles, at least twice the 64-rounded nint floats)
+float *prims32(ns3d_ctx *c, int q) {
+    return reinterpret_cast<float *>(c->rz[q]) + ((c->nint + 63) / 64) * 64;
+}
 
 int run_faces(ns3d_ctx *c) {
     FaceArgs<double> a64{};
@@ -283,12 +294,17 @@ struct StageIO {
 // pencils; stages 1..3 update the state in place, or from io->Uin into
 // io->Uout.  The kernels read no ghost layer (periodic images instead); with
 // ghosts the layers of the new state (the context's) are refilled for the
-// callers that read them.
+// callers that read them.  q32 (mixed method, stage 1 or 2, set only by
+// callers whose next step is the following stage): the x pencil hands the
+// primitives to the next stage's viscous kernel in FP32.
 int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false, bool ghosts = true,
-                const StageIO *io = nullptr) {
+                const StageIO *io = nullptr, bool q32 = false) {
+    const bool mixed = c->precision == NS3D_PRECISION_MIXED;
     {
+        ViscArgs va = visc_args(c, false, true);
+        for (int q = 0; q < 4; ++q) va.Q32[q] = mixed && c->prims_fp32 ? prims32(c, q) : nullptr;
         Timed t(c, NS3D_K_VISCOUS);
-        launch_visc_march(visc_args(c, false, true), c->precision, c->stream);
+        launch_visc_march(va, c->precision, c->stream);
     }
     NS3D_LAUNCHED("viscous");
     c->visc_fp32 = c->precision == NS3D_PRECISION_MIXED;
@@ -314,6 +330,7 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     a.dt = d_dt;
     a.dtv = dt;
     a.stage = stage;
+    for (int q = 0; q < 4; ++q) a.Q32[q] = q32 && mixed && (stage == 1 || stage == 2) ? prims32(c, q) : nullptr;
     const SchemeCoef<double> c64 = coef64(c->P);
     const SchemeCoef<float> c32 = coef32(c->P);
     const int interp = c->P.interpolation;
@@ -330,6 +347,7 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
         }
     }
     NS3D_LAUNCHED("pencils");
+    if (stage >= 1) c->prims_fp32 = a.Q32[0] != nullptr;
     if (stage == 0 || !ghosts) return NS3D_OK;
     if (io && io->Uout[0] != conserved(c)[0]) return fail(NS3D_E_ARG, "internal: ghost refill of a foreign state");
     return halo(c, true);
@@ -339,11 +357,15 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
 // The intermediate stage states are not handed out, so only the last stage
 // refills the ghost layers, and only with ghosts set: the device time loops
 // run their steps without (no kernel of a step reads a ghost layer) and
-// refill once before they return.
+// refill once before they return.  The mixed method hands stages 1 and 2's
+// primitives to the next viscous kernel in FP32.
 int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool ghosts = true) {
     int rc;
     for (int stage = 1; stage <= 3; ++stage)
-        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, ghosts && stage == 3))) return rc;
+        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, ghosts && stage == 3, nullptr, stage < 3))) {
+            c->prims_fp32 = false;
+            return rc;
+        }
     return NS3D_OK;
 }
 
@@ -735,9 +757,11 @@ int ns3d_step_stages(ns3d_ctx *c, double dt, double *const *U1, double *const *U
     // the stages written straight into the caller's arrays (interior cells),
     // U0 recorded by the stage-1 x pencil
     const StageIO st1{conserved(c), U1, recorded(c)}, st2{U1, U2, recorded(c)}, st3{U2, conserved(c), recorded(c)};
-    if ((rc = fused_stage(c, 1, nullptr, dt, true, false, &st1)) ||
-        (rc = fused_stage(c, 2, nullptr, dt, false, false, &st2)))
+    if ((rc = fused_stage(c, 1, nullptr, dt, true, false, &st1, true)) ||
+        (rc = fused_stage(c, 2, nullptr, dt, false, false, &st2, true))) {
+        c->prims_fp32 = false;
         return rc;
+    }
     return fused_stage(c, 3, nullptr, dt, false, true, &st3);
 }
 
@@ -807,6 +831,7 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     c->fft_plan = nullptr;
     c->timing = false;
     c->visc_fp32 = false;
+    c->prims_fp32 = false;
     if (hipMemset(c->slab, 0, total * sizeof(double)) != hipSuccess) {
         (void)hipFree(c->slab);
         delete c;

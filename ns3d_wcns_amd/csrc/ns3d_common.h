@@ -110,6 +110,7 @@ struct PencilArgs {
     int record;               // stage 1: also store U0 = U (recordConservedTo0) for the interior
     int seg;                  // y / z segment length
     int line_threads;         // x pencil: threads per line
+    float *Q32[4];            // mixed, stages 1-2: compact FP32 u, v, w, T - 1 written instead of Q (else null)
 };
 
 template <int RS>

@@ -60,7 +60,11 @@ constexpr int VT_WAVES = 4;  // minimum waves per SIMD asked of the register all
 // stencils; the temperature enters as T - T_ref (subtracted in FP64, T_ref =
 // 1, the nondimensional reference temperature) and the nine fluxes are
 // stored as FP32 in the first half of each FP64 buffer.
-template <class R>
+//
+// Q32 (mixed only): the primitives are the compact FP32 u, v, w, T - T_ref
+// that the previous stage's x pencil stored (a.Q32; the same values this
+// kernel otherwise rounds from the ghosted FP64 ones).
+template <class R, bool Q32 = false>
 __global__ void __launch_bounds__(64 * VT_Y, VT_WAVES) k_visc_march(ViscArgs a) {
     constexpr int TX = 64, TY = VT_Y, HX = TX + 6, HY = TY + 6, NT = TX * TY;
     constexpr int NHALO = 6 * TX + 6 * TY, NH = (NHALO + NT - 1) / NT;
@@ -75,13 +79,18 @@ __global__ void __launch_bounds__(64 * VT_Y, VT_WAVES) k_visc_march(ViscArgs a) 
     // staged but never read
     auto per = [](int x, int n) { return ((x % n) + n) % n; };
     const int k0 = blockIdx.z * VZ_SEG, k1 = min(k0 + VZ_SEG, b.nz);
-    const long sxy = (long)b.sx * b.sy;
+    // plane stride and (i, j, 0) index of the primitives' layout
+    const long sxy = Q32 ? (long)b.nx * b.ny : (long)b.sx * b.sy;
+    auto at = [&](int x, int y) { return Q32 ? b.oidx(x, y, 0) : b.gidx(x, y, 0); };
     const double tref = sizeof(R) == 4 ? 1.0 : 0.0;
-    auto ld = [&](int q, long g) -> R { return q == 3 ? (R)(a.Q[3][g] - tref) : (R)a.Q[q][g]; };
+    auto ld = [&](int q, long g) -> R {
+        if constexpr (Q32) return a.Q32[q][g];
+        else return q == 3 ? (R)(a.Q[3][g] - tref) : (R)a.Q[q][g];
+    };
     const R i60[3] = {(R)a.i60[0], (R)a.i60[1], (R)a.i60[2]};
     const R mu_ref = (R)a.mu, S_ref = (R)a.S_ref, cp_pr = (R)a.cp_pr;
-    // ghosted index of the periodic image of (i, j, 0)
-    const long col = b.gidx(per(i, b.nx), per(j, b.ny), 0);
+    // index of the periodic image of (i, j, 0)
+    const long col = at(per(i, b.nx), per(j, b.ny));
     R wq[4][7];
 #pragma unroll
     for (int m = 0; m < 6; ++m)
@@ -107,7 +116,7 @@ __global__ void __launch_bounds__(64 * VT_Y, VT_WAVES) k_visc_march(ViscArgs a) 
         hi[h] = x;
         hj[h] = y;
         hv[h] = e < NHALO;
-        hc[h] = b.gidx(per(i0 + x, b.nx), per(j0 + y, b.ny), 0);
+        hc[h] = at(per(i0 + x, b.nx), per(j0 + y, b.ny));
     }
     for (int k = k0; k < k1; ++k) {
 #pragma unroll
@@ -488,7 +497,8 @@ void launch_visc(const ViscArgs &a, hipStream_t s) {
 }
 void launch_visc_march(const ViscArgs &a, int precision, hipStream_t s) {
     const dim3 g((a.b.nx + 63) / 64, (a.b.ny + VT_Y - 1) / VT_Y, (a.b.nz + VZ_SEG - 1) / VZ_SEG);
-    if (precision == 1) hipLaunchKernelGGL(k_visc_march<float>, g, dim3(64, VT_Y), 0, s, a);
+    if (precision == 1 && a.Q32[0]) hipLaunchKernelGGL((k_visc_march<float, true>), g, dim3(64, VT_Y), 0, s, a);
+    else if (precision == 1) hipLaunchKernelGGL(k_visc_march<float>, g, dim3(64, VT_Y), 0, s, a);
     else hipLaunchKernelGGL(k_visc_march<double>, g, dim3(64, VT_Y), 0, s, a);
 }
 void launch_visc_from_grad(const ViscArgs &a, hipStream_t s) {

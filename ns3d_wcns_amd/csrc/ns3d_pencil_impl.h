@@ -81,8 +81,9 @@ __device__ __forceinline__ double get_partial(const double *P, long o) {
 // state u0v and the cell's current state ucur, then the conserved -> primitive -> conserved round
 // trip; writes the new state and the primitives (range_kutta.cpp:53-91,
 // field_structures.h:444-488).
-__device__ __forceinline__ void stage_update(const PencilArgs &a, long g, const double *u0v, const double *ucur,
-                                             const double *r) {
+template <class R>
+__device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o, const double *u0v,
+                                             const double *ucur, const double *r) {
     const double dt = a.dt ? *a.dt : a.dtv;
     double un[5];
 #pragma unroll
@@ -109,11 +110,25 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, const 
     // pressure is read by no kernel of the next stage; it is stored with the
     // step's last stage, whose primitives are handed out (an adjoint step,
     // which stops after stage 2, recomputes its primitives by C2P).
+    const double T = p / (rr * a.Rgas);
+    if constexpr (kPert<R>) {
+        // mixed stages 1 and 2: the next viscous kernel reads u, v, w and
+        // T - T_ref (T_ref = 1) in FP32, which it would round from these
+        // same FP64 values; stored compact as FP32 at compact cell o (stage
+        // 3 stores the FP64 primitives that are handed out)
+        if (a.Q32[0]) {
+            __builtin_nontemporal_store((float)u, a.Q32[0] + o);
+            __builtin_nontemporal_store((float)v, a.Q32[1] + o);
+            __builtin_nontemporal_store((float)w, a.Q32[2] + o);
+            __builtin_nontemporal_store((float)(T - 1.0), a.Q32[3] + o);
+            return;
+        }
+    }
     __builtin_nontemporal_store(u, &a.Q[0][g]);
     __builtin_nontemporal_store(v, &a.Q[1][g]);
     __builtin_nontemporal_store(w, &a.Q[2][g]);
     if (a.stage == 3) __builtin_nontemporal_store(p, &a.Q[3][g]);
-    __builtin_nontemporal_store(p / (rr * a.Rgas), &a.Q[4][g]);
+    __builtin_nontemporal_store(T, &a.Q[4][g]);
 }
 
 // FP64 component-wise WCNS5 / WENO5-JS: the marches form both
@@ -257,7 +272,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
 #pragma unroll
                 for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
             }
-            stage_update(a, g, u0, u, r);
+            stage_update<R>(a, g, orow + c, u0, u, r);
         }
         if constexpr (ONE) break;
     }
@@ -383,7 +398,7 @@ __global__ void __launch_bounds__(512) k_pencil_xp(PencilArgs a, SchemeCoef<R> c
 #pragma unroll
             for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
         }
-        stage_update(a, g, u0, u, r);
+        stage_update<R>(a, g, orow + c, u0, u, r);
     }
 }
 
