@@ -44,7 +44,7 @@ NS3D_HD double weno_vjp(const double *f, double yb, double *fb) {
     const double N = e[0] * p[0] + e[1] * p[1] + e[2] * p[2];
     const double D = K * (e[0] + e[1] + e[2]);
     // y = N / D: Nb = yb / D and Db = -Nb y from one reciprocal
-    const double iD = 1.0 / D;
+    const double iD = frcp(D);
     const double y = N * iD;
     const double Nb = yb * iD, Db = -Nb * y;
     double eb[3], pb[3];
@@ -109,7 +109,7 @@ NS3D_HD void weno_pair_vjp(const double *f, double ybL, double ybR, double *fb) 
     // y = N / D, so Nb = yb / D and Db = -yb y / D = -Nb y: one reciprocal
     // per side serves the value and both adjoints
     const double DL = K * (eL[0] + eL[1] + eL[2]), DR = K * (eR[0] + eR[1] + eR[2]);
-    const double iDL = 1.0 / DL, iDR = 1.0 / DR;
+    const double iDL = frcp(DL), iDR = frcp(DR);
     const double yL = (eL[0] * pL[0] + eL[1] * pL[1] + eL[2] * pL[2]) * iDL;
     const double yR = (eR[0] * pR[0] + eR[1] * pR[1] + eR[2] * pR[2]) * iDR;
     const double NbL = ybL * iDL, DbL = -NbL * yL;
@@ -204,7 +204,7 @@ NS3D_HD void hllc_vjp(const SchemeCoef<double> &c, const double *UL, const doubl
         double SM, iden = 0.0;
         double sqL = 0.0, sqR = 0.0;
         if (!roeavg) {
-            iden = 1.0 / den;  // SM = num / den; its adjoints below reuse 1 / den
+            iden = frcp(den);  // SM = num / den; its adjoints below reuse 1 / den
             SM = num * iden;
         } else {
             sqL = ::sqrt(L.rho);
@@ -218,8 +218,8 @@ NS3D_HD void hllc_vjp(const SchemeCoef<double> &c, const double *UL, const doubl
         const double *U = left ? UL : UR;
         double *Ubd = left ? ULb : URb;
         const double dS = S - s.un;
-        const double iq = 1.0 / (S - SM);
-        const double idS = 1.0 / dS, irdS = s.ir * idS;  // 1 / dS, 1 / (rho dS)
+        const double iq = frcp(S - SM);
+        const double idS = frcp(dS), irdS = s.ir * idS;  // 1 / dS, 1 / (rho dS)
         const double rs = s.rho * dS * iq;
         const double Y = s.p * irdS;
         const double Z = SM + Y;
@@ -405,7 +405,7 @@ NS3D_HD void visc_local_vjp(double mu_ref, double S_ref, double cp_pr, double u,
 // (conservedToPrimitive, field_structures.h:444-466, with T = p / (rho Rgas)):
 // Ub += (dq/dU)^T qb.
 NS3D_HD void c2p_vjp(double gamma, double Rgas, const double *U, const double *qb, double *Ub) {
-    const double ir = 1.0 / U[0];
+    const double ir = frcp(U[0]);
     const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
     const double cT = (gamma - 1.0) / Rgas * ir;  // dT/dE
     // T = (gamma - 1) / Rgas (E / rho - |u|^2 / 2)
