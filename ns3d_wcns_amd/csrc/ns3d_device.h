@@ -395,8 +395,8 @@ NS3D_HD void hllc_star(const SchemeCoef<R> &c, const Prim<R> &L, const Prim<R> &
         F[3] = Fs[3] + S * (rs * s.u2 - U[3]);
         F[4] = Fs[4] + S * dE;
     } else {
-        const R rs = s.rho * dS / (S - SM);
-        const R Es = s.E * s.ir + (SM - s.un) * (SM + s.p / (s.rho * dS));
+        const R rs = fdiv(s.rho * dS, S - SM);
+        const R Es = s.E * s.ir + (SM - s.un) * (SM + fdiv(s.p, s.rho * dS));
         const R Us[5] = {rs, rs * SM, rs * s.u1, rs * s.u2, rs * Es};
 #pragma unroll
         for (int n = 0; n < 5; ++n) F[n] = Fs[n] + S * (Us[n] - U[n]);
