@@ -149,3 +149,59 @@ def test_mixed_time_advance_matches_advance_steps(gpu):
         assert np.array_equal(a.interior(a.download(name)), b.interior(b.download(name))), name
     a.close()
     b.close()
+
+
+def _edges_and_corners(shape, g=3):
+    """Ghost cells outside the interior along two or three axes: no halo
+    refill writes them."""
+    out = [np.zeros(shape, dtype=np.int8) for _ in range(3)]
+    for ax in range(3):
+        idx = np.arange(shape[ax])
+        o = ((idx < g) | (idx >= shape[ax] - g)).astype(np.int8)
+        out[ax] = o.reshape([-1 if a == ax else 1 for a in range(3)])
+    return (out[0] + out[1] + out[2]) >= 2
+
+
+def test_mixed_steps_hand_out_fp64_primitives(gpu):
+    """The mixed method hands stages 1 and 2's u, v, w, T to the next viscous
+    kernel as compact FP32 in work memory of its own.  After runge_kutta_3,
+    advance_steps and step_stages the primitive arrays that are handed out
+    hold the FP64 primitives of the new conserved state (1e-13; the stage
+    update's conversion round trip differs from a fresh one by a few ulp),
+    their face ghosts hold the periodic images, and their edge and corner
+    ghosts, which no refill writes, are untouched."""
+    import torch
+
+    from test_gpu_parity import faces_periodic
+
+    n = (20, 18, 22)
+    prim, ma = state(n, 0.3, noise=0.02, seed=11)
+    P = params(ma)
+    d = DeviceField(box(n), P, precision=PRECISION_MIXED)
+    d.initialize_primitive(*prim)
+    names = ("u", "v", "w", "p", "T")
+    before = {q: d.download(q) for q in names}
+    mask = _edges_and_corners(before["u"].shape)
+    assert mask.any()
+
+    def check(tag):
+        rho, ru, rv, rw, E = (d.interior(d.download(q)) for q in ("rho", "rhou", "rhov", "rhow", "E"))
+        u, v, w = ru / rho, rv / rho, rw / rho
+        p = (P.gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v + w * w))
+        for q, ref in zip(names, (u, v, w, p, p / (rho * P.Rgas))):
+            a = d.download(q)
+            assert rel_err(d.interior(a), ref) < 1e-13, (tag, q)
+            assert faces_periodic(a), (tag, q)
+            assert np.array_equal(a[mask], before[q][mask]), (tag, q)
+
+    d.step(1e-3)
+    check("runge_kutta_3")
+    d.advance_steps(2)
+    check("advance_steps")
+    cells = int(np.prod(d.shape))
+    U1 = [torch.zeros(cells, dtype=torch.float64, device="cuda") for _ in range(5)]
+    U2 = [torch.zeros(cells, dtype=torch.float64, device="cuda") for _ in range(5)]
+    d.step_stages(1e-3, [t.data_ptr() for t in U1], [t.data_ptr() for t in U2])
+    torch.cuda.synchronize()
+    check("step_stages")
+    d.close()
