@@ -77,10 +77,29 @@ NS3D_HD R fdiv(R a, R b) {
 #endif
     return a / b;
 }
+// Square root: FP32, the hardware v_sqrt_f32.  FP64: Goldschmidt's
+// iteration from the hardware estimate y of 1/sqrt(a) (v_rsq_f64): g = a y,
+// h = y / 2, r = 1/2 - g h, g += g r, h += h r, then two Newton corrections
+// g += (a - g g) h; ten instructions instead of about seventeen with the
+// operand scaling and special-case fix-ups of the correctly rounded
+// expansion.  Operands are positive normal numbers (gamma p / rho, the
+// Sutherland temperature).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double gs_sqrt64(double a) {
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a * y, h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(__builtin_fma(-g, g, a), h, g);
+    return __builtin_fma(__builtin_fma(-g, g, a), h, g);
+}
+#endif
 template <class R>
 NS3D_HD R fsqrt(R a) {
 #if defined(__HIP_DEVICE_COMPILE__)
     if constexpr (kPert<R>) return __builtin_amdgcn_sqrtf(a);
+    if constexpr (std::is_same<R, double>::value) return gs_sqrt64(a);
 #endif
     return sqrt(a);
 }
