@@ -163,8 +163,9 @@ NS3D_HD void phys_flux_vjp(const Prim<double> &s, const double *Fb, PrimBar &sb)
 // Reverse mode of unpack: Ub += (ds/dU)^T sb.
 NS3D_HD void unpack_vjp(double g, const double *U, const Prim<double> &s, PrimBar sb, double *Ub) {
     // a = sqrt(g p ir)
-    sb.p += sb.a * g * s.ir / (2.0 * s.a);
-    sb.ir += sb.a * g * s.p / (2.0 * s.a);
+    const double i2a = frcp(2.0 * s.a);
+    sb.p += sb.a * g * s.ir * i2a;
+    sb.ir += sb.a * g * s.p * i2a;
     // p = (E - rho k / 2)(g - 1), k = un^2 + u1^2 + u2^2
     const double k = s.un * s.un + s.u1 * s.u1 + s.u2 * s.u2;
     sb.E += sb.p * (g - 1.0);
@@ -398,7 +399,7 @@ NS3D_HD void visc_local_vjp(double mu_ref, double S_ref, double cp_pr, double u,
     gb[1][2] += mu * tyzb;
     gb[2][1] += mu * tyzb;
     // mu = mu_ref T^1.5 (1 + S) / (T + S): dmu/dT = mu (1.5 / T - 1 / (T + S))
-    qb[3] += mub * mu * (1.5 / T - 1.0 / (T + S_ref));
+    qb[3] += mub * mu * (1.5 * frcp(T) - frcp(T + S_ref));
 }
 
 // Reverse mode of the cell primitives (u, v, w, T) of a conserved state U
@@ -407,7 +408,7 @@ NS3D_HD void visc_local_vjp(double mu_ref, double S_ref, double cp_pr, double u,
 NS3D_HD void c2p_vjp(double gamma, double Rgas, const double *U, const double *qb, double *Ub) {
     const double ir = frcp(U[0]);
     const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
-    const double cT = (gamma - 1.0) / Rgas * ir;  // dT/dE
+    const double cT = fdiv(gamma - 1.0, Rgas) * ir;  // dT/dE
     // T = (gamma - 1) / Rgas (E / rho - |u|^2 / 2)
     Ub[0] += -(qb[0] * u + qb[1] * v + qb[2] * w) * ir + qb[3] * cT * (u * u + v * v + w * w - U[4] * ir);
     Ub[1] += qb[0] * ir - qb[3] * cT * u;
