@@ -37,7 +37,8 @@ struct SchemeCoef {
 };
 
 // The FP32 arithmetic (mixed method) works in the perturbation form below;
-// the FP64 arithmetic is plain, with the reference's operation sequence.
+// the FP64 arithmetic is plain, with the reference's operation sequence
+// (its divisions and square roots by the iterations below).
 template <class R>
 constexpr bool kPert = sizeof(R) == 4;
 
