@@ -1,0 +1,417 @@
+"""The workloads the device path is measured on, run on the device through the
+C-ABI and checked against the CPU checker (or against size-independent
+properties where the checker would take too long).
+
+Decaying compressible isotropic turbulence on the periodic [0, 2pi]^3 box,
+gamma = 1.4, Pr = 0.72, Sutherland mu, seeded solenoidal initial velocity with
+E(k) ~ k^4 exp(-2 k^2 / k0^2), k0 = 4, rho = T = 1.
+
+- Throughput sweep (Ma_t = 0.6, Re = 100, CFL 0.5; {WCNS5, WENO5-JS,
+  MDCD-hybrid} x {HLLC, Rusanov}):
+    * every combination, 40^3 x 10 CFL steps (time_advance, maincycle.cpp:18-56)
+      against the checker: dt sequence 1e-12, conserved fields 1e-10
+      (norm-wise relative);
+    * WCNS5 + HLLC at the full 256^3: one RK3 step against the checker
+      (dt bit-exact, fields 1e-13, the step's increment 1e-10);
+    * every combination at 256^3: one step conserves mass, momentum and
+      energy (|sum dU| <= 1e-12 sum |dU| per field: the half-node flux
+      differences telescope on the periodic box).
+- Mixed vs plain (Ma_t = 0.3, Re = 1600, WCNS5 + HLLC):
+    * one RHS of the mixed method against the checker on 48^3: 5e-5 per
+      field (FP64: 1e-12);
+    * 64^3 x 50 CFL steps of the mixed method against the checker's FP64
+      run: fields, kinetic energy, TKE, spectral dissipation and E(k) of
+      every shell holding >= 1e-6 of the peak within 1e-5 relative.
+- Adjoint (128^3, Ma_t = 0.3, Re = 100, WCNS5 + HLLC, 50 RK3 steps,
+  checkpoint every 10 steps; J = mean 0.5 rho |u|^2 at the final time;
+  gradient with respect to the initial rho u, rho v, rho w, E): see
+  test_adjoint_128_cubed_50_steps for the directions and bars.
+- Adjoint at scale, reduced to 64^3 x 20 steps (Ma_t = 0.5, Re = 1600,
+  mixed forward, FP64 adjoint, binomial checkpoints, J = time-integrated
+  viscous dissipation, gradient with respect to the initial velocity at
+  fixed rho and p): see test_adjoint_at_scale_reduced.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cases import box, params, rel_err
+from ns3d_wcns_amd import PRECISION_FP64, PRECISION_MIXED, DeviceField, Interpolation, RiemannSolver
+from ns3d_wcns_amd.isotropic import isotropic_state
+from ns3d_wcns_amd.observables import compare_runs
+from oracle.ns3d_oracle import OracleField, set_threads
+
+pytestmark = pytest.mark.gpu
+
+CONS = ("rho", "rhou", "rhov", "rhow", "E")
+SWEEP = [(Interpolation.WENO5, RiemannSolver.HLLC), (Interpolation.WENO5, RiemannSolver.Rusanov),
+         (Interpolation.WENO5_JS, RiemannSolver.HLLC), (Interpolation.WENO5_JS, RiemannSolver.Rusanov),
+         (Interpolation.MDCD_HYBRID, RiemannSolver.HLLC), (Interpolation.MDCD_HYBRID, RiemannSolver.Rusanov)]
+NG = 3
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def shocklet_256():
+    """The 256^3 Ma_t = 0.6 initial state (seed 12345), generated once."""
+    return isotropic_state(256, mach_t=0.6, seed=12345)
+
+
+# ------------------------------------------------------------------ throughput sweep
+@pytest.mark.parametrize("interp,rs", SWEEP, ids=lambda x: x.name)
+def test_sweep_schemes_40_cubed_10_steps(gpu, interp, rs):
+    n, steps = 40, 10
+    say(f"sweep {interp.name}+{rs.name}: {n}^3 x {steps} steps, Ma_t 0.6")
+    prim, ma = isotropic_state(n, mach_t=0.6, seed=12345)
+    P = params(ma, interp, rs)
+    G = box(n)
+    dev = DeviceField(G, P)
+    dev.initialize_primitive(*prim)
+    dts = dev.time_advance(steps)
+    set_threads(16)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    rdts = ref.time_advance(steps)
+    assert len(dts) == len(rdts) == steps
+    e_dt = float(np.max(np.abs(dts - rdts) / rdts))
+    errs = {q: rel_err(dev.interior(dev.download(q)), ref.interior(ref.view(q))) for q in CONS}
+    say(f"  dt {e_dt:.1e} fields {', '.join(f'{q} {e:.1e}' for q, e in errs.items())}")
+    assert e_dt < 1e-12
+    for q, e in errs.items():
+        assert e < 1e-10, (q, e)
+    dev.close()
+    ref.close()
+
+
+def test_sweep_wcns5_hllc_256_cubed_one_step(gpu, shocklet_256):
+    """The throughput configuration itself: one RK3 step at 256^3."""
+    prim, ma = shocklet_256
+    P = params(ma, Interpolation.WENO5, RiemannSolver.HLLC)
+    G = box(256)
+    say("256^3 WCNS5+HLLC: one RK3 step on the device")
+    dev = DeviceField(G, P)
+    dev.initialize_primitive(*prim)
+    dt = dev.compute_timestep()
+    dev.recordConservedTo0()
+    dev.runge_kutta_3(dt)
+    out = {q: dev.interior(dev.download(q)).copy() for q in CONS}
+    dev.close()
+    say("256^3 WCNS5+HLLC: one RK3 step on the checker (16 threads)")
+    set_threads(16)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    rdt = ref.compute_timestep()
+    assert dt == rdt, (dt, rdt)
+    ref.record_conserved_to0()
+    ref.runge_kutta_3(rdt)
+    for q in CONS:
+        b = ref.interior(ref.view(q))
+        b0 = ref.interior(ref.view(q + "0"))
+        e = rel_err(out[q], b)
+        e_inc = rel_err(out[q] - b0, b - b0)
+        say(f"  {q}: field {e:.1e}, increment {e_inc:.1e}")
+        assert e < 1e-13, (q, e)
+        assert e_inc < 1e-10, (q, e_inc)
+    ref.close()
+
+
+@pytest.mark.parametrize("interp,rs", SWEEP, ids=lambda x: x.name)
+def test_sweep_256_cubed_conservation(gpu, shocklet_256, interp, rs):
+    """The fused RHS sums to zero over the periodic box (the half-node flux
+    and viscous-flux differences telescope): |sum R| <= 1e-13 sum |R| per
+    field.  One step then conserves each field up to the rounding of the
+    stage updates, which is biased where the state straddles a power of two
+    (rho ~ 1): |sum dU| <= 4 N eps max|U| (eps = 2^-52, N cells).  The
+    CPU checker shows the same drift, -1.6e-10 in rho over one step at 128^3."""
+    import torch
+
+    from ns3d_wcns_amd.adjoint import interior, state_tensor
+
+    prim, ma = shocklet_256
+    P = params(ma, interp, rs)
+    say(f"256^3 {interp.name}+{rs.name}: RHS sums and conservation over one step")
+    dev = DeviceField(box(256), P)
+    dev.set_stream(torch.cuda.current_stream().cuda_stream)
+    dev.initialize_primitive(*prim)
+    dev.compute_rhs()
+    for name in CONS:
+        r = dev.interior(dev.download("rhs_" + name))
+        s, a = float(np.sum(r)), float(np.sum(np.abs(r)))
+        say(f"  rhs_{name}: |sum R| / sum |R| = {abs(s) / a:.1e}")
+        assert abs(s) <= 1e-13 * a, (name, s, a)
+    U0 = state_tensor(dev)
+    dt = dev.compute_timestep()
+    dev.step(dt)
+    U1 = state_tensor(dev)
+    dU = interior(U1) - interior(U0)
+    assert bool(torch.isfinite(dU).all())
+    cells = 256 ** 3
+    for q, name in enumerate(CONS):
+        s = float(dU[q].sum())
+        bound = 4 * cells * 2.0 ** -52 * float(interior(U1)[q].abs().max())
+        say(f"  {name}: |sum dU| = {abs(s):.2e}, rounding bound {bound:.2e}")
+        assert abs(s) <= bound, (name, s, bound)
+    dev.close()
+
+
+# ------------------------------------------------------------------ mixed vs plain
+def test_mixed_rhs_re1600_48_cubed(gpu):
+    n = 48
+    say(f"Re 1600 RHS on {n}^3, mixed and FP64 against the checker")
+    prim, ma = isotropic_state(n, mach_t=0.3, seed=12345)
+    P = params(ma, re=1600.0)
+    G = box(n)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    ref.compute_rhs()
+    for prec, bar in ((PRECISION_MIXED, 5e-5), (PRECISION_FP64, 1e-12)):
+        dev = DeviceField(G, P, precision=prec)
+        dev.initialize_primitive(*prim)
+        dev.compute_rhs()
+        for q in CONS:
+            e = rel_err(dev.interior(dev.download("rhs_" + q)), ref.interior(ref.view("rhs_" + q)))
+            say(f"  precision {prec} rhs_{q}: {e:.2e}")
+            assert e < bar, (prec, q, e)
+        dev.close()
+    ref.close()
+
+
+def test_mixed_vs_checker_re1600_64_cubed_50_steps(gpu):
+    n, steps = 64, 50
+    say(f"Re 1600 mixed method, {n}^3 x {steps} steps, against the checker's FP64 run")
+    prim, ma = isotropic_state(n, mach_t=0.3, seed=12345)
+    P = params(ma, re=1600.0)
+    G = box(n)
+    out = {}
+    for prec in (PRECISION_MIXED, PRECISION_FP64):
+        dev = DeviceField(G, P, precision=prec)
+        dev.initialize_primitive(*prim)
+        dev.advance_steps(steps)
+        out[prec] = dev.snapshot()
+        dev.close()
+    set_threads(16)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    ref.time_advance(steps)
+    rsnap = {q: ref.interior(ref.view(q)).copy() for q in CONS + ("u", "v", "w", "p", "T")}
+    ref.close()
+    for q in CONS:  # the FP64 device run is the checker's run to rounding
+        assert rel_err(out[PRECISION_FP64][q], rsnap[q]) < 1e-10, q
+    rep = compare_runs(out[PRECISION_MIXED], rsnap, P, G.dx * G.dy * G.dz)
+    say(f"  fields {rep['fields']}")
+    say(f"  kinetic energy {rep['kinetic_energy']:.2e} TKE {rep['tke']:.2e} dissipation {rep['dissipation']:.2e} "
+        f"E(k) max {rep['spectrum_max']:.2e}")
+    say(f"  E(k) per shell {[float(f'{e:.1e}') for e in rep['spectrum_per_shell']]}")
+    for q, e in rep["fields"].items():
+        assert e < 1e-5, (q, e)
+    for k in ("kinetic_energy", "tke", "dissipation", "spectrum_max"):
+        assert rep[k] < 1e-5, (k, rep[k])
+
+
+# ------------------------------------------------------------------ adjoint
+def _ptrs(t):
+    return [t[q].data_ptr() for q in range(5)]
+
+
+def _interior_randn(shape, gen, fields):
+    import torch
+
+    d = torch.zeros(shape, dtype=torch.float64)
+    n = tuple(s - 2 * NG for s in shape[1:])
+    for q in fields:
+        d[q, NG:-NG, NG:-NG, NG:-NG] = torch.randn(n, generator=gen, dtype=torch.float64)
+    return d.cuda()
+
+
+def test_adjoint_128_cubed_50_steps(gpu):
+    """J = mean 0.5 rho |u|^2 after 50 CFL steps at 128^3 (the dt sequence of
+    a first forward run, then held fixed), gradient with respect to the
+    initial (rho u, rho v, rho w, E) by the checkpointed discrete adjoint
+    (uniform:10), checked by central differences along 8 seeded random
+    directions in two normalisations:
+
+    - unit Euclidean norm over the 4 x 128^3 entries, eps = 1e-6 and 1e-4.
+      Such a direction perturbs each entry by ~3.5e-10 eps and is nearly
+      orthogonal to the gradient (its derivative is ~1e-3 |grad J|), so at
+      eps = 1e-6 the difference sits on the rounding floor of the two
+      perturbed trajectories (~1e-16 J / eps): judged there against the
+      Cauchy-Schwarz scale, |fd - ad| <= 1e-8 |grad J|; at eps = 1e-4 against
+      its own derivative, 1e-6 relative;
+    - unit volume-mean norm (mean of d^2 over the entries = 1, i.e. unit RMS
+      per entry), eps = 1e-6: 1e-6 relative to its own derivative.
+    Tangent/adjoint dot test of one step: 1e-13 of |J dU| |lam|."""
+    import torch
+
+    from ns3d_wcns_amd.adjoint import central_difference, mean_kinetic_energy, rollout, state_tensor
+
+    n, nsteps = 128, 50
+    say(f"adjoint {n}^3 x {nsteps} steps: forward for the dt sequence")
+    prim, ma = isotropic_state(n, mach_t=0.3, seed=12345)
+    P = params(ma)
+    f = DeviceField(box(n), P)
+    f.set_stream(torch.cuda.current_stream().cuda_stream)
+    f.initialize_primitive(*prim)
+    U = state_tensor(f)
+    dts = []
+    for _ in range(nsteps):
+        dts.append(f.compute_timestep())
+        f.step(dts[-1])
+    say("  checkpointed gradient (uniform:10)")
+    U0 = U.clone().requires_grad_(True)
+    stats = {}
+    J = mean_kinetic_energy(rollout(U0, f, dts, "uniform:10", stats))
+    J.backward()
+    grad = U0.grad
+    assert stats["adjoint_steps"] == nsteps
+    gnorm = float(torch.linalg.vector_norm(grad[1:]))
+    gen = torch.Generator(device="cpu").manual_seed(2024)
+    entries = 4 * n ** 3
+    worst = {"unit_l2_1e-6_cs": 0.0, "unit_l2_1e-4": 0.0, "unit_rms_1e-6": 0.0}
+    for k in range(8):
+        d = _interior_randn(U.shape, gen, (1, 2, 3, 4))
+        d /= torch.linalg.vector_norm(d)
+        ad = float((grad * d).sum())
+        fd6 = float(central_difference(f, U, d, dts, 1e-6))
+        fd4 = float(central_difference(f, U, d, dts, 1e-4))
+        cs6 = abs(fd6 - ad) / gnorm
+        r4 = abs(fd4 - ad) / abs(ad)
+        dr = d * math.sqrt(entries)  # unit RMS per entry
+        adr = float((grad * dr).sum())
+        fdr = float(central_difference(f, U, dr, dts, 1e-6))
+        rr = abs(fdr - adr) / abs(adr)
+        say(f"  direction {k}: unit-L2 ad {ad:.6e} eps1e-6 rel {abs(fd6 - ad) / abs(ad):.2e} (over |grad| "
+            f"{cs6:.2e}), eps1e-4 rel {r4:.2e}; unit-RMS ad {adr:.6e} eps1e-6 rel {rr:.2e}")
+        worst["unit_l2_1e-6_cs"] = max(worst["unit_l2_1e-6_cs"], cs6)
+        worst["unit_l2_1e-4"] = max(worst["unit_l2_1e-4"], r4)
+        worst["unit_rms_1e-6"] = max(worst["unit_rms_1e-6"], rr)
+    say(f"  worst {worst}")
+    assert worst["unit_l2_1e-6_cs"] < 1e-8
+    assert worst["unit_l2_1e-4"] < 1e-6
+    assert worst["unit_rms_1e-6"] < 1e-6
+    # dot test of one step at the initial state
+    gg = torch.Generator(device="cpu").manual_seed(8)
+    dU = _interior_randn(U.shape, gg, range(5)) * 1e-6
+    lam = _interior_randn(U.shape, gg, range(5))
+    tl, adj = dU.clone(), lam.clone()
+    f.set_state(_ptrs(U))
+    f.tangent_step(dts[0], _ptrs(tl))
+    f.set_state(_ptrs(U))
+    f.adjoint_step(dts[0], _ptrs(adj))
+    torch.cuda.synchronize()
+    I = slice(NG, -NG)
+    a = float((tl[:, I, I, I] * lam[:, I, I, I]).sum())
+    b = float((dU[:, I, I, I] * adj[:, I, I, I]).sum())
+    scale = float(tl[:, I, I, I].norm() * lam[:, I, I, I].norm())
+    say(f"  dot test: {a!r} vs {b!r}, |diff| / scale {abs(a - b) / scale:.2e}")
+    assert abs(a - b) <= 1e-13 * scale
+    f.close()
+
+
+def _velocity_perturbed(U, d, eps, gamma):
+    """The state with velocity u + eps d at fixed rho and p (interior)."""
+    from ns3d_wcns_amd.adjoint import interior
+
+    V = U.clone()
+    I, Iv = interior(U), interior(V)
+    rho = I[0]
+    p = (gamma - 1.0) * (I[4] - 0.5 * (I[1] ** 2 + I[2] ** 2 + I[3] ** 2) / rho)
+    vel = [I[1 + c] / rho + eps * d[c] for c in range(3)]
+    for c in range(3):
+        Iv[1 + c] = rho * vel[c]
+    Iv[4] = p / (gamma - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+    return V
+
+
+def test_adjoint_at_scale_reduced(gpu):
+    """J = sum_n dt_n eps(U_{n+1}) (volume-mean viscous dissipation) over 20
+    CFL steps at 64^3, Ma_t = 0.5, Re = 1600, WCNS5 + HLLC, gradient with
+    respect to the initial velocity at fixed rho and p
+    (dJ/du_c = rho dJ/d(rho u_c) + rho u_c dJ/dE).
+
+    - binomial checkpoints (3 snapshots) give the gradient of every state
+      kept to 1e-12, and the checkpointed J equals a plain forward's;
+    - FP64 forward: central differences (eps = 1e-6) along 4 seeded velocity
+      directions of unit RMS per cell agree with the adjoint to 1e-6;
+    - mixed forward with the FP64 adjoint about its trajectory: the
+      gradient is the FP64 one to 1e-4 (norm-wise), and so is its
+      agreement with the FP64 central differences;
+    - tangent/adjoint dot test of one step: 1e-13."""
+    import torch
+
+    from ns3d_wcns_amd.adjoint import (dissipation_rate, interior, state_tensor, time_integral,
+                                       time_integral_value)
+
+    n, nsteps = 64, 20
+    prim, ma = isotropic_state(n, mach_t=0.5, seed=12345)
+    P = params(ma, re=1600.0)
+    G = box(n)
+    stream = torch.cuda.current_stream().cuda_stream
+    fields = {}
+    for prec in (PRECISION_FP64, PRECISION_MIXED):
+        f = DeviceField(G, P, precision=prec)
+        f.set_stream(stream)
+        f.initialize_primitive(*prim)
+        fields[prec] = f
+    U = state_tensor(fields[PRECISION_MIXED])
+    dts = []
+    fm = fields[PRECISION_MIXED]
+    for _ in range(nsteps):
+        dts.append(fm.compute_timestep())
+        fm.step(dts[-1])
+    density = lambda X: dissipation_rate(X, P, G)  # noqa: E731
+
+    def grad_vel(f, policy):
+        U0 = U.clone().requires_grad_(True)
+        st = {}
+        J = time_integral(U0, f, dts, density, policy, st)
+        J.backward()
+        I, gI = interior(U), interior(U0.grad)
+        gv = torch.stack([I[0] * gI[1 + c] + I[1 + c] * gI[4] for c in range(3)])
+        return float(J.detach()), gv, st
+
+    say(f"adjoint at scale (reduced): {n}^3 x {nsteps} steps, binomial:3, FP64 and mixed forwards")
+    J64, g64, st = grad_vel(fields[PRECISION_FP64], "binomial:3")
+    say(f"  FP64 forward J {J64!r}, stats { {k: v for k, v in st.items() if k != 'live'} }")
+    J64_all, g64_all, _ = grad_vel(fields[PRECISION_FP64], "all")
+    assert J64 == J64_all
+    assert float((g64 - g64_all).abs().max() / g64_all.abs().max()) < 1e-12
+    assert abs(J64 - time_integral_value(fields[PRECISION_FP64], U, dts, density)) <= 1e-14 * abs(J64)
+    Jm, gm, stm = grad_vel(fields[PRECISION_MIXED], "binomial:3")
+    gdiff = float(torch.linalg.vector_norm(gm - g64) / torch.linalg.vector_norm(g64))
+    say(f"  mixed forward J {Jm!r} (rel to FP64 {abs(Jm - J64) / abs(J64):.1e}); gradient vs FP64 {gdiff:.2e}")
+    assert gdiff < 1e-4
+    gen = torch.Generator(device="cpu").manual_seed(99)
+    eps = 1e-6
+    for k in range(4):
+        d = torch.randn((3, n, n, n), generator=gen, dtype=torch.float64).cuda()
+        vals = [time_integral_value(fields[PRECISION_FP64], _velocity_perturbed(U, d, s * eps, P.gamma), dts, density)
+                for s in (1.0, -1.0)]
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        ad64, adm = float((g64 * d).sum()), float((gm * d).sum())
+        r64, rm = abs(fd - ad64) / abs(ad64), abs(fd - adm) / abs(adm)
+        say(f"  direction {k}: fd {fd:.10e} FP64 adjoint rel {r64:.2e}, mixed-forward adjoint rel {rm:.2e}")
+        assert r64 < 1e-6, (k, r64)
+        assert rm < 1e-4, (k, rm)
+    # dot test of one step
+    f = fields[PRECISION_FP64]
+    gg = torch.Generator(device="cpu").manual_seed(5)
+    dU = _interior_randn(U.shape, gg, range(5)) * 1e-6
+    lam = _interior_randn(U.shape, gg, range(5))
+    tl, adj = dU.clone(), lam.clone()
+    f.set_state(_ptrs(U))
+    f.tangent_step(dts[0], _ptrs(tl))
+    f.set_state(_ptrs(U))
+    f.adjoint_step(dts[0], _ptrs(adj))
+    torch.cuda.synchronize()
+    I = slice(NG, -NG)
+    a = float((tl[:, I, I, I] * lam[:, I, I, I]).sum())
+    b = float((dU[:, I, I, I] * adj[:, I, I, I]).sum())
+    scale = float(tl[:, I, I, I].norm() * lam[:, I, I, I].norm())
+    say(f"  dot test |diff| / scale {abs(a - b) / scale:.2e}")
+    assert abs(a - b) <= 1e-13 * scale
+    for f in fields.values():
+        f.close()
