@@ -167,7 +167,9 @@ NS3D_HD R mdcd_linear(const SchemeCoef<R> &c, R f0, R f1, R f2, R f3, R f4, R f5
 // smoothness sensor picks the linear MDCD scheme when sai > 0.4, otherwise a
 // four-stencil WENO-Z-type combination with the 6-point indicator beta3.
 // The 6-point smoothness indicator beta3 of the MDCD hybrid scheme
-// (interpolation.cpp:193-200), in the reference's evaluation order.
+// (interpolation.cpp:193-200), in the reference's evaluation order and with
+// its f0^2 coefficient 271799 (the indicator's definition gives 271779, so
+// this beta3 exceeds it by f0^2 / 6048 and is c^2 / 6048 on constant data).
 template <class T>
 NS3D_HD T beta3_of(T f0, T f1, T f2, T f3, T f4, T f5) {
     return (T(271799) * (f0 * f0) +

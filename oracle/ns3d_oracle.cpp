@@ -10,7 +10,18 @@
 // `-O3` build on x86-64 without -march is), it would reproduce the reference's
 // bits.  The reference itself cannot be compiled here (every hot-path file
 // includes <fftw3-mpi.h>, which this image lacks), so each function is pinned by
-// the closed forms, invariants and brute-force checks in tests/test_oracle_*.py.
+// the closed forms, invariants, symbolic derivations and brute-force checks in
+// tests/test_oracle_*.py; the orc_*_parts hooks expose scheme internals to them.
+// Parity unpinned (read from the reference's text, no mathematics fixes them):
+//   - characteristic-wise interpolation: which cell's eigenvectors project the
+//     stencil (cell i for the left state, cell i+1 for the right);
+//   - the MDCD-WENO linear weights' slopes in (disp, diss) (3/2 and 9/2), beyond
+//     their consistency, central value, mirror symmetry and linearity;
+//   - AUSM's pressure- and velocity-diffusion terms beyond their vanishing on
+//     equal states (Kp = 0.25, Ku = 0.75 and the f_a scaling).
+// Two reference quirks are reproduced: the 6-cell MDCD indicator's f0^2
+// coefficient 271799 (the indicator's definition gives 271779), and MDCD-WENO
+// linear weights whose smooth limit is exact only through quadratics.
 //
 // Storage is the reference's: SoA double arrays of sx*sy*sz cells (ghosts
 // included, x fastest, field_structures.h:64-67) and face arrays of
@@ -98,8 +109,14 @@ double mdcd_linear(const double *s, const orc_params &P) {
 }
 
 // interpolation.cpp:135-209.  sai > 0.4 selects the linear scheme (the code, not
-// the comment at 166-167, is followed).
-double mdcd_hybrid(const double *s, const orc_params &P) {
+// the comment at 166-167, is followed).  The pieces are kept so that the tests
+// can check them one by one (orc_mdcd_parts); the WENO-branch value is formed
+// for every stencil, and returned only when sai <= 0.4.
+struct MdcdParts {
+    double sai, beta[4], tau6, d[4], q[4], weno, value;
+};
+MdcdParts mdcd_hybrid_parts(const double *s, const orc_params &P) {
+    MdcdParts m;
     double diss = P.mdcd_diss, disp = P.mdcd_disp;
     double eps_small = 1e-40;
     double f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4], f5 = s[5];
@@ -110,33 +127,37 @@ double mdcd_hybrid(const double *s, const orc_params &P) {
     double b2 = std::abs(f3 - f4) + std::abs(f3 - 2 * f4 + f5);
     double sai1 = (2 * a1 * b1 + eps) / (a1 * a1 + b1 * b1 + eps);
     double sai2 = (2 * a2 * b2 + eps) / (a2 * a2 + b2 * b2 + eps);
-    double sai = std::min(sai1, sai2);
-    if (sai > 0.4) return mdcd_linear(s, P);
-    double q0 = (3 * f0 - 10 * f1 + 15 * f2) / 8.0;
-    double q1 = (-f1 + 6 * f2 + 3 * f3) / 8.0;
-    double q2 = (3 * f2 + 6 * f3 - f4) / 8.0;
-    double q3 = (15 * f3 - 10 * f4 + 3 * f5) / 8.0;
-    double d0 = 1.5 * (disp + diss);
-    double d1 = 0.5 - 1.5 * (disp - 3 * diss);
-    double d2 = 0.5 - 1.5 * (disp + 3 * diss);
-    double d3 = 1.5 * (disp - diss);
-    double beta0 = 13.0 / 12.0 * std::pow(f0 - 2 * f1 + f2, 2) + 0.25 * std::pow(f0 - 4 * f1 + 3 * f2, 2);
-    double beta1 = 13.0 / 12.0 * std::pow(f1 - 2 * f2 + f3, 2) + 0.25 * std::pow(f1 - f3, 2);
-    double beta2 = 13.0 / 12.0 * std::pow(f2 - 2 * f3 + f4, 2) + 0.25 * std::pow(3 * f2 - 4 * f3 + f4, 2);
-    double beta3 = (271799 * std::pow(f0, 2) +
-                    f0 * (-2380800 * f1 + 4086352 * f2 - 3462252 * f3 + 1458762 * f4 - 245620 * f5) +
-                    f1 * (5653317 * f1 - 20427884 * f2 + 17905032 * f3 - 7727988 * f4 + 1325006 * f5) +
-                    f2 * (19510972 * f2 - 35817664 * f3 + 15929912 * f4 - 2792660 * f5) +
-                    f3 * (17195652 * f3 - 15880404 * f4 + 2863984 * f5) + f4 * (3824847 * f4 - 1429976 * f5) +
-                    139633 * std::pow(f5, 2)) /
-                   120960.0;
-    double tau6 = std::abs(beta3 - (beta0 + 4 * beta1 + beta2) / 6.0);
-    double c0 = d0 * std::pow(20 + tau6 / (beta0 + eps_small), 2);
-    double c1 = d1 * std::pow(20 + tau6 / (beta1 + eps_small), 2);
-    double c2 = d2 * std::pow(20 + tau6 / (beta2 + eps_small), 2);
-    double c3 = d3 * std::pow(20 + tau6 / (beta3 + eps_small), 2);
-    return (c0 * q0 + c1 * q1 + c2 * q2 + c3 * q3) / (c0 + c1 + c2 + c3);
+    m.sai = std::min(sai1, sai2);
+    m.q[0] = (3 * f0 - 10 * f1 + 15 * f2) / 8.0;
+    m.q[1] = (-f1 + 6 * f2 + 3 * f3) / 8.0;
+    m.q[2] = (3 * f2 + 6 * f3 - f4) / 8.0;
+    m.q[3] = (15 * f3 - 10 * f4 + 3 * f5) / 8.0;
+    m.d[0] = 1.5 * (disp + diss);
+    m.d[1] = 0.5 - 1.5 * (disp - 3 * diss);
+    m.d[2] = 0.5 - 1.5 * (disp + 3 * diss);
+    m.d[3] = 1.5 * (disp - diss);
+    m.beta[0] = 13.0 / 12.0 * std::pow(f0 - 2 * f1 + f2, 2) + 0.25 * std::pow(f0 - 4 * f1 + 3 * f2, 2);
+    m.beta[1] = 13.0 / 12.0 * std::pow(f1 - 2 * f2 + f3, 2) + 0.25 * std::pow(f1 - f3, 2);
+    m.beta[2] = 13.0 / 12.0 * std::pow(f2 - 2 * f3 + f4, 2) + 0.25 * std::pow(3 * f2 - 4 * f3 + f4, 2);
+    // the reference's 6-point indicator (interpolation.cpp:193-200), f0^2
+    // coefficient 271799 included
+    m.beta[3] = (271799 * std::pow(f0, 2) +
+                 f0 * (-2380800 * f1 + 4086352 * f2 - 3462252 * f3 + 1458762 * f4 - 245620 * f5) +
+                 f1 * (5653317 * f1 - 20427884 * f2 + 17905032 * f3 - 7727988 * f4 + 1325006 * f5) +
+                 f2 * (19510972 * f2 - 35817664 * f3 + 15929912 * f4 - 2792660 * f5) +
+                 f3 * (17195652 * f3 - 15880404 * f4 + 2863984 * f5) + f4 * (3824847 * f4 - 1429976 * f5) +
+                 139633 * std::pow(f5, 2)) /
+                120960.0;
+    m.tau6 = std::abs(m.beta[3] - (m.beta[0] + 4 * m.beta[1] + m.beta[2]) / 6.0);
+    double c0 = m.d[0] * std::pow(20 + m.tau6 / (m.beta[0] + eps_small), 2);
+    double c1 = m.d[1] * std::pow(20 + m.tau6 / (m.beta[1] + eps_small), 2);
+    double c2 = m.d[2] * std::pow(20 + m.tau6 / (m.beta[2] + eps_small), 2);
+    double c3 = m.d[3] * std::pow(20 + m.tau6 / (m.beta[3] + eps_small), 2);
+    m.weno = (c0 * m.q[0] + c1 * m.q[1] + c2 * m.q[2] + c3 * m.q[3]) / (c0 + c1 + c2 + c3);
+    m.value = m.sai > 0.4 ? mdcd_linear(s, P) : m.weno;
+    return m;
 }
+double mdcd_hybrid(const double *s, const orc_params &P) { return mdcd_hybrid_parts(s, P).value; }
 
 // interpolation.cpp:13-77: reverse the 6-point stencil for flag < 0, then
 // dispatch.  Interpolation id 4 routes to weno5_reconstruction
@@ -325,25 +346,66 @@ void hll(double *F, const double *UL, const double *UR, double nx, double ny, do
     }
 }
 
+// Star state of side K for contact speed SM (InviscousFlux.cpp:540-560 for the
+// left side, 566-585 for the right; 665-716 for HLLC_p).
+void star_state(const LR &K, double unK, double SK, double SM, double nx, double ny, double nz, double Us[5]) {
+    double rs = K.rho * (SK - unK) / (SK - SM);
+    double tx = K.u - unK * nx, ty = K.v - unK * ny, tz = K.w - unK * nz;
+    double sx = SM * nx + tx, sy = SM * ny + ty, sz = SM * nz + tz;
+    double Es = K.E / K.rho + (SM - unK) * (SM + K.p / (K.rho * (SK - unK)));
+    Us[0] = rs;
+    Us[1] = rs * sx;
+    Us[2] = rs * sy;
+    Us[3] = rs * sz;
+    Us[4] = rs * Es;
+}
+
 // Star-region flux shared by HLLC and HLLC_p (InviscousFlux.cpp:538-591, 663-716).
 void hllc_star(double *F, const LR &L, const LR &R, const double *UL, const double *UR, const double *FL,
                const double *FR, double unL, double unR, double SL, double SR, double SM, double nx, double ny,
                double nz) {
+    double Us[5];
     if (SM >= 0.0) {
-        double rs = L.rho * (SL - unL) / (SL - SM);
-        double tx = L.u - unL * nx, ty = L.v - unL * ny, tz = L.w - unL * nz;
-        double sx = SM * nx + tx, sy = SM * ny + ty, sz = SM * nz + tz;
-        double Es = L.E / L.rho + (SM - unL) * (SM + L.p / (L.rho * (SL - unL)));
-        double Us[5] = {rs, rs * sx, rs * sy, rs * sz, rs * Es};
+        star_state(L, unL, SL, SM, nx, ny, nz, Us);
         for (int i = 0; i < 5; ++i) F[i] = FL[i] + SL * (Us[i] - UL[i]);
     } else {
-        double rs = R.rho * (SR - unR) / (SR - SM);
-        double tx = R.u - unR * nx, ty = R.v - unR * ny, tz = R.w - unR * nz;
-        double sx = SM * nx + tx, sy = SM * ny + ty, sz = SM * nz + tz;
-        double Es = R.E / R.rho + (SM - unR) * (SM + R.p / (R.rho * (SR - unR)));
-        double Us[5] = {rs, rs * sx, rs * sy, rs * sz, rs * Es};
+        star_state(R, unR, SR, SM, nx, ny, nz, Us);
         for (int i = 0; i < 5; ++i) F[i] = FR[i] + SR * (Us[i] - UR[i]);
     }
+}
+
+// Wave speeds of HLLC (Davis estimates; contact speed from the pressure
+// balance, or the Roe-averaged normal velocity when |den| <= 1e-12;
+// InviscousFlux.cpp:489-527) and of HLLC_p (Toro's PVRS pressure and shock
+// factors, InviscousFlux.cpp:640-661).
+struct Waves {
+    double SL, SR, SM, den;
+};
+Waves hllc_waves(const LR &L, const LR &R, double unL, double unR, double nx, double ny, double nz) {
+    Waves w;
+    w.SL = std::min(unL - L.a, unR - R.a);
+    w.SR = std::max(unL + L.a, unR + R.a);
+    double sqL = std::sqrt(L.rho), sqR = std::sqrt(R.rho), ssum = sqL + sqR;
+    double u_roe = (sqL * L.u + sqR * R.u) / ssum;
+    double v_roe = (sqL * L.v + sqR * R.v) / ssum;
+    double w_roe = (sqL * L.w + sqR * R.w) / ssum;
+    double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+    double num = R.rho * unR * (w.SR - unR) - L.rho * unL * (w.SL - unL) + L.p - R.p;
+    w.den = R.rho * (w.SR - unR) - L.rho * (w.SL - unL);
+    if (std::fabs(w.den) > 1e-12) w.SM = num / w.den; else w.SM = un_roe;
+    return w;
+}
+Waves hllc_p_waves(const LR &L, const LR &R, double unL, double unR, double g) {
+    Waves w;
+    double ps = 0.5 * (L.p + R.p) - 0.125 * (unR - unL) * (L.rho + R.rho) * (L.a + R.a);
+    ps = std::max(0.0, ps);
+    if (ps <= L.p) w.SL = unL - L.a;
+    else w.SL = unL - L.a * std::sqrt((g + 1.0) * ps / (2.0 * g * L.p) + (g - 1.0) / (2.0 * g));
+    if (ps <= R.p) w.SR = unR + R.a;
+    else w.SR = unR + R.a * std::sqrt((g + 1.0) * ps / (2.0 * g * R.p) + (g - 1.0) / (2.0 * g));
+    w.den = L.rho * (w.SL - unL) - R.rho * (w.SR - unR);
+    w.SM = (R.p - L.p + L.rho * unL * (w.SL - unL) - R.rho * unR * (w.SR - unR)) / w.den;
+    return w;
 }
 
 // InviscousFlux.cpp:449-592
@@ -356,16 +418,8 @@ void hllc(double *F, const double *UL, const double *UR, double nx, double ny, d
     double SR = std::max(unL + L.a, unR + R.a);
     if (SL >= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FL[i]; return; }
     if (SR <= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FR[i]; return; }
-    double sqL = std::sqrt(L.rho), sqR = std::sqrt(R.rho), ssum = sqL + sqR;
-    double u_roe = (sqL * L.u + sqR * R.u) / ssum;
-    double v_roe = (sqL * L.v + sqR * R.v) / ssum;
-    double w_roe = (sqL * L.w + sqR * R.w) / ssum;
-    double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
-    double SM = un_roe;
-    double num = R.rho * unR * (SR - unR) - L.rho * unL * (SL - unL) + L.p - R.p;
-    double den = R.rho * (SR - unR) - L.rho * (SL - unL);
-    if (std::fabs(den) > 1e-12) SM = num / den; else SM = un_roe;
-    hllc_star(F, L, R, UL, UR, FL, FR, unL, unR, SL, SR, SM, nx, ny, nz);
+    Waves w = hllc_waves(L, R, unL, unR, nx, ny, nz);
+    hllc_star(F, L, R, UL, UR, FL, FR, unL, unR, SL, SR, w.SM, nx, ny, nz);
 }
 
 // InviscousFlux.cpp:594-717 (pressure-based wave speeds, Toro)
@@ -374,18 +428,22 @@ void hllc_p(double *F, const double *UL, const double *UR, double nx, double ny,
     double FL[5], FR[5], unL, unR;
     phys_flux(L, UL, nx, ny, nz, FL, unL);
     phys_flux(R, UR, nx, ny, nz, FR, unR);
-    double ps = 0.5 * (L.p + R.p) - 0.125 * (unR - unL) * (L.rho + R.rho) * (L.a + R.a);
-    ps = std::max(0.0, ps);
-    double SL, SR;
-    if (ps <= L.p) SL = unL - L.a;
-    else SL = unL - L.a * std::sqrt((g + 1.0) * ps / (2.0 * g * L.p) + (g - 1.0) / (2.0 * g));
-    if (ps <= R.p) SR = unR + R.a;
-    else SR = unR + R.a * std::sqrt((g + 1.0) * ps / (2.0 * g * R.p) + (g - 1.0) / (2.0 * g));
-    if (SL >= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FL[i]; return; }
-    if (SR <= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FR[i]; return; }
-    double SM = (R.p - L.p + L.rho * unL * (SL - unL) - R.rho * unR * (SR - unR)) /
-                (L.rho * (SL - unL) - R.rho * (SR - unR));
-    hllc_star(F, L, R, UL, UR, FL, FR, unL, unR, SL, SR, SM, nx, ny, nz);
+    Waves w = hllc_p_waves(L, R, unL, unR, g);
+    if (w.SL >= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FL[i]; return; }
+    if (w.SR <= 0.0) { for (int i = 0; i < 5; ++i) F[i] = FR[i]; return; }
+    hllc_star(F, L, R, UL, UR, FL, FR, unL, unR, w.SL, w.SR, w.SM, nx, ny, nz);
+}
+
+// Split Mach numbers and pressure weights of the reference's AUSM
+// (InviscousFlux.cpp:777-818): outside |M| < 1 the upwind values, inside the
+// polynomials 1/4 (M +- 1)^2 and 1/4 (M +- 1)^2 (2 -+ M).
+double ausm_m4p(double M) { return std::abs(M) >= 1.0 ? 0.5 * (M + std::abs(M)) : 0.25 * (M + 1.0) * (M + 1.0); }
+double ausm_m4m(double M) { return std::abs(M) >= 1.0 ? 0.5 * (M - std::abs(M)) : -0.25 * (M - 1.0) * (M - 1.0); }
+double ausm_p5p(double M) {
+    return std::abs(M) >= 1.0 ? 0.5 * (1.0 + (M > 0 ? 1.0 : -1.0)) : 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M);
+}
+double ausm_p5m(double M) {
+    return std::abs(M) >= 1.0 ? 0.5 * (1.0 - (M > 0 ? 1.0 : -1.0)) : 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M);
 }
 
 // InviscousFlux.cpp:719-846.  Note the reference's p_half uses the x-velocity
@@ -410,18 +468,10 @@ void ausm(double *F, const double *UL, const double *UR, double nx, double ny, d
     double M_bar2 = (un_L * un_L + un_R * un_R) / (2.0 * a_bar * a_bar);
     double M_o2 = std::min(1.0, std::max(M_bar2, 0.5));
     double f_a = M_o2 * (2.0 - M_o2);
-    auto M4p = [](double M) { return std::abs(M) >= 1.0 ? 0.5 * (M + std::abs(M)) : 0.25 * (M + 1.0) * (M + 1.0); };
-    auto M4m = [](double M) { return std::abs(M) >= 1.0 ? 0.5 * (M - std::abs(M)) : -0.25 * (M - 1.0) * (M - 1.0); };
-    double Mp = M4p(ML), Mm = M4m(MR);
+    double Mp = ausm_m4p(ML), Mm = ausm_m4m(MR);
     double M_half = Mp + Mm - Kp / f_a * std::max(1.0 - M_bar2, 0.0) * ((p_R - p_L) / (rho_bar * a_bar * a_bar));
     double m_dot = a_bar * M_half * (M_half >= 0.0 ? rho_L : rho_R);
-    auto P5p = [](double M) {
-        return std::abs(M) >= 1.0 ? 0.5 * (1.0 + (M > 0 ? 1.0 : -1.0)) : 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M);
-    };
-    auto P5m = [](double M) {
-        return std::abs(M) >= 1.0 ? 0.5 * (1.0 - (M > 0 ? 1.0 : -1.0)) : 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M);
-    };
-    double pp = P5p(ML), pm = P5m(MR);
+    double pp = ausm_p5p(ML), pm = ausm_p5m(MR);
     double p_half = pp * p_L + pm * p_R - Ku * pm * pp * (rho_L + rho_R) * f_a * (u_R - u_L) * a_bar;
     double uf, vf, wf, Hf;
     if (m_dot >= 0.0) { uf = u_L; vf = v_L; wf = w_L; Hf = H_L; }
@@ -1009,6 +1059,46 @@ void orc_eigen(const double *ul, const double *ur, int dim, double gamma, double
             R[5 * a + b] = Rm[a][b];
         }
 }
+// Pieces of the schemes, for the checks of the checker itself.
+// mdcd: out[16] = sai, beta0..3, tau6, d0..3, q0..3, WENO-branch value, result.
+void orc_mdcd_parts(const double *s, const orc_params *P, double *out) {
+    MdcdParts m = mdcd_hybrid_parts(s, *P);
+    out[0] = m.sai;
+    for (int k = 0; k < 4; ++k) out[1 + k] = m.beta[k];
+    out[5] = m.tau6;
+    for (int k = 0; k < 4; ++k) out[6 + k] = m.d[k];
+    for (int k = 0; k < 4; ++k) out[10 + k] = m.q[k];
+    out[14] = m.weno;
+    out[15] = m.value;
+}
+// HLLC (pressure_based 0) or HLLC_p (1) along axis dim: out[19] = SL, SR, SM,
+// den, the left and right star states U*L, U*R (each for the computed SM),
+// and the face flux.
+void orc_hllc_parts(double gamma, const double *UL, const double *UR, int dim, int pressure_based, double *out) {
+    double n[3] = {0.0, 0.0, 0.0};
+    n[dim] = 1.0;
+    LR L = unpack(UL, gamma), R = unpack(UR, gamma);
+    double FL[5], FR[5], unL, unR;
+    phys_flux(L, UL, n[0], n[1], n[2], FL, unL);
+    phys_flux(R, UR, n[0], n[1], n[2], FR, unR);
+    Waves w = pressure_based ? hllc_p_waves(L, R, unL, unR, gamma) : hllc_waves(L, R, unL, unR, n[0], n[1], n[2]);
+    out[0] = w.SL;
+    out[1] = w.SR;
+    out[2] = w.SM;
+    out[3] = w.den;
+    star_state(L, unL, w.SL, w.SM, n[0], n[1], n[2], out + 4);
+    star_state(R, unR, w.SR, w.SM, n[0], n[1], n[2], out + 9);
+    if (pressure_based) hllc_p(out + 14, UL, UR, n[0], n[1], n[2], gamma);
+    else hllc(out + 14, UL, UR, n[0], n[1], n[2], gamma);
+}
+// AUSM split functions at Mach number M: out[4] = M+, M-, P+, P-.
+void orc_ausm_split(double M, double *out) {
+    out[0] = ausm_m4p(M);
+    out[1] = ausm_m4m(M);
+    out[2] = ausm_p5p(M);
+    out[3] = ausm_p5m(M);
+}
+
 double orc_sutherland(const orc_params *P, double T) { return P->mu * std::pow(T, 1.5) * (1.0 + P->S_ref) / (T + P->S_ref); }
 
 }  // extern "C"
