@@ -167,6 +167,21 @@ int ns3d_runge_kutta_3_dev(ns3d_ctx *ctx, const double *d_dt);
 int ns3d_compute_timestep(ns3d_ctx *ctx, double *dt_out);
 /* Same, result left in device memory at d_dt. */
 int ns3d_compute_timestep_dev(ns3d_ctx *ctx, double *d_dt);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  ns3d_compute_timestep_dev without a context: the same kernels
+ * and the same dt, from the caller's ghosted device fields rho, rho u and E
+ * of the box `grid` (interior read), into the device double d_dt, enqueued on
+ * `stream` (a hipStream_t as void*; NULL = the null stream).  It allocates
+ * nothing: the reduction's block maxima live in a fixed device array of
+ * NS3D_NOALLOC_SCRATCH_DOUBLES doubles (64 KiB) per device, held until the
+ * process ends.  Any box ns3d_create accepts fits it (the first pass uses at
+ * most 8192 blocks); other boxes are rejected with NS3D_E_ARG /
+ * NS3D_E_UNSUPPORTED.  Every call on one device shares that array: calls
+ * that overlap on one device (concurrent streams or host threads) race. */
+#define NS3D_NOALLOC_SCRATCH_DOUBLES 8192
+int ns3d_compute_timestep_noalloc(const ns3d_grid *grid, const ns3d_params *params, const double *d_rho,
+                                  const double *d_rhou, const double *d_E, double *d_dt, void *stream);
 
 /* time_advance (maincycle.cpp:7-113) without monitors and output: per step
  * record U0, dt from the CFL (or dt_fixed when > 0), clip dt to

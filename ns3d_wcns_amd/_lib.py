@@ -93,6 +93,7 @@ _SIGS = {
     "ns3d_runge_kutta_3_dev": [_VP, _VP],
     "ns3d_compute_timestep": [_VP, _DP],
     "ns3d_compute_timestep_dev": [_VP, _VP],
+    "ns3d_compute_timestep_noalloc": [ctypes.POINTER(CGrid), ctypes.POINTER(CParams), _VP, _VP, _VP, _VP, _VP],
     "ns3d_time_advance": [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _DP,
                           ctypes.POINTER(ctypes.c_int)],
     "ns3d_advance_steps": [_VP, ctypes.c_int, _VP],

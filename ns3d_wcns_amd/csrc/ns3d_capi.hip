@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -22,6 +23,15 @@
 #include "ns3d_observe.h"
 
 using namespace ns3d;
+
+// Block maxima of the CFL reduction for ns3d_compute_timestep_noalloc: one
+// slot per block of the first pass (at most REDUCE_BLOCKS blocks, whatever
+// the box).  Each device holds its own copy until the process ends; every
+// call on a device shares it, so calls that overlap on one device race.
+// Nothing in it needs clearing: each block writes its slot before the
+// second pass reads it.
+static_assert(REDUCE_BLOCKS <= NS3D_NOALLOC_SCRATCH_DOUBLES, "CFL block maxima exceed the no-allocation scratch");
+static __device__ __align__(256) double scratch[NS3D_NOALLOC_SCRATCH_DOUBLES];
 
 struct ns3d_ctx {
     Box b;
@@ -111,6 +121,48 @@ int fail(int code, const char *fmt, ...) {
     do {                                                                \
         if (!(c)) return fail(NS3D_E_ARG, "%s: null context", __func__); \
     } while (0)
+
+int check_box(const ns3d_grid *grid) {
+    if (grid->ng != NG) return fail(NS3D_E_UNSUPPORTED, "ghost layers must be %d (got %d)", NG, grid->ng);
+    if (grid->nx < 6 || grid->ny < 6 || grid->nz < 6)
+        return fail(NS3D_E_ARG, "each interior extent must be >= 6 (got %d x %d x %d)", grid->nx, grid->ny,
+                    grid->nz);
+    const double ghosted = (double)(grid->nx + 2 * NG) * (grid->ny + 2 * NG) * (grid->nz + 2 * NG);
+    if (ghosted >= 2147483647.0)
+        return fail(NS3D_E_ARG, "box of %g cells exceeds the reference's 32-bit cell index", ghosted);
+    if (!(grid->dx > 0 && grid->dy > 0 && grid->dz > 0)) return fail(NS3D_E_ARG, "grid spacing must be positive");
+    return NS3D_OK;
+}
+
+Box box_of(const ns3d_grid &g) {
+    Box b;
+    b.nx = g.nx; b.ny = g.ny; b.nz = g.nz;
+    b.sx = b.nx + 2 * NG; b.sy = b.ny + 2 * NG; b.sz = b.nz + 2 * NG;
+    return b;
+}
+
+// Per-device address of the allocation-free entry point's scratch array
+// (hipGetSymbolAddress, once per device with that device current; the
+// array's name is a host-side stand-in and is never passed to a kernel).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(double **out) {
+    int dev = -1;
+    NS3D_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(NS3D_E_UNSUPPORTED, "device ordinal %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(NS3D_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return NS3D_OK;
+}
 
 int check_params(const ns3d_params *p) {
     if (!p) return fail(NS3D_E_ARG, "null params");
@@ -783,24 +835,19 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     *out = nullptr;
     int rc = check_params(params);
     if (rc) return rc;
-    if (grid->ng != NG) return fail(NS3D_E_UNSUPPORTED, "ghost layers must be %d (got %d)", NG, grid->ng);
-    if (grid->nx < 6 || grid->ny < 6 || grid->nz < 6)
-        return fail(NS3D_E_ARG, "each interior extent must be >= 6 (got %d x %d x %d)", grid->nx, grid->ny,
-                    grid->nz);
-    const double ghosted = (double)(grid->nx + 2 * NG) * (grid->ny + 2 * NG) * (grid->nz + 2 * NG);
-    if (ghosted >= 2147483647.0)
-        return fail(NS3D_E_ARG, "box of %g cells exceeds the reference's 32-bit cell index", ghosted);
-    if (!(grid->dx > 0 && grid->dy > 0 && grid->dz > 0)) return fail(NS3D_E_ARG, "grid spacing must be positive");
+    if ((rc = check_box(grid))) return rc;
     if (precision != NS3D_PRECISION_FP64 && precision != NS3D_PRECISION_MIXED)
         return fail(NS3D_E_ARG, "unknown precision %d", precision);
+    // set-up of the allocation-free entry point on the current device
+    double *scratch_dev = nullptr;
+    if ((rc = noalloc_scratch(&scratch_dev))) return rc;
     ns3d_ctx *c = new ns3d_ctx();
     c->grid = *grid;
     c->P = *params;
     c->precision = precision;
     c->stream = nullptr;
     Box &b = c->b;
-    b.nx = grid->nx; b.ny = grid->ny; b.nz = grid->nz;
-    b.sx = b.nx + 2 * NG; b.sy = b.ny + 2 * NG; b.sz = b.nz + 2 * NG;
+    b = box_of(*grid);
     c->ncell = (size_t)b.sx * b.sy * b.sz;
     c->nint = (size_t)b.nx * b.ny * b.nz;
     const size_t npart = (size_t)ke_partial_count(b);
@@ -1107,6 +1154,33 @@ int ns3d_compute_timestep_dev(ns3d_ctx *c, double *d_dt) {
     NS3D_CTX(c);
     if (!d_dt) return fail(NS3D_E_ARG, "null device dt");
     return cfl(c, d_dt);
+}
+
+int ns3d_compute_timestep_noalloc(const ns3d_grid *grid, const ns3d_params *params, const double *d_rho,
+                                  const double *d_rhou, const double *d_E, double *d_dt, void *stream) {
+    if (!grid || !params) return fail(NS3D_E_ARG, "ns3d_compute_timestep_noalloc: null grid or params");
+    if (!d_rho || !d_rhou || !d_E || !d_dt)
+        return fail(NS3D_E_ARG, "ns3d_compute_timestep_noalloc: null device pointer");
+    int rc = check_box(grid);
+    if (rc) return rc;
+    if (!(params->gamma > 1.0)) return fail(NS3D_E_ARG, "gamma must exceed 1 (got %g)", params->gamma);
+    CflArgs a{};
+    a.U[0] = d_rho;
+    a.U[1] = d_rhou;
+    a.U[4] = d_E;
+    a.dt_out = d_dt;
+    a.b = box_of(*grid);
+    a.gamma = params->gamma;
+    a.dx = grid->dx;
+    a.cfl = params->cfl;
+    const long blocks = cfl_block_count(a.b);
+    if (blocks > NS3D_NOALLOC_SCRATCH_DOUBLES)
+        return fail(NS3D_E_ARG, "ns3d_compute_timestep_noalloc: %ld block maxima exceed the %d-slot scratch", blocks,
+                    (int)NS3D_NOALLOC_SCRATCH_DOUBLES);
+    if ((rc = noalloc_scratch(&a.partial))) return rc;
+    launch_cfl(a, (hipStream_t)stream);
+    NS3D_LAUNCHED("cfl (no allocation)");
+    return NS3D_OK;
 }
 
 int ns3d_time_advance(ns3d_ctx *c, int max_steps, double dt_fixed, double total_time, double output_timefreq,

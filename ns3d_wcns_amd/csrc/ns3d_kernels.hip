@@ -520,8 +520,9 @@ void launch_halo(const HaloArgs &h, hipStream_t s) {
     const long m = (long)std::max(std::max(h.b.ny * h.b.nz, h.b.nx * h.b.nz), h.b.nx * h.b.ny) * 2 * NG;
     hipLaunchKernelGGL(k_halo, dim3((unsigned)((m + 255) / 256), 3), dim3(256), 0, s, h);
 }
+long cfl_block_count(const Box &b) { return reduce_grid(((long)b.nx * b.ny * b.nz + 1023) / 1024); }
 void launch_cfl(const CflArgs &a, hipStream_t s) {
-    const unsigned g = reduce_grid(((long)a.b.nx * a.b.ny * a.b.nz + 1023) / 1024);
+    const unsigned g = (unsigned)cfl_block_count(a.b);
     hipLaunchKernelGGL(k_cfl, dim3(g), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, (long)g);
 }
