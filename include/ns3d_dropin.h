@@ -169,6 +169,17 @@ void write_tecplot_field(const Field3D &F, const GridDesc &G, const CartDecomp &
 bool initialize_from_hdf5(Field3D &F, const GridDesc &G, const CartDecomp &C, const SolverParams &P,
                           const std::string &filename);
 
+// Built-in initial conditions of the reference's driver (primitives rho, u, v,
+// w, p; the caller applies apply_boundary and primitiveToConserved, as
+// main.cpp:98-99 does).
+// initialize_riemann_2d (initialization.cpp:516-568): four quadrants about
+// x = y = 0.5, on every cell including the ghost layers; the reference's
+// default run (solver.in, main.cpp:85).
+void initialize_riemann_2d(Field3D &F, const GridDesc &G, const SolverParams &P);
+// initialize_sod_shock_tube (initialization.cpp:570-599): Sod's tube along
+// x, interior cells.
+void initialize_sod_shock_tube(Field3D &F, const GridDesc &G, const SolverParams &P);
+
 // Mirror control (no counterpart in the reference).
 bool ns3d_is_resident(const Field3D &F);
 void ns3d_set_resident(Field3D &F, bool resident);

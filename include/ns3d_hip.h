@@ -19,8 +19,12 @@
  *     is available from ns3d_last_error() (per thread).  The reference
  *     reports errors on std::cerr and returns early (InviscousFlux.cpp:19-31);
  *     here the caller gets the code and the message instead.
- *   - Boundary conditions: periodic on all six sides (the one-rank periodic
- *     case of boundary.cpp:14-54).  Non-periodic BCs are rejected.
+ *   - Boundary conditions (ns3d_params.bc): periodic on all six sides runs
+ *     the fused pencil path; a box with physical faces (Wall, Symmetry,
+ *     Outflow, Inflow; boundary.cpp:58-300, one-sided viscous orders of
+ *     ViscousFlux.cpp:15-50) runs the reference's pass structure in FP64
+ *     (stepwise kernels with reference-layout face arrays).  The mixed
+ *     method and the derivative entry points need a periodic box.
  */
 #ifndef NS3D_HIP_H
 #define NS3D_HIP_H
@@ -52,6 +56,15 @@ extern "C" {
 #define NS3D_RIEMANN_HLLC_P 4
 #define NS3D_RIEMANN_AUSM 5
 
+/* Boundary types, same values as SolverParams::BCType (field_structures.h:159).
+ * An axis is periodic when both of its faces are Periodic (initialization.cpp:
+ * 160-168); a Periodic face opposite a non-periodic one is rejected. */
+#define NS3D_BC_PERIODIC 0
+#define NS3D_BC_WALL 1     /* mirror, normal velocity negated (boundary.cpp:58-134) */
+#define NS3D_BC_SYMMETRY 2 /* same as Wall (boundary.cpp:137-141) */
+#define NS3D_BC_OUTFLOW 3  /* copy of the nearest interior cell (boundary.cpp:144-176) */
+#define NS3D_BC_INFLOW 4   /* rho = 1, u = 1, v = w = 0, p = 1 (boundary.cpp:179-198) */
+
 #define NS3D_PRECISION_FP64 0  /* plain FP64 method */
 #define NS3D_PRECISION_MIXED 1 /* FP32 interpolation/flux/viscous stencils, FP64 state and accumulation */
 
@@ -65,6 +78,7 @@ typedef struct ns3d_params {
     int interpolation;  /* NS3D_INTERP_* */
     int riemann_solver; /* NS3D_RIEMANN_* */
     int char_recon;     /* 0 component-wise, 1 characteristic-wise */
+    int bc[6];          /* NS3D_BC_* of the xmin, xmax, ymin, ymax, zmin, zmax faces */
     int pad_;
 } ns3d_params;
 
@@ -120,12 +134,18 @@ int ns3d_record_conserved_to0(ns3d_ctx *ctx);   /* Field3D::recordConservedTo0 :
 /* ---- boundary (boundary.cpp, field_structures.h:511-1056) ---- */
 /* apply_boundary (boundary.cpp:14-54 -> exchange_halos_physical,
  * field_structures.h:936-994): periodic copy of rho,u,v,w,p into the three
- * ghost layers of each face (tangential range interior only). */
+ * ghost layers of each face (tangential range interior only); on a box with
+ * physical faces the exchange zeroes the face ghosts of a non-periodic axis
+ * (its MPI_PROC_NULL receive) and each physical face, xmin to zmax, is then
+ * filled over its full planes (apply_wall_bc / apply_symmetry_bc /
+ * apply_outflow_bc / apply_inflow_bc, boundary.cpp:58-205). */
 int ns3d_apply_boundary(ns3d_ctx *ctx);
-/* apply_boundary_halfnode_flux (boundary.cpp:207-244) and
- * exchange_halos_viscous_flux (field_structures.h:877-932): on one periodic
- * rank the copied values are the periodic images the consuming kernels read
- * by wrapped indexing, so these complete without moving data. */
+/* apply_boundary_halfnode_flux (boundary.cpp:207-300) and
+ * exchange_halos_viscous_flux (field_structures.h:877-932): on a periodic
+ * axis the copied values are the periodic images the consuming kernels read
+ * by wrapped indexing, so the periodic box moves no data.  On a box with
+ * physical faces the ghost faces are set: periodic images, zero on Wall /
+ * Symmetry / Inflow faces, the boundary face on Outflow faces. */
 int ns3d_apply_boundary_halfnode_flux(ns3d_ctx *ctx);
 int ns3d_exchange_halos_viscous_flux(ns3d_ctx *ctx);
 

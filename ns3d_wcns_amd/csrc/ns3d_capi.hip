@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/ns3d_hip.h"
+#include "ns3d_bounded.h"
 #include "ns3d_kernels.h"
 #include "ns3d_observe.h"
 
@@ -67,6 +68,12 @@ struct ns3d_ctx {
     // prims32(): only between the stages of one rk3 / step_stages call of
     // the mixed method (stage 3 stores the FP64 primitives)
     bool prims_fp32;
+    // a face is not periodic: the stepwise path of the bounded box runs
+    // (ns3d_bounded.h), with the reference-layout face arrays fref
+    bool bounded;
+    int per[3];  // axis periodic
+    double *fref_slab;
+    double *fref[3][5];
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;  // (class, (start, stop))
 };
 
@@ -176,7 +183,23 @@ int check_params(const ns3d_params *p) {
     if (p->riemann_solver < NS3D_RIEMANN_ROE || p->riemann_solver > NS3D_RIEMANN_AUSM)
         return fail(NS3D_E_ARG, "Unknown Riemann solver %d", p->riemann_solver);
     if (!(p->gamma > 1.0)) return fail(NS3D_E_ARG, "gamma must exceed 1 (got %g)", p->gamma);
+    static const char *face[6] = {"xmin", "xmax", "ymin", "ymax", "zmin", "zmax"};
+    for (int f = 0; f < 6; ++f)
+        if (p->bc[f] < NS3D_BC_PERIODIC || p->bc[f] > NS3D_BC_INFLOW)
+            return fail(NS3D_E_ARG, "unknown boundary type %d on bc_%s", p->bc[f], face[f]);
+    for (int a = 0; a < 3; ++a)
+        if ((p->bc[2 * a] == NS3D_BC_PERIODIC) != (p->bc[2 * a + 1] == NS3D_BC_PERIODIC))
+            return fail(NS3D_E_UNSUPPORTED,
+                        "bc_%s and bc_%s: a Periodic face opposite a non-periodic one is not supported (the "
+                        "reference leaves that face's ghosts at zero)",
+                        face[2 * a], face[2 * a + 1]);
     return NS3D_OK;
+}
+
+bool any_physical(const ns3d_params &p) {
+    for (int f = 0; f < 6; ++f)
+        if (p.bc[f] != NS3D_BC_PERIODIC) return true;
+    return false;
 }
 
 SchemeCoef<double> coef64(const ns3d_params &P) {
@@ -313,7 +336,12 @@ ConvArgs conv_args(ns3d_ctx *c) {
     return a;
 }
 
+int bnd_boundary(ns3d_ctx *c);
+
 int halo(ns3d_ctx *c, bool all) {
+    // a bounded box keeps its ghost layers consistent after every stage; its
+    // apply_boundary is the exchange plus the physical faces
+    if (c->bounded) return all ? NS3D_OK : bnd_boundary(c);
     HaloArgs h{};
     h.b = c->b;
     if (all) {  // conserved and primitives after a fused stage
@@ -411,8 +439,15 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
 // run their steps without (no kernel of a step reads a ghost layer) and
 // refill once before they return.  The mixed method hands stages 1 and 2's
 // primitives to the next viscous kernel in FP32.
+int record0(ns3d_ctx *c);
+int bnd_rk3(ns3d_ctx *c, const double *d_dt, double dt);
+
 int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool ghosts = true) {
     int rc;
+    if (c->bounded) {
+        if (record && (rc = record0(c))) return rc;
+        return bnd_rk3(c, d_dt, dt);
+    }
     for (int stage = 1; stage <= 3; ++stage)
         if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, ghosts && stage == 3, nullptr, stage < 3))) {
             c->prims_fp32 = false;
@@ -425,6 +460,149 @@ int record0(ns3d_ctx *c) {
     for (int n = 0; n < 5; ++n)
         NS3D_HIP(hipMemcpyAsync(recorded(c)[n], conserved(c)[n], c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
                                 c->stream));
+    return NS3D_OK;
+}
+
+// ------------------------------------------------------------ bounded box
+// The reference's pass structure on a box with physical faces
+// (range_kutta.cpp:11-92, boundary.cpp, ViscousFlux.cpp:15-255).
+BndArgs bnd_args(ns3d_ctx *c) {
+    BndArgs a{};
+    a.P[0] = c->cell[NS3D_RHO];
+    for (int n = 0; n < 4; ++n) a.P[1 + n] = prims(c)[n];  // u, v, w, p
+    a.b = c->b;
+    for (int d = 0; d < 3; ++d) a.per[d] = c->per[d];
+    return a;
+}
+
+// apply_boundary (boundary.cpp:14-54): the face-ghost exchange of rho, u, v,
+// w, p, then each physical face, xmin to zmax, over its full planes.
+int bnd_boundary(ns3d_ctx *c) {
+    BndArgs a = bnd_args(c);
+    launch_bnd_exchange(a, c->stream);
+    for (int f = 0; f < 6; ++f) {
+        if (c->per[f / 2] || c->P.bc[f] == NS3D_BC_PERIODIC) continue;
+        a.face = f;
+        a.type = c->P.bc[f];
+        launch_bnd_fill(a, c->stream);
+    }
+    NS3D_LAUNCHED("boundary");
+    return NS3D_OK;
+}
+
+// compute_invis_flux (InviscousFlux.cpp:10-144): faces -1/2 .. n-1/2 of every
+// interior line, reference face layout.
+int bnd_faces(ns3d_ctx *c) {
+    FaceArgs<double> a{};
+    for (int n = 0; n < 5; ++n) a.U[n] = conserved(c)[n];
+    for (int n = 0; n < 4; ++n) a.Q[n] = prims(c)[n];
+    a.b = c->b;
+    a.c = coef64(c->P);
+    a.ref = MixedRef{0.0, 0.0};
+    const int interp = c->P.interpolation;
+    const bool chr = c->P.char_recon != 0;
+    for (int dir = 0; dir < 3; ++dir) {
+        for (int n = 0; n < 5; ++n) a.F[n] = c->fref[dir][n];
+        switch (c->P.riemann_solver) {
+            case R_ROE: launch_face_ref<R_ROE>(dir, interp, chr, a, c->stream); break;
+            case R_RUSANOV: launch_face_ref<R_RUSANOV>(dir, interp, chr, a, c->stream); break;
+            case R_HLLC: launch_face_ref<R_HLLC>(dir, interp, chr, a, c->stream); break;
+            case R_HLL: launch_face_ref<R_HLL>(dir, interp, chr, a, c->stream); break;
+            case R_HLLC_P: launch_face_ref<R_HLLC_P>(dir, interp, chr, a, c->stream); break;
+            default: launch_face_ref<R_AUSM>(dir, interp, chr, a, c->stream); break;
+        }
+    }
+    NS3D_LAUNCHED("face flux (bounded box)");
+    return NS3D_OK;
+}
+
+// apply_boundary_halfnode_flux (boundary.cpp:207-300): the ghost faces.
+int bnd_ghost_faces(ns3d_ctx *c) {
+    for (int dir = 0; dir < 3; ++dir) {
+        GhostFaceArgs g{};
+        for (int n = 0; n < 5; ++n) g.F[n] = c->fref[dir][n];
+        g.b = c->b;
+        g.dir = dir;
+        g.per = c->per[dir];
+        g.type_lo = c->P.bc[2 * dir];
+        g.type_hi = c->P.bc[2 * dir + 1];
+        launch_ghost_faces(g, c->stream);
+    }
+    NS3D_LAUNCHED("ghost faces");
+    return NS3D_OK;
+}
+
+// compute_gradients / compute_viscous_flux (ViscousFlux.cpp:109-208) with the
+// near-face orders.
+int bnd_visc(ns3d_ctx *c, bool grads, bool flux) {
+    BndViscArgs a{};
+    const double *q[4] = {c->cell[NS3D_U], c->cell[NS3D_V], c->cell[NS3D_W], c->cell[NS3D_T]};
+    for (int n = 0; n < 4; ++n) a.Q[n] = q[n];
+    for (int n = 0; n < 12; ++n) a.G[n] = grads ? c->grad[n] : nullptr;
+    for (int n = 0; n < NVISC; ++n) a.V[n] = flux ? c->visc[n] : nullptr;
+    a.b = c->b;
+    for (int d = 0; d < 3; ++d) a.per[d] = c->per[d];
+    a.h[0] = c->grid.dx;
+    a.h[1] = c->grid.dy;
+    a.h[2] = c->grid.dz;
+    a.mu = c->P.mu;
+    a.S_ref = c->P.S_ref;
+    a.cp_pr = c->P.Cp / c->P.Pr;
+    launch_bnd_visc(a, c->stream);
+    NS3D_LAUNCHED("viscous (bounded box)");
+    if (flux) c->visc_fp32 = false;
+    return NS3D_OK;
+}
+
+// compute_invis_dflux / compute_vis_flux (InviscousFlux.cpp:1027-1056,
+// ViscousFlux.cpp:210-255).
+int bnd_assemble(ns3d_ctx *c, int mode) {
+    BndRhsArgs a{};
+    for (int d = 0; d < 3; ++d)
+        for (int n = 0; n < 5; ++n) a.F[d][n] = c->fref[d][n];
+    for (int n = 0; n < NVISC; ++n) a.V[n] = c->visc[n];
+    for (int n = 0; n < 5; ++n) a.RHS[n] = rhs(c)[n];
+    a.b = c->b;
+    for (int d = 0; d < 3; ++d) a.per[d] = c->per[d];
+    a.h[0] = c->grid.dx;
+    a.h[1] = c->grid.dy;
+    a.h[2] = c->grid.dz;
+    a.mode = mode;
+    launch_bnd_rhs(a, c->stream);
+    NS3D_LAUNCHED("rhs (bounded box)");
+    return NS3D_OK;
+}
+
+// compute_rhs (range_kutta.cpp:11-43) of the interior cells.
+int bnd_rhs(ns3d_ctx *c) {
+    int rc;
+    if ((rc = bnd_faces(c)) || (rc = bnd_ghost_faces(c)) || (rc = bnd_visc(c, false, true))) return rc;
+    return bnd_assemble(c, ASM_INVISCID | ASM_VISCOUS);
+}
+
+// runge_kutta_3 (range_kutta.cpp:46-92): each stage on every cell, then
+// conservedToPrimitive -> apply_boundary -> primitiveToConserved.
+int bnd_rk3(ns3d_ctx *c, const double *d_dt, double dt) {
+    int rc;
+    for (int stage = 1; stage <= 3; ++stage) {
+        if ((rc = bnd_rhs(c))) return rc;
+        RkAllArgs r{};
+        for (int n = 0; n < 5; ++n) {
+            r.U[n] = conserved(c)[n];
+            r.U0[n] = recorded(c)[n];
+            r.R[n] = rhs(c)[n];
+        }
+        r.b = c->b;
+        r.dt = d_dt;
+        r.dtv = dt;
+        r.stage = stage;
+        launch_rk_all(r, c->stream);
+        launch_c2p(conv_args(c), c->stream);
+        NS3D_LAUNCHED("stage (bounded box)");
+        if ((rc = bnd_boundary(c))) return rc;
+        launch_p2c(conv_args(c), c->stream);
+        NS3D_LAUNCHED("primitiveToConserved");
+    }
     return NS3D_OK;
 }
 
@@ -454,19 +632,23 @@ void scatter_cells(const ns3d_ctx *c, const std::vector<double> &compact, double
         for (int j = 0; j < b.ny; ++j)
             for (int i = 0; i < b.nx; ++i) host[b.gidx(i, j, k)] = at(i, j, k);
     if (!face_ghosts) return;
+    // a non-periodic axis keeps the zeros of the reference's exchange
     auto w = [](int i, int n) { return ((i % n) + n) % n; };
+    if (c->per[0])
     for (int k = 0; k < b.nz; ++k)
         for (int j = 0; j < b.ny; ++j)
             for (int g = 1; g <= NG; ++g) {
                 host[b.gidx(-g, j, k)] = at(w(-g, b.nx), j, k);
                 host[b.gidx(b.nx - 1 + g, j, k)] = at(w(b.nx - 1 + g, b.nx), j, k);
             }
+    if (c->per[1])
     for (int k = 0; k < b.nz; ++k)
         for (int i = 0; i < b.nx; ++i)
             for (int g = 1; g <= NG; ++g) {
                 host[b.gidx(i, -g, k)] = at(i, w(-g, b.ny), k);
                 host[b.gidx(i, b.ny - 1 + g, k)] = at(i, w(b.ny - 1 + g, b.ny), k);
             }
+    if (c->per[2])
     for (int j = 0; j < b.ny; ++j)
         for (int i = 0; i < b.nx; ++i)
             for (int g = 1; g <= NG; ++g) {
@@ -543,7 +725,12 @@ int prims_from(ns3d_ctx *c, const double *const *U) {
 int refresh_state(ns3d_ctx *c) {
     launch_c2p(conv_args(c), c->stream);
     NS3D_LAUNCHED("c2p");
-    return halo(c, true);
+    if (!c->bounded) return halo(c, true);
+    int rc = bnd_boundary(c);
+    if (rc) return rc;
+    launch_p2c(conv_args(c), c->stream);
+    NS3D_LAUNCHED("primitiveToConserved");
+    return NS3D_OK;
 }
 
 int fill_ghosts(ns3d_ctx *c, double *const *f, int nf) {
@@ -748,9 +935,16 @@ int ns3d_get_state(ns3d_ctx *c, double *const *U) {
     return copy_fields(c, U, conserved(c), 5);
 }
 
+// The derivative and stored-stage entry points run on periodic boxes only.
+int check_periodic_box(ns3d_ctx *c, const char *what) {
+    if (c->bounded) return fail(NS3D_E_UNSUPPORTED, "%s: periodic boxes only (this box has physical faces)", what);
+    return NS3D_OK;
+}
+
 int ns3d_tangent_rhs(ns3d_ctx *c, const double *const *dU, double *const *out) {
     NS3D_CTX(c);
     int rc;
+    if ((rc = check_periodic_box(c, "tangent_rhs"))) return rc;
     if ((rc = check_ptrs((const void *const *)dU, 5)) || (rc = check_ptrs((const void *const *)out, 5))) return rc;
     if ((rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = load_input(c, dU))) return rc;
     if ((rc = zero_fields(c, out, 5))) return rc;  // zero ghosts
@@ -760,6 +954,7 @@ int ns3d_tangent_rhs(ns3d_ctx *c, const double *const *dU, double *const *out) {
 int ns3d_adjoint_rhs(ns3d_ctx *c, const double *const *lam, double *const *out) {
     NS3D_CTX(c);
     int rc;
+    if ((rc = check_periodic_box(c, "adjoint_rhs"))) return rc;
     if ((rc = check_ptrs((const void *const *)lam, 5)) || (rc = check_ptrs((const void *const *)out, 5))) return rc;
     if ((rc = check_adjoint(c)) || (rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = load_input(c, lam)))
         return rc;
@@ -770,6 +965,7 @@ int ns3d_adjoint_rhs(ns3d_ctx *c, const double *const *lam, double *const *out) 
 int ns3d_tangent_step(ns3d_ctx *c, double dt, double *const *dU) {
     NS3D_CTX(c);
     int rc;
+    if ((rc = check_periodic_box(c, "tangent_step"))) return rc;
     if ((rc = check_ptrs((const void *const *)dU, 5))) return rc;
     if ((rc = ensure_adj(c)) || (rc = refresh_state(c)) || (rc = record0(c))) return rc;
     if ((rc = copy_fields(c, c->a0, dU, 5))) return rc;  // delta U^n
@@ -788,6 +984,7 @@ int ns3d_tangent_step(ns3d_ctx *c, double dt, double *const *dU) {
 int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
     NS3D_CTX(c);
     int rc;
+    if ((rc = check_periodic_box(c, "adjoint_step"))) return rc;
     if ((rc = check_ptrs((const void *const *)adj, 5))) return rc;
     if ((rc = check_adjoint(c)) || (rc = ensure_adj(c))) return rc;
     launch_c2p(conv_args(c), c->stream);  // the primitives of U^n
@@ -805,6 +1002,7 @@ int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
 int ns3d_step_stages(ns3d_ctx *c, double dt, double *const *U1, double *const *U2) {
     NS3D_CTX(c);
     int rc;
+    if ((rc = check_periodic_box(c, "step_stages"))) return rc;
     if ((rc = check_ptrs((const void *const *)U1, 5)) || (rc = check_ptrs((const void *const *)U2, 5))) return rc;
     // the stages written straight into the caller's arrays (interior cells),
     // U0 recorded by the stage-1 x pencil
@@ -821,6 +1019,7 @@ int ns3d_adjoint_step_stages(ns3d_ctx *c, double dt, double *const *adj, const d
                              const double *const *U2) {
     NS3D_CTX(c);
     int rc;
+    if ((rc = check_periodic_box(c, "adjoint_step_stages"))) return rc;
     if ((rc = check_ptrs((const void *const *)adj, 5)) || (rc = check_ptrs((const void *const *)U1, 5)) ||
         (rc = check_ptrs((const void *const *)U2, 5)))
         return rc;
@@ -838,6 +1037,8 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     if ((rc = check_box(grid))) return rc;
     if (precision != NS3D_PRECISION_FP64 && precision != NS3D_PRECISION_MIXED)
         return fail(NS3D_E_ARG, "unknown precision %d", precision);
+    if (any_physical(*params) && precision != NS3D_PRECISION_FP64)
+        return fail(NS3D_E_UNSUPPORTED, "the mixed method runs on periodic boxes only");
     // set-up of the allocation-free entry point on the current device
     double *scratch_dev = nullptr;
     if ((rc = noalloc_scratch(&scratch_dev))) return rc;
@@ -879,6 +1080,23 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     c->timing = false;
     c->visc_fp32 = false;
     c->prims_fp32 = false;
+    c->bounded = any_physical(*params);
+    for (int a = 0; a < 3; ++a) c->per[a] = params->bc[2 * a] == NS3D_BC_PERIODIC;
+    c->fref_slab = nullptr;
+    if (c->bounded) {
+        // reference-layout face arrays, 5 per direction, each smaller than a cell field
+        const size_t n = 15 * cstride;
+        if (hipMalloc(&c->fref_slab, n * sizeof(double)) != hipSuccess ||
+            hipMemset(c->fref_slab, 0, n * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (c->fref_slab) (void)hipFree(c->fref_slab);
+            (void)hipFree(c->slab);
+            delete c;
+            return fail(NS3D_E_NOMEM, "face arrays of the bounded box: out of device memory for %zu doubles", n);
+        }
+        for (int d = 0; d < 3; ++d)
+            for (int v = 0; v < 5; ++v) c->fref[d][v] = c->fref_slab + (5 * d + v) * cstride;
+    }
     if (hipMemset(c->slab, 0, total * sizeof(double)) != hipSuccess) {
         (void)hipFree(c->slab);
         delete c;
@@ -932,6 +1150,7 @@ int ns3d_destroy(ns3d_ctx *c) {
     if (c->adj_slab) (void)hipFree(c->adj_slab);
     if (c->obs_work) (void)hipFree(c->obs_work);
     destroy_fft_plan(c->fft_plan);
+    if (c->fref_slab) (void)hipFree(c->fref_slab);
     (void)hipFree(c->slab);
     delete c;
     return NS3D_OK;
@@ -947,6 +1166,9 @@ int ns3d_set_params(ns3d_ctx *c, const ns3d_params *params) {
     NS3D_CTX(c);
     int rc = check_params(params);
     if (rc) return rc;
+    for (int f = 0; f < 6; ++f)
+        if (params->bc[f] != c->P.bc[f])
+            return fail(NS3D_E_ARG, "boundary types are fixed when the context is created");
     c->P = *params;
     return NS3D_OK;
 }
@@ -1009,7 +1231,7 @@ int ns3d_apply_boundary(ns3d_ctx *c) {
 
 int ns3d_apply_boundary_halfnode_flux(ns3d_ctx *c) {
     NS3D_CTX(c);
-    return NS3D_OK;
+    return c->bounded ? bnd_ghost_faces(c) : NS3D_OK;
 }
 
 int ns3d_exchange_halos_viscous_flux(ns3d_ctx *c) {
@@ -1019,11 +1241,12 @@ int ns3d_exchange_halos_viscous_flux(ns3d_ctx *c) {
 
 int ns3d_compute_invis_flux(ns3d_ctx *c) {
     NS3D_CTX(c);
-    return run_faces(c);
+    return c->bounded ? bnd_faces(c) : run_faces(c);
 }
 
 int ns3d_compute_invis_dflux(ns3d_ctx *c) {
     NS3D_CTX(c);
+    if (c->bounded) return bnd_assemble(c, ASM_INVISCID | ASM_ACCUMULATE);
     launch_assemble(asm_args(c, 0, ASM_INVISCID | ASM_ACCUMULATE, nullptr, 0.0), c->stream);
     NS3D_LAUNCHED("invis_dflux");
     return NS3D_OK;
@@ -1039,6 +1262,12 @@ size_t ns3d_face_count(const ns3d_ctx *c, int dir) {
 int ns3d_download_face_flux(ns3d_ctx *c, int dir, int comp, double *host) {
     NS3D_CTX(c);
     if (dir < 0 || dir > 2 || comp < 0 || comp > 4 || !host) return fail(NS3D_E_ARG, "bad face/component");
+    if (c->bounded) {  // already in the reference's face layout
+        NS3D_HIP(hipMemcpyAsync(host, c->fref[dir][comp], ns3d_face_count(c, dir) * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+        NS3D_HIP(hipStreamSynchronize(c->stream));
+        return NS3D_OK;
+    }
     std::vector<double> f;
     int rc = fetch_compact(c, c->face[dir][comp], f);
     if (rc) return rc;
@@ -1068,6 +1297,7 @@ int ns3d_compute_gradients(ns3d_ctx *c) {
     NS3D_CTX(c);
     int rc = ensure_grad(c);
     if (rc) return rc;
+    if (c->bounded) return bnd_visc(c, true, false);
     launch_visc(visc_args(c, true, false), c->stream);
     NS3D_LAUNCHED("gradients");
     return NS3D_OK;
@@ -1077,6 +1307,7 @@ int ns3d_compute_viscous_flux(ns3d_ctx *c) {
     NS3D_CTX(c);
     int rc = ensure_grad(c);
     if (rc) return rc;
+    if (c->bounded) return bnd_visc(c, true, true);
     launch_visc_from_grad(visc_args(c, true, true), c->stream);
     NS3D_LAUNCHED("viscous flux");
     c->visc_fp32 = false;
@@ -1088,6 +1319,7 @@ int ns3d_compute_vis_flux(ns3d_ctx *c) {
     if (c->visc_fp32)
         return fail(NS3D_E_ARG, "the viscous fluxes were last written by a mixed-precision stage: call "
                                 "compute_viscous_flux first");
+    if (c->bounded) return bnd_assemble(c, ASM_VISCOUS | ASM_ACCUMULATE);
     launch_assemble(asm_args(c, 0, ASM_VISCOUS | ASM_ACCUMULATE, nullptr, 0.0), c->stream);
     NS3D_LAUNCHED("vis_flux");
     return NS3D_OK;
@@ -1126,7 +1358,7 @@ int ns3d_compute_rhs(ns3d_ctx *c) {
     NS3D_CTX(c);
     // the RHS of ghost cells is zero, as after the reference's std::fill
     for (int n = 0; n < 5; ++n) NS3D_HIP(hipMemsetAsync(rhs(c)[n], 0, c->ncell * sizeof(double), c->stream));
-    return fused_stage(c, 0, nullptr, 0.0);
+    return c->bounded ? bnd_rhs(c) : fused_stage(c, 0, nullptr, 0.0);
 }
 
 int ns3d_runge_kutta_3(ns3d_ctx *c, double dt) {

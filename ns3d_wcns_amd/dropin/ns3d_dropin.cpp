@@ -22,6 +22,7 @@ struct ns3d_mirror {
     double dx = 0, dy = 0, dz = 0;
     bool grid_known = false;
     bool resident = false;
+    int bc[6] = {0, 0, 0, 0, 0, 0};  // boundary types of the context
 };
 
 namespace {
@@ -49,27 +50,22 @@ ns3d_params to_c(const SolverParams &P) {
     c.interpolation = static_cast<int>(P.interpolation);
     c.riemann_solver = static_cast<int>(P.riemann_solver);
     c.char_recon = P.char_recon ? 1 : 0;
+    const SolverParams::BCType bc[6] = {P.bc_xmin, P.bc_xmax, P.bc_ymin, P.bc_ymax, P.bc_zmin, P.bc_zmax};
+    for (int f = 0; f < 6; ++f) c.bc[f] = static_cast<int>(bc[f]);
     return c;
 }
 
-// The field's mirror with the given spacing (nullptr: keep the current one
-// or, for a new mirror, use unit spacing until a call supplies it).
-bool all_periodic(const SolverParams &P) {
-    using BC = SolverParams::BCType;
-    return P.bc_xmin == BC::Periodic && P.bc_xmax == BC::Periodic && P.bc_ymin == BC::Periodic &&
-           P.bc_ymax == BC::Periodic && P.bc_zmin == BC::Periodic && P.bc_zmax == BC::Periodic;
-}
-
+// The field's mirror with the given spacing and boundary types (nullptr:
+// keep the current ones or, for a new mirror, use unit spacing and periodic
+// faces until a call supplies them).
 ns3d_ctx *mirror(Field3D &F, const SolverParams *P, const GridDesc *G) {
-    if (P && !all_periodic(*P)) {
-        // the device path is the one-rank periodic box: refuse rather than
-        // silently treat a Wall / Symmetry / Outflow / Inflow face as periodic
-        std::cerr << "ns3d: only periodic boundaries are supported (bc_xmin .. bc_zmax must all be Periodic)\n";
-        return nullptr;
-    }
     if (!F.dev) F.dev = new ns3d_mirror();
     ns3d_mirror &m = *F.dev;
-    const bool regrid = G && (!m.grid_known || m.dx != G->dx || m.dy != G->dy || m.dz != G->dz);
+    const ns3d_params pc = to_c(P ? *P : SolverParams());
+    bool rebc = false;
+    if (P)
+        for (int f = 0; f < 6; ++f) rebc = rebc || pc.bc[f] != m.bc[f];
+    const bool regrid = (G && (!m.grid_known || m.dx != G->dx || m.dy != G->dy || m.dz != G->dz)) || rebc;
     if (m.ctx && regrid) {
         if (m.resident) {
             std::cerr << "ns3d: a resident field was used with a different grid spacing; its state is reloaded "
@@ -88,8 +84,11 @@ ns3d_ctx *mirror(Field3D &F, const SolverParams *P, const GridDesc *G) {
             m.dx = m.dy = m.dz = 1.0;
         }
         ns3d_grid g{F.L.nx, F.L.ny, F.L.nz, F.L.ngx, m.dx, m.dy, m.dz};
-        const ns3d_params cp = to_c(P ? *P : SolverParams());
+        ns3d_params cp = pc;
+        if (!P)
+            for (int f = 0; f < 6; ++f) cp.bc[f] = m.bc[f];
         if (!ok(ns3d_create(&m.ctx, &g, &cp, NS3D_PRECISION_FP64), "create")) return nullptr;
+        for (int f = 0; f < 6; ++f) m.bc[f] = cp.bc[f];
         if (m.resident) {  // a new context starts from the host state
             m.resident = false;
             ns3d_sync_device(F);
@@ -468,6 +467,48 @@ void compute_total_energy(Field3D &F, const GridDesc &G, CartDecomp &, const Sol
 }
 
 // ------------------------------------------------------------ mirror control
+void initialize_riemann_2d(Field3D &F, const GridDesc &G, const SolverParams &P) {
+    (void)P;
+    const LocalDesc &L = F.L;
+    for (int k = 0; k < L.sz; ++k)
+        for (int j = 0; j < L.sy; ++j) {
+            const double y = (L.oy + j - L.ngy + 0.5) * G.dy;
+            for (int i = 0; i < L.sx; ++i) {
+                const double x = (L.ox + i - L.ngx + 0.5) * G.dx;
+                // quadrant states (initialization.cpp:536-556)
+                const bool right = x >= 0.5, top = y >= 0.5;
+                const double rho = right == top ? 1.1 : 0.5065;
+                const double u = right ? 0.0 : 0.8939;
+                const double v = top ? 0.0 : 0.8939;
+                const double p = right == top ? 1.1 : 0.35;
+                const int id = F.I(i, j, k);
+                F.rho[id] = rho;
+                F.u[id] = u;
+                F.v[id] = v;
+                F.w[id] = 0.0;
+                F.p[id] = p;
+            }
+        }
+}
+
+void initialize_sod_shock_tube(Field3D &F, const GridDesc &G, const SolverParams &P) {
+    (void)P;
+    const LocalDesc &L = F.L;
+    const double x_mid = 0.5 * G.global_nx * G.dx;
+    for (int k = L.ngz; k < L.ngz + L.nz; ++k)
+        for (int j = L.ngy; j < L.ngy + L.ny; ++j)
+            for (int i = L.ngx; i < L.ngx + L.nx; ++i) {
+                const int id = F.I(i, j, k);
+                const double x = (L.ox + i - L.ngx + 0.5) * G.dx;
+                const bool left = x < x_mid;
+                F.rho[id] = left ? 1.0 : 0.125;
+                F.u[id] = 0.0;
+                F.v[id] = 0.0;
+                F.w[id] = 0.0;
+                F.p[id] = left ? 1.0 : 0.1;
+            }
+}
+
 bool ns3d_is_resident(const Field3D &F) { return resident(F); }
 
 void ns3d_set_resident(Field3D &F, bool on) {

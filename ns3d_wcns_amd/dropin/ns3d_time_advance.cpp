@@ -6,12 +6,14 @@
 // TotalTime and the output times, stop on dt <= 1e-12; runge_kutta_3 per
 // step; the kinetic-energy monitor; field.h5 at the output times).
 //
-//   ns3d_time_advance solver.in initial.bin|- final.bin [resident] [--write-fields]
+//   ns3d_time_advance solver.in initial.bin|riemann2d|sod|- final.bin [resident] [--write-fields]
 //   ns3d_time_advance solver.in --print-params    (parsed and derived values)
 //   ns3d_time_advance solver.in --convert initial.bin|in.h5 out.h5 [time]
 //
 // initial.bin: int32 nx, ny, nz, then rho, u, v, w, p over the interior
-// (nz, ny, nx), doubles; "-" with restart = true.  final.bin: int32 steps,
+// (nz, ny, nx), doubles; "riemann2d" / "sod": the reference's built-in
+// initialize_riemann_2d (its default run, main.cpp:85) / initialize_sod_shock_tube;
+// "-" with restart = true.  Every boundary type of solver.in is honoured.  final.bin: int32 steps,
 // the dt of each step, the final conserved fields rho, rho u, rho v, rho w, E
 // (ghosted, sz*sy*sx each) and the monitored kinetic energy after every step.
 // --write-fields writes output/time_<int(t * 1e5)>/field.h5 in the working
@@ -110,7 +112,7 @@ bool ends_with(const std::string &s, const std::string &x) {
 int main(int argc, char **argv) {
     const bool print_only = argc == 3 && std::strcmp(argv[2], "--print-params") == 0;
     if (argc < 4 && !print_only) {
-        std::cerr << "usage: " << argv[0] << " solver.in initial.bin|- final.bin [resident] [--write-fields]\n"
+        std::cerr << "usage: " << argv[0] << " solver.in initial.bin|riemann2d|sod|- final.bin [resident] [--write-fields]\n"
                   << "       " << argv[0] << " solver.in --print-params\n"
                   << "       " << argv[0] << " solver.in --convert initial.bin|in.h5 out.h5 [time]\n";
         return 2;
@@ -137,10 +139,6 @@ int main(int argc, char **argv) {
     }
     Field3D F;
     F.allocate(make_local_desc(G, P));
-    if (!(C.periods[0] && C.periods[1] && C.periods[2]) && std::strcmp(argv[2], "--convert") != 0) {
-        std::cerr << "ns3d: only periodic boundaries are supported (bc_xmin .. bc_zmax must all be Periodic)\n";
-        return 1;
-    }
     if (std::strcmp(argv[2], "--convert") == 0) {
         // host only: no hot-path function is called
         if (argc < 5) {
@@ -171,6 +169,10 @@ int main(int argc, char **argv) {
             std::cerr << "Restart initialization from HDF5 failed, file='" << P.restart_file << "'\n";
             return 1;
         }
+    } else if (std::strcmp(argv[2], "riemann2d") == 0) {
+        initialize_riemann_2d(F, G, P);
+    } else if (std::strcmp(argv[2], "sod") == 0) {
+        initialize_sod_shock_tube(F, G, P);
     } else if (!read_initial(argv[2], G, F)) {
         return 1;
     }

@@ -74,6 +74,7 @@ class CParams(ctypes.Structure):
         ("interpolation", ctypes.c_int),
         ("riemann_solver", ctypes.c_int),
         ("char_recon", ctypes.c_int),
+        ("bc", ctypes.c_int * 6),
         ("pad_", ctypes.c_int),
     ]
 
@@ -141,16 +142,20 @@ class SolverParams:
         return self
 
     def to_c(self) -> CParams:
-        """The C-ABI parameter block.  The device path is the one-rank
-        periodic box: any other boundary type is refused here."""
-        if any(b != BCType.Periodic for b in self.bc):
-            raise ValueError("only periodic boundaries are supported on the device path "
-                             f"(got {[b.name for b in self.bc]})")
+        """The C-ABI parameter block, boundary types included (an axis with
+        one Periodic and one other face is refused by the library)."""
+        if len(self.bc) != 6:
+            raise ValueError(f"six boundary types expected (got {len(self.bc)})")
         return CParams(
             self.gamma, self.Pr, self.Ma, self.Re, self.cfl, self.mu, self.S_ref, self.Cv, self.Cp,
             self.Rgas, self.mdcd_diss, self.mdcd_disp, int(self.interpolation), int(self.riemann_solver),
-            int(bool(self.char_recon)), 0,
+            int(bool(self.char_recon)), (ctypes.c_int * 6)(*[int(b) for b in self.bc]), 0,
         )
+
+    @property
+    def periodic(self) -> bool:
+        """True when all six faces are periodic."""
+        return all(b == BCType.Periodic for b in self.bc)
 
 
 @dataclass

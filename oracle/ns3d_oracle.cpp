@@ -49,6 +49,7 @@ struct orc_params {
     int interpolation;   // 0 WENO5(=WCNS5 weights) 2 MDCD_LINEAR 3 MDCD_HYBRID 4 WENO5-JS
     int riemann_solver;  // 0 Roe 1 Rusanov 2 HLLC 3 HLL 4 HLLC_p 5 AUSM
     int char_recon;
+    int bc[6];  // SolverParams::BCType of xmin..zmax: 0 Periodic 1 Wall 2 Symmetry 3 Outflow 4 Inflow
     int pad_;
 };
 
@@ -539,6 +540,10 @@ struct Ctx {
     int nx, ny, nz, ng, sx, sy, sz;
     double dx, dy, dz;
     orc_params P;
+    // an axis is periodic when both faces are (initialization.cpp:160-168);
+    // on one rank its neighbours are then the rank itself, otherwise
+    // MPI_PROC_NULL (field_structures.h:1104-1106)
+    bool per[3];
     // conserved, U0, primitives, RHS (field_structures.h:202-220)
     std::vector<double> rho, rhou, rhov, rhow, E, rho0, rhou0, rhov0, rhow0, E0;
     std::vector<double> u, v, w, p, T, rhs[5];
@@ -591,42 +596,49 @@ void primitive_to_conserved(Ctx &F) {
     }
 }
 
-// Periodic one-rank image of a ghost layer: cells [0,ng) take i+n, cells
-// [ng+n, ng+2ng) take i-n (field_structures.h:517-649 pack/unpack ranges).
+enum { BC_PERIODIC = 0, BC_WALL = 1, BC_SYMMETRY = 2, BC_OUTFLOW = 3, BC_INFLOW = 4 };
+
+void update_periodic(Ctx &F) {
+    for (int a = 0; a < 3; ++a) F.per[a] = F.P.bc[2 * a] == BC_PERIODIC && F.P.bc[2 * a + 1] == BC_PERIODIC;
+}
+
+// The face-ghost exchange of one rank (field_structures.h:517-649, 936-994 and
+// the other exchanges): along a periodic axis cells [0,ng) take i+n and cells
+// [ng+n, ng+2ng) take i-n; along a non-periodic axis the receive from
+// MPI_PROC_NULL leaves the zero-initialised buffer, which is unpacked, so the
+// ghosts become 0.  Tangential ranges are interior only.  fn(dst, src) with
+// src < 0 for a zero.
 template <class Fn>
 void for_face_ghosts(const Ctx &F, Fn fn) {
     const int ng = F.ng, nx = F.nx, ny = F.ny, nz = F.nz;
-    // x faces: j,k interior (field_structures.h:514-559)
     for (int k = ng; k < ng + nz; ++k)
         for (int j = ng; j < ng + ny; ++j)
             for (int g = 0; g < ng; ++g) {
-                fn(F.I(g, j, k), F.I(g + nx, j, k));
-                fn(F.I(ng + nx + g, j, k), F.I(ng + g, j, k));
+                fn(F.I(g, j, k), F.per[0] ? F.I(g + nx, j, k) : -1);
+                fn(F.I(ng + nx + g, j, k), F.per[0] ? F.I(ng + g, j, k) : -1);
             }
-    // y faces: i,k interior (561-605)
     for (int k = ng; k < ng + nz; ++k)
         for (int g = 0; g < ng; ++g)
             for (int i = ng; i < ng + nx; ++i) {
-                fn(F.I(i, g, k), F.I(i, g + ny, k));
-                fn(F.I(i, ng + ny + g, k), F.I(i, ng + g, k));
+                fn(F.I(i, g, k), F.per[1] ? F.I(i, g + ny, k) : -1);
+                fn(F.I(i, ng + ny + g, k), F.per[1] ? F.I(i, ng + g, k) : -1);
             }
-    // z faces: i,j interior (607-649)
     for (int g = 0; g < ng; ++g)
         for (int j = ng; j < ng + ny; ++j)
             for (int i = ng; i < ng + nx; ++i) {
-                fn(F.I(i, j, g), F.I(i, j, g + nz));
-                fn(F.I(i, j, ng + nz + g), F.I(i, j, ng + g));
+                fn(F.I(i, j, g), F.per[2] ? F.I(i, j, g + nz) : -1);
+                fn(F.I(i, j, ng + nz + g), F.per[2] ? F.I(i, j, ng + g) : -1);
             }
 }
 
 // exchange_halos_physical (field_structures.h:936-994): rho,u,v,w,p only.
 void exchange_physical(Ctx &F) {
     for_face_ghosts(F, [&](int d, int s) {
-        F.rho[d] = F.rho[s];
-        F.u[d] = F.u[s];
-        F.v[d] = F.v[s];
-        F.w[d] = F.w[s];
-        F.p[d] = F.p[s];
+        F.rho[d] = s < 0 ? 0.0 : F.rho[s];
+        F.u[d] = s < 0 ? 0.0 : F.u[s];
+        F.v[d] = s < 0 ? 0.0 : F.v[s];
+        F.w[d] = s < 0 ? 0.0 : F.w[s];
+        F.p[d] = s < 0 ? 0.0 : F.p[s];
     });
 }
 
@@ -634,16 +646,16 @@ void exchange_physical(Ctx &F) {
 void exchange_viscous(Ctx &F) {
     for_face_ghosts(F, [&](int d, int s) {
         for (int n = 1; n < 5; ++n) {
-            F.Fv[n][d] = F.Fv[n][s];
-            F.Hv[n][d] = F.Hv[n][s];
-            F.Gv[n][d] = F.Gv[n][s];
+            F.Fv[n][d] = s < 0 ? 0.0 : F.Fv[n][s];
+            F.Hv[n][d] = s < 0 ? 0.0 : F.Hv[n][s];
+            F.Gv[n][d] = s < 0 ? 0.0 : F.Gv[n][s];
         }
     });
 }
 
 // exchange_halos_halfnode_flux (field_structures.h:773-874, 998-1056): the
-// ng-1 missing ghost faces per side.  Left ghost faces [0,ng-1) take face
-// f+n; right ghost faces [ng+n, ng+n+ng-1) take face f-n.
+// ng-1 missing ghost faces per side.  Periodic axis: left ghost faces [0,ng-1)
+// take face f+n, right ghost faces [ng+n, ng+n+ng-1) take face f-n; otherwise 0.
 void exchange_halfnode(Ctx &F) {
     const int ng = F.ng, nx = F.nx, ny = F.ny, nz = F.nz;
     for (int v = 0; v < 5; ++v) {
@@ -651,26 +663,106 @@ void exchange_halfnode(Ctx &F) {
         for (int k = ng; k < ng + nz; ++k)
             for (int j = ng; j < ng + ny; ++j)
                 for (int g = 0; g < ng - 1; ++g) {
-                    X[F.IFX(g, j, k)] = X[F.IFX(g + nx, j, k)];
-                    X[F.IFX(ng + nx + g, j, k)] = X[F.IFX(ng + g, j, k)];
+                    X[F.IFX(g, j, k)] = F.per[0] ? X[F.IFX(g + nx, j, k)] : 0.0;
+                    X[F.IFX(ng + nx + g, j, k)] = F.per[0] ? X[F.IFX(ng + g, j, k)] : 0.0;
                 }
         for (int k = ng; k < ng + nz; ++k)
             for (int i = ng; i < ng + nx; ++i)
                 for (int g = 0; g < ng - 1; ++g) {
-                    Y[F.IFY(i, g, k)] = Y[F.IFY(i, g + ny, k)];
-                    Y[F.IFY(i, ng + ny + g, k)] = Y[F.IFY(i, ng + g, k)];
+                    Y[F.IFY(i, g, k)] = F.per[1] ? Y[F.IFY(i, g + ny, k)] : 0.0;
+                    Y[F.IFY(i, ng + ny + g, k)] = F.per[1] ? Y[F.IFY(i, ng + g, k)] : 0.0;
                 }
         for (int i = ng; i < ng + nx; ++i)
             for (int j = ng; j < ng + ny; ++j)
                 for (int g = 0; g < ng - 1; ++g) {
-                    Z[F.IFZ(i, j, g)] = Z[F.IFZ(i, j, g + nz)];
-                    Z[F.IFZ(i, j, ng + nz + g)] = Z[F.IFZ(i, j, ng + g)];
+                    Z[F.IFZ(i, j, g)] = F.per[2] ? Z[F.IFZ(i, j, g + nz)] : 0.0;
+                    Z[F.IFZ(i, j, ng + nz + g)] = F.per[2] ? Z[F.IFZ(i, j, ng + g)] : 0.0;
                 }
     }
 }
 
-// apply_boundary (boundary.cpp:14-54) on a periodic rank: the exchange only.
-void apply_boundary(Ctx &F) { exchange_physical(F); }
+// apply_outflow_bc_halfnode_flux (boundary.cpp:246-300): the ghost faces of an
+// outflow face copy its boundary face, over the full tangential planes.
+void outflow_halfnode(Ctx &F, int face) {
+    const int ng = F.ng, sx = F.sx, sy = F.sy, sz = F.sz;
+    for (int v = 0; v < 5; ++v) {
+        std::vector<double> &X = F.fx[v], &Y = F.fy[v], &Z = F.fz[v];
+        if (face == 0)
+            for (int k = 0; k < sz; ++k) for (int j = 0; j < sy; ++j) for (int i = 0; i < ng - 1; ++i)
+                X[F.IFX(i, j, k)] = X[F.IFX(ng - 1, j, k)];
+        if (face == 1)
+            for (int k = 0; k < sz; ++k) for (int j = 0; j < sy; ++j) for (int i = sx - ng; i < sx - 1; ++i)
+                X[F.IFX(i, j, k)] = X[F.IFX(sx - ng - 1, j, k)];
+        if (face == 2)
+            for (int k = 0; k < sz; ++k) for (int j = 0; j < ng - 1; ++j) for (int i = 0; i < sx; ++i)
+                Y[F.IFY(i, j, k)] = Y[F.IFY(i, ng - 1, k)];
+        if (face == 3)
+            for (int k = 0; k < sz; ++k) for (int j = sy - ng; j < sy - 1; ++j) for (int i = 0; i < sx; ++i)
+                Y[F.IFY(i, j, k)] = Y[F.IFY(i, sy - ng - 1, k)];
+        if (face == 4)
+            for (int k = 0; k < ng - 1; ++k) for (int j = 0; j < sy; ++j) for (int i = 0; i < sx; ++i)
+                Z[F.IFZ(i, j, k)] = Z[F.IFZ(i, j, ng - 1)];
+        if (face == 5)
+            for (int k = sz - ng; k < sz - 1; ++k) for (int j = 0; j < sy; ++j) for (int i = 0; i < sx; ++i)
+                Z[F.IFZ(i, j, k)] = Z[F.IFZ(i, j, sz - ng - 1)];
+    }
+}
+
+// apply_boundary_halfnode_flux (boundary.cpp:207-244): the exchange, then the
+// outflow faces of the non-periodic axes (Wall, Symmetry and Inflow keep the
+// exchange's zeros).
+void apply_boundary_halfnode(Ctx &F) {
+    exchange_halfnode(F);
+    for (int face = 0; face < 6; ++face)
+        if (!F.per[face / 2] && F.P.bc[face] == BC_OUTFLOW) outflow_halfnode(F, face);
+}
+
+// The ghost cell of face `face` at layer position (i, j, k) and the cell it
+// takes its value from: mirror about the face (Wall/Symmetry,
+// boundary.cpp:58-141) or the nearest interior cell (Outflow, 144-176).
+void bc_face(Ctx &F, int face) {
+    const int ng = F.ng, sx = F.sx, sy = F.sy, sz = F.sz;
+    const int type = F.P.bc[face], axis = face / 2;
+    const bool hi = face & 1;
+    const int s[3] = {sx, sy, sz};
+    std::vector<double> *vel[3] = {&F.u, &F.v, &F.w};
+    for (int k = 0; k < sz; ++k)
+        for (int j = 0; j < sy; ++j)
+            for (int i = 0; i < sx; ++i) {
+                int c[3] = {i, j, k};
+                const int g = c[axis];
+                if (hi ? g < s[axis] - ng : g >= ng) continue;
+                const int id = F.I(i, j, k);
+                if (type == BC_INFLOW) {  // boundary.cpp:179-198 (its E is overwritten by the P2C that follows)
+                    F.rho[id] = 1.0;
+                    F.u[id] = 1.0;
+                    F.v[id] = 0.0;
+                    F.w[id] = 0.0;
+                    F.p[id] = 1.0;
+                    continue;
+                }
+                int m[3] = {i, j, k};
+                if (type == BC_OUTFLOW) m[axis] = hi ? s[axis] - ng - 1 : ng;
+                else m[axis] = hi ? 2 * (s[axis] - ng) - 1 - g : 2 * ng - 1 - g;
+                const int src = F.I(m[0], m[1], m[2]);
+                F.rho[id] = F.rho[src];
+                F.u[id] = F.u[src];
+                F.v[id] = F.v[src];
+                F.w[id] = F.w[src];
+                F.p[id] = F.p[src];
+                if (type == BC_WALL || type == BC_SYMMETRY) (*vel[axis])[id] = -(*vel[axis])[src];
+            }
+}
+
+// apply_boundary (boundary.cpp:14-54): the exchange, then the physical
+// boundary of every face of a non-periodic axis, xmin to zmax, each over its
+// full face planes (edges and corners included).  A Periodic face of a
+// non-periodic axis keeps the exchange's zeros (boundary.cpp:48-50).
+void apply_boundary(Ctx &F) {
+    exchange_physical(F);
+    for (int face = 0; face < 6; ++face)
+        if (!F.per[face / 2] && F.P.bc[face] != BC_PERIODIC) bc_face(F, face);
+}
 
 // compute_invis_flux (InviscousFlux.cpp:10-144): faces i in [ng-1, ng+n-1],
 // tangential indices interior, stencil cells i-2..i+3 (mid = (6-1)/2).
@@ -729,17 +821,40 @@ void compute_invis_dflux(Ctx &F) {
             }
 }
 
-// diff_6th_central (ViscousFlux.cpp:27-29) around index 3 of a 7-point line
-inline double d6(const double *f, double h) {
+// The derivative stencils of ViscousFlux.cpp:15-29 around index 3 of a 7-point
+// line, by the order choose_scheme picks (ViscousFlux.cpp:33-50): 6 central
+// 6th order, 4 and 2 central 4th/2nd order, +1 / -1 the 2nd-order one-sided
+// difference forward / backward.
+inline double dstencil(const double *f, double h, int order) {
+    if (order == 1 || order == -1) return (-3.0 * f[3] + 4.0 * f[3 + order] - f[3 + 2 * order]) / (2.0 * h);
+    if (order == 2) return (f[4] - f[2]) / (2.0 * h);
+    if (order == 4) return (-f[5] + 8.0 * f[4] - 8.0 * f[2] + f[1]) / (12.0 * h);
     return (f[6] - 9.0 * f[5] + 45.0 * f[4] - 45.0 * f[2] + 9.0 * f[1] - f[0]) / (60.0 * h);
 }
-inline double dline(const std::vector<double> &a, int id, int stride, double h) {
+// choose_scheme (ViscousFlux.cpp:33-50) for ghosted index idx of an axis with
+// interior [nstart, nend)
+inline int choose_scheme(int idx, int nstart, int nend, bool periodic) {
+    if (periodic) return 6;
+    int dl = idx - nstart;
+    if (dl == 0) return 1;
+    int dr = (nend - 1) - idx;
+    if (dr == 0) return -1;
+    int dmin = std::min(dl, dr);
+    return dmin == 1 ? 2 : (dmin == 2 ? 4 : 6);
+}
+inline double dline(const std::vector<double> &a, int id, int stride, double h, int order) {
     double f[7];
-    for (int m = 0; m < 7; ++m) f[m] = a[id + (m - 3) * stride];
-    return d6(f, h);
+    for (int m = 0; m < 7; ++m) {  // only the points the stencil uses are read
+        const int o = m - 3;
+        const bool used = order == 6 || (order == 4 && std::abs(o) <= 2) || (order == 2 && std::abs(o) == 1) ||
+                          (order == 1 && o >= 0 && o <= 2) || (order == -1 && o <= 0 && o >= -2);
+        f[m] = used ? a[id + o * stride] : 0.0;
+    }
+    return dstencil(f, h, order);
 }
 
-// compute_gradients (ViscousFlux.cpp:109-148); periodic => 6th order everywhere.
+// compute_gradients (ViscousFlux.cpp:109-148): 6th order along periodic axes,
+// the orders of choose_scheme near the faces of a non-periodic one.
 void compute_gradients(Ctx &F) {
     const int ng = F.ng, sx = F.sx, sxy = F.sx * F.sy;
     const std::vector<double> *src[4] = {&F.u, &F.v, &F.w, &F.T};
@@ -748,10 +863,13 @@ void compute_gradients(Ctx &F) {
         for (int j = ng; j < ng + F.ny; ++j)
             for (int i = ng; i < ng + F.nx; ++i) {
                 int id = F.I(i, j, k);
+                const int ox = choose_scheme(i, ng, ng + F.nx, F.per[0]);
+                const int oy = choose_scheme(j, ng, ng + F.ny, F.per[1]);
+                const int oz = choose_scheme(k, ng, ng + F.nz, F.per[2]);
                 for (int q = 0; q < 4; ++q) {
-                    F.grad[3 * q + 0][id] = dline(*src[q], id, 1, F.dx);
-                    F.grad[3 * q + 1][id] = dline(*src[q], id, sx, F.dy);
-                    F.grad[3 * q + 2][id] = dline(*src[q], id, sxy, F.dz);
+                    F.grad[3 * q + 0][id] = dline(*src[q], id, 1, F.dx, ox);
+                    F.grad[3 * q + 1][id] = dline(*src[q], id, sx, F.dy, oy);
+                    F.grad[3 * q + 2][id] = dline(*src[q], id, sxy, F.dz, oz);
                 }
             }
 }
@@ -802,7 +920,8 @@ void compute_viscous_flux(Ctx &F) {
 }
 
 // compute_vis_flux (ViscousFlux.cpp:210-255): rhs += d/dx Fv + d/dy Hv + d/dz Gv
-// for momentum and energy (the mass term is commented out in the reference).
+// for momentum and energy (the mass term is commented out in the reference),
+// with the orders of choose_scheme.
 void compute_vis_flux(Ctx &F) {
     const int ng = F.ng, sx = F.sx, sxy = F.sx * F.sy;
 #pragma omp parallel for collapse(2) schedule(static)
@@ -810,10 +929,13 @@ void compute_vis_flux(Ctx &F) {
         for (int j = ng; j < ng + F.ny; ++j)
             for (int i = ng; i < ng + F.nx; ++i) {
                 int id = F.I(i, j, k);
+                const int ox = choose_scheme(i, ng, ng + F.nx, F.per[0]);
+                const int oy = choose_scheme(j, ng, ng + F.ny, F.per[1]);
+                const int oz = choose_scheme(k, ng, ng + F.nz, F.per[2]);
                 for (int n = 1; n < 5; ++n) {
-                    F.rhs[n][id] += dline(F.Fv[n], id, 1, F.dx);
-                    F.rhs[n][id] += dline(F.Hv[n], id, sx, F.dy);
-                    F.rhs[n][id] += dline(F.Gv[n], id, sxy, F.dz);
+                    F.rhs[n][id] += dline(F.Fv[n], id, 1, F.dx, ox);
+                    F.rhs[n][id] += dline(F.Hv[n], id, sx, F.dy, oy);
+                    F.rhs[n][id] += dline(F.Gv[n], id, sxy, F.dz, oz);
                 }
             }
 }
@@ -822,7 +944,7 @@ void compute_vis_flux(Ctx &F) {
 void compute_rhs(Ctx &F) {
     for (int n = 0; n < 5; ++n) std::fill(F.rhs[n].begin(), F.rhs[n].end(), 0.0);
     compute_invis_flux(F);
-    exchange_halfnode(F);
+    apply_boundary_halfnode(F);
     compute_invis_dflux(F);
     compute_gradients(F);
     compute_viscous_flux(F);
@@ -944,6 +1066,7 @@ void *orc_create(int nx, int ny, int nz, double dx, double dy, double dz, const 
     F->sx = nx + 6; F->sy = ny + 6; F->sz = nz + 6;
     F->dx = dx; F->dy = dy; F->dz = dz;
     F->P = *P;
+    update_periodic(*F);
     const size_t tot = (size_t)F->sx * F->sy * F->sz;
     const size_t fxc = (size_t)(F->sx - 1) * F->sy * F->sz;
     const size_t fyc = (size_t)F->sx * (F->sy - 1) * F->sz;
@@ -967,7 +1090,10 @@ void *orc_create(int nx, int ny, int nz, double dx, double dy, double dz, const 
 
 void orc_destroy(void *h) { delete static_cast<Ctx *>(h); }
 
-void orc_set_params(void *h, const orc_params *P) { static_cast<Ctx *>(h)->P = *P; }
+void orc_set_params(void *h, const orc_params *P) {
+    static_cast<Ctx *>(h)->P = *P;
+    update_periodic(*static_cast<Ctx *>(h));
+}
 
 double *orc_field(void *h, const char *name, size_t *count) {
     std::vector<double> *v = field_by_name(*static_cast<Ctx *>(h), name, count);
@@ -989,7 +1115,7 @@ void orc_primitive_to_conserved(void *h) { primitive_to_conserved(*static_cast<C
 void orc_apply_boundary(void *h) { apply_boundary(*static_cast<Ctx *>(h)); }
 void orc_record_conserved_to0(void *h) { record_conserved_to0(*static_cast<Ctx *>(h)); }
 void orc_compute_invis_flux(void *h) { compute_invis_flux(*static_cast<Ctx *>(h)); }
-void orc_apply_boundary_halfnode_flux(void *h) { exchange_halfnode(*static_cast<Ctx *>(h)); }
+void orc_apply_boundary_halfnode_flux(void *h) { apply_boundary_halfnode(*static_cast<Ctx *>(h)); }
 void orc_compute_invis_dflux(void *h) { compute_invis_dflux(*static_cast<Ctx *>(h)); }
 void orc_compute_gradients(void *h) { compute_gradients(*static_cast<Ctx *>(h)); }
 void orc_compute_viscous_flux(void *h) { compute_viscous_flux(*static_cast<Ctx *>(h)); }

@@ -78,3 +78,30 @@ def test_algorithmic_flops_host_side():
     assert fc["face"] > f["face"]
     fr = _lib.algorithmic_flops(SolverParams(Ma=0.2, riemann_solver=RiemannSolver.Rusanov).derive())
     assert fr["face"] < f["face"]
+
+
+def test_boundary_combinations_checked_before_device_work():
+    """ns3d_create validates the boundary types before any HIP call: a
+    Periodic face opposite a non-periodic one, an unknown type, and the mixed
+    method on a box with physical faces are refused (NS3D_E_UNSUPPORTED /
+    NS3D_E_ARG), so this runs without a GPU."""
+    import ctypes
+
+    from ns3d_wcns_amd import periodic_box
+
+    lib = _lib.load_library()
+    grid = periodic_box(8).to_c(3)
+    cases = [
+        ([0, 3, 0, 0, 0, 0], _lib.PRECISION_FP64, -2, "Periodic face opposite"),
+        ([9, 9, 0, 0, 0, 0], _lib.PRECISION_FP64, -1, "unknown boundary type"),
+        ([3, 3, 0, 0, 0, 0], _lib.PRECISION_MIXED, -2, "periodic boxes only"),
+    ]
+    for bc, prec, code, msg in cases:
+        P = SolverParams(Ma=0.3, Re=100.0).derive()
+        cp = P.to_c()
+        for f in range(6):
+            cp.bc[f] = bc[f]
+        h = ctypes.c_void_p()
+        rc = lib.ns3d_create(ctypes.byref(h), ctypes.byref(grid), ctypes.byref(cp), prec)
+        assert rc == code, (bc, rc)
+        assert msg in lib.ns3d_last_error().decode(), lib.ns3d_last_error()

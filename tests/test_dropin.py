@@ -159,21 +159,80 @@ def test_dropin_time_advance_matches_checker(gpu, tmp_path, mode):
     ref.close()
 
 
-def test_dropin_refuses_non_periodic_boundaries(tmp_path):
-    """A Wall / Symmetry / Outflow / Inflow face is refused with a message
-    (no silent periodic treatment), before any device work."""
-    f = tmp_path / "solver.in"
-    f.write_text("global_nx = 8\nglobal_ny = 8\nglobal_nz = 8\nbc_xmin = Wall\n")
-    r = subprocess.run([DRIVER, str(f), "-", str(tmp_path / "final.bin")], capture_output=True, text=True)
-    assert r.returncode == 1
-    assert "only periodic boundaries are supported" in r.stderr
-
-
-def test_python_params_refuse_non_periodic_boundaries():
+def test_python_params_carry_boundary_types():
     from ns3d_wcns_amd import SolverParams
     from ns3d_wcns_amd.params import BCType
 
     P = SolverParams().derive()
-    P.bc[3] = BCType.Outflow
-    with pytest.raises(ValueError, match="only periodic"):
-        P.to_c()
+    P.bc[2] = P.bc[3] = BCType.Outflow
+    assert list(P.to_c().bc) == [0, 0, 3, 3, 0, 0]
+    assert not P.periodic
+
+
+def test_driver_parses_physical_faces(tmp_path):
+    """solver.in's bc_* keys reach the driver: an axis is periodic only when
+    both of its faces are (initialization.cpp:160-168)."""
+    f = tmp_path / "solver.in"
+    f.write_text("global_nx = 8\nglobal_ny = 8\nglobal_nz = 8\nbc_xmin = Wall\nbc_xmax = Outflow\n"
+                 "bc_ymin = Periodic\nbc_ymax = Periodic\nbc_zmin = Inflow\nbc_zmax = Symmetry\n")
+    r = subprocess.run([DRIVER, str(f), "--print-params"], capture_output=True, text=True, check=True)
+    assert "periods=0,1,0" in r.stdout
+
+
+REFERENCE_DEFAULT = """# the reference's default run (2D Riemann problem, Outflow faces), on a smaller grid
+Re = 500000
+Ma = 0.5
+gamma = 1.4
+Pr = 0.71
+cfl = 0.2
+dt_fixed = -0.0005
+interpolation = WENO5
+char_recon = False
+Riemann_Solver = Roe
+mdcd_disp = 0.0463783
+mdcd_diss = 0.020
+global_nx = 32
+global_ny = 32
+global_nz = 6
+bc_xmin = Outflow
+bc_xmax = Outflow
+bc_ymin = Outflow
+bc_ymax = Outflow
+bc_zmin = Outflow
+bc_zmax = Outflow
+Lx = 1.0
+Ly = 1.0
+Lz = 1.0
+max_steps = {steps}
+monitor_Stepfreq = 1
+output_Timefreq = 0.1
+TotalTime = 0.1
+restart = False
+"""
+
+
+@pytest.mark.gpu
+def test_dropin_reference_default_run(gpu, tmp_path):
+    """The driver with initialize_riemann_2d (the reference's built-in
+    initial state) and Outflow on all six faces against the checker."""
+    from oracle.ns3d_oracle import OracleField
+    from test_gpu_boundaries import riemann_2d_case
+
+    n, steps = (32, 32, 6), 10
+    (tmp_path / "solver.in").write_text(REFERENCE_DEFAULT.format(steps=steps))
+    args = [DRIVER, str(tmp_path / "solver.in"), "riemann2d", str(tmp_path / "final.bin")]
+    print("running", " ".join(args), flush=True)
+    subprocess.run(args, check=True, timeout=120)
+    dts, fields, energy = read_final(tmp_path / "final.bin", n, steps)
+    P, G, prim = riemann_2d_case(n)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    rdts = ref.time_advance(steps, -1.0, 0.1, 0.1)
+    assert len(dts) == len(rdts) == steps
+    assert np.max(np.abs(dts - rdts) / rdts) < 1e-12
+    from test_gpu_boundaries import state_errors
+
+    names = ("rho", "rhou", "rhov", "rhow", "E")
+    for name, err in state_errors(lambda q: fields[names.index(q)], ref.view).items():
+        assert err < 1e-10, (name, err)
+    ref.close()
