@@ -10,16 +10,16 @@ device through the C-ABI, and the checkpointed autograd rollout.
   last shorter);
 - tangent RHS vs central differences of compute_rhs: 1e-6;
 - gradient of J = mean 0.5 rho |u|^2 at the final time with respect to the
-  initial conserved fields vs central differences (eps = 1e-5) on 8 seeded
-  unit-norm directions: 1e-6 relative (1e-7 of |grad J| for directions
-  nearly orthogonal to it);
+  initial conserved fields vs central differences (eps = 1e-7) on 8 seeded
+  directions with unit-variance entries: |fd - ad| <= 1e-6 |grad J|_2 (the
+  RMS of such directional derivatives) on every direction;
 - checkpoint policies give the same gradient to 1e-12, and so do the
   adjoint steps with stored stage states (step_stages / adjoint_step_stages)
   and with recomputed ones;
 - time-integrated viscous dissipation J = sum_n dt_n eps(U_{n+1}): the
   checkpointed forward value equals a plain forward's, the gradient matches
-  central differences (eps = 1e-5) on 4 seeded directions to 1e-6 (1e-7 of
-  |grad J| for directions nearly orthogonal to it), and a mixed-precision
+  central differences (eps = 1e-7) on 4 seeded directions with
+  unit-variance entries to 1e-6 |grad J|_2 on every direction, and a mixed-precision
   forward with the FP64 adjoint gives the FP64 gradient to 1e-4.
 """
 import numpy as np
@@ -155,7 +155,18 @@ def rollout_grad(f, U, dts, policy, stats=None):
 
 def test_gradient_matches_central_differences(gpu):
     """J = mean 0.5 rho |u|^2 after 10 steps; gradient with respect to the
-    initial rho u, rho v, rho w and E, checkpoint every 5 steps."""
+    initial rho u, rho v, rho w and E, checkpoint every 5 steps, against
+    central differences (eps = 1e-7) along 8 seeded random directions d
+    with independent unit-variance entries.  For such directions the
+    derivative <grad J, d> has RMS |grad J|_2, so every direction is judged
+    against that scale: |fd - ad| <= 1e-6 |grad J|_2 (a single direction's
+    own derivative can be arbitrarily small, when d happens to be nearly
+    orthogonal to the gradient; the relative error is printed).  eps from a
+    sweep of 1e-4 .. 1e-8 on this case: above 1e-7 the differences carry
+    O(eps^2) truncation and cross branch switches of the flux (min / max of
+    wave speeds), up to 1.3e-6 |grad J| at eps = 1e-6; below, the rounding
+    floor of the perturbed trajectories (~1e-16 J / eps) takes over; at 1e-7
+    every direction agrees to 5e-9 |grad J|."""
     import torch
     from ns3d_wcns_amd.adjoint import central_difference
 
@@ -167,26 +178,16 @@ def test_gradient_matches_central_differences(gpu):
         dts.append(f.compute_timestep())
         f.step(dts[-1])
     J0, grad = rollout_grad(f, U, dts, "uniform:5")
-    # The central difference carries a rounding floor of about 1e-16 J / eps
-    # in absolute terms, which at eps = 1e-6 reaches 1e-6 relative on a
-    # direction whose derivative is ten times below the gradient's norm, and
-    # moves with ulp-level changes of the forward.  eps = 1e-5 keeps that
-    # floor below the bar and the O(eps^2) truncation far below it; every
-    # direction is judged against |grad J| |d| (the Cauchy-Schwarz scale),
-    # and against its own derivative unless that is below 1 % of |grad J|.
-    eps = 1e-5
+    eps = 1e-7
     gnorm = float(torch.linalg.vector_norm(grad[1:]))
     for k in range(8):
         d = rand_interior(U.shape, 100 + k)
         d[0] = 0.0  # directions in (rho u, rho v, rho w, E)
-        d /= torch.linalg.vector_norm(d)
         fd = float(central_difference(f, U, d, dts, eps))
         ad = float((grad * d).sum())
-        rel, cs = abs(fd - ad) / max(abs(ad), 1e-300), abs(fd - ad) / gnorm
-        print(f"direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e} over |grad| {cs:.2e}")
-        assert cs < 1e-7, (k, cs)
-        if abs(ad) >= 1e-2 * gnorm:
-            assert rel < 1e-6, (k, rel)
+        err = abs(fd - ad)
+        print(f"direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {err / abs(ad):.2e} over |grad J| {err / gnorm:.2e}")
+        assert err <= 1e-6 * gnorm, (k, err / gnorm)
     f.close()
 
 
@@ -314,24 +315,24 @@ def test_time_integrated_dissipation_gradient(gpu):
         _, g = integral_grad(f, U, dts, density, policy)
         err = float((g - grad).abs().max() / grad.abs().max())
         assert err < 1e-12, (policy, err)
-    # J is a sum of volume means rounded at ~1e-16 J each: eps = 1e-5 keeps
-    # that floor (~1e-16 J / eps) below the 1e-6 bar, and the O(eps^2)
-    # truncation far below it.  A direction nearly orthogonal to the
-    # gradient is judged against |grad J| |d| (the Cauchy-Schwarz scale).
-    eps = 1e-5
+    # Directions with independent unit-variance entries: their derivative
+    # has RMS |grad J|_2, the scale every direction is judged against
+    # (|fd - ad| <= 1e-6 |grad J|_2; the relative error is printed).  eps =
+    # 1e-7 from a sweep of 1e-4 .. 1e-8: larger steps carry truncation and
+    # branch crossings (4.9e-7 |grad J| at 1e-6), smaller ones the rounding of
+    # J, a sum of volume means (~1e-16 J / eps); at 1e-7 the worst direction
+    # agrees to 5.8e-8 |grad J|.
+    eps = 1e-7
     gnorm = float(torch.linalg.vector_norm(grad[1:]))
     for k in range(4):
         d = rand_interior(U.shape, 200 + k)
         d[0] = 0.0
-        d /= torch.linalg.vector_norm(d)
         fd = (time_integral_value(f, U + eps * d, dts, density) - time_integral_value(f, U - eps * d, dts, density)) / (
             2 * eps)
         ad = float((grad * d).sum())
-        rel, cs = abs(fd - ad) / abs(ad), abs(fd - ad) / gnorm
-        print(f"direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e} over |grad| {cs:.2e}")
-        assert cs < 1e-7, (k, cs)
-        if abs(ad) >= 1e-2 * gnorm:
-            assert rel < 1e-6, (k, rel)
+        err = abs(fd - ad)
+        print(f"direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {err / abs(ad):.2e} over |grad J| {err / gnorm:.2e}")
+        assert err <= 1e-6 * gnorm, (k, err / gnorm)
     f.close()
 
 

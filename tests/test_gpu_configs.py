@@ -13,9 +13,9 @@ E(k) ~ k^4 exp(-2 k^2 / k0^2), k0 = 4, rho = T = 1.
       (norm-wise relative);
     * WCNS5 + HLLC at the full 256^3: one RK3 step against the checker
       (dt bit-exact, fields 1e-13, the step's increment 1e-10);
-    * every combination at 256^3: one step conserves mass, momentum and
-      energy (|sum dU| <= 1e-12 sum |dU| per field: the half-node flux
-      differences telescope on the periodic box).
+    * every combination at 256^3: the RHS sums to zero and one step
+      conserves mass, momentum and energy, up to rounding (the half-node
+      flux differences telescope on the periodic box).
 - Mixed vs plain (Ma_t = 0.3, Re = 1600, WCNS5 + HLLC):
     * one RHS of the mixed method against the checker on 48^3: 5e-5 per
       field (FP64: 1e-12);
@@ -122,8 +122,10 @@ def test_sweep_wcns5_hllc_256_cubed_one_step(gpu, shocklet_256):
 @pytest.mark.parametrize("interp,rs", SWEEP, ids=lambda x: x.name)
 def test_sweep_256_cubed_conservation(gpu, shocklet_256, interp, rs):
     """The fused RHS sums to zero over the periodic box (the half-node flux
-    and viscous-flux differences telescope): |sum R| <= 1e-13 sum |R| per
-    field.  One step then conserves each field up to the rounding of the
+    and viscous-flux differences telescope) up to the rounding of N cell
+    values: |sum R| <= 16 N eps max|R| per field (eps = 2^-52; the density
+    RHS of the solenoidal rho = 1 initial state nearly vanishes, so sum |R|
+    is no scale for it).  One step then conserves each field up to the rounding of the
     stage updates, which is biased where the state straddles a power of two
     (rho ~ 1): |sum dU| <= 4 N eps max|U| (eps = 2^-52, N cells).  The
     CPU checker shows the same drift, -1.6e-10 in rho over one step at 128^3."""
@@ -141,8 +143,9 @@ def test_sweep_256_cubed_conservation(gpu, shocklet_256, interp, rs):
     for name in CONS:
         r = dev.interior(dev.download("rhs_" + name))
         s, a = float(np.sum(r)), float(np.sum(np.abs(r)))
-        say(f"  rhs_{name}: |sum R| / sum |R| = {abs(s) / a:.1e}")
-        assert abs(s) <= 1e-13 * a, (name, s, a)
+        bound = 16 * r.size * 2.0 ** -52 * float(np.max(np.abs(r)))
+        say(f"  rhs_{name}: |sum R| = {abs(s):.2e}, sum |R| = {a:.2e}, rounding bound {bound:.2e}")
+        assert abs(s) <= bound, (name, s, bound)
     U0 = state_tensor(dev)
     dt = dev.compute_timestep()
     dev.step(dt)

@@ -205,3 +205,40 @@ def test_mixed_steps_hand_out_fp64_primitives(gpu):
     torch.cuda.synchronize()
     check("step_stages")
     d.close()
+
+
+def test_mixed_step_non_cubic_multi_tile_box(gpu):
+    """One mixed runge_kutta_3 step (stages 1 and 2 hand u, v, w, T to the
+    next viscous kernel as compact FP32) on a 72 x 40 x 66 box: two viscous
+    x tiles, several z segments, nx != ny != nz, so a stride or extent slip
+    in that hand-off would corrupt the viscous fluxes of stages 2 and 3.
+    The step's increment U1 - U0 against the checker's FP64 step: 5e-5
+    norm-wise per field (the mixed RHS bar on boxes of this size); and the
+    state equals the mixed time_advance's first step bit for bit."""
+    n = (72, 40, 66)
+    prim, ma = state(n, 0.3, noise=0.02, seed=31)
+    P = params(ma)
+    G = box(n)
+    ref = OracleField(G, P)
+    ref.initialize_primitive(*prim)
+    dt = ref.compute_timestep()
+    ref.record_conserved_to0()
+    ref.runge_kutta_3(dt)
+    dev = DeviceField(G, P, precision=PRECISION_MIXED)
+    dev.initialize_primitive(*prim)
+    assert dev.compute_timestep() == dt
+    dev.recordConservedTo0()
+    dev.runge_kutta_3(dt)
+    for name in ("rho", "rhou", "rhov", "rhow", "E"):
+        b0 = ref.interior(ref.view(name + "0"))
+        e = rel_err(dev.interior(dev.download(name)) - b0, ref.interior(ref.view(name)) - b0)
+        print(f"{name}: increment error {e:.2e}")
+        assert e < 5e-5, (name, e)
+    other = DeviceField(G, P, precision=PRECISION_MIXED)
+    other.initialize_primitive(*prim)
+    other.time_advance(1)
+    for name in ("rho", "rhou", "rhov", "rhow", "E"):
+        assert np.array_equal(other.interior(other.download(name)), dev.interior(dev.download(name))), name
+    other.close()
+    dev.close()
+    ref.close()
