@@ -41,6 +41,21 @@ def kernel_flops_per_cell(f, cls):
     return 0.0
 
 
+def profiled_valu(cls, n):
+    """VALU-busy percentage, VALU issue fraction and counted FP64 TFLOP/s of
+    one launch of a kernel class at n^3 from the committed rocprofv3 --pmc
+    summary (SQ_ACTIVE_INST_VALU, SQ_INSTS_VALU, SQ_INSTS_VALU_*_F64), or {}."""
+    path = os.path.join(PROFILE_DIR, f"{cls}_valu_{n}.md")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        for key in ("valu_busy_pct", "valu_issue", "fp64_tflops_counted"):
+            if line.startswith(key + ":"):
+                out[key] = float(line.split(":", 1)[1])
+    return out
+
+
 def profiled_traffic(cls, n):
     """HBM bytes per launch of a kernel class at n^3 from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, calibrated on a plain
@@ -86,13 +101,25 @@ def parse():
     ap.add_argument("--scale-adjoint-fp64", type=int, default=1,
                     help="also the FP64-forward gradient of the same objective, for comparison")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU checker on a bounded sample")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the multi-core CPU baseline (0: the box's CPU share, OMP_NUM_THREADS or 16, "
+                         "at most os.cpu_count())")
+    ap.add_argument("--cpu-1core-n", type=int, default=128,
+                    help="box of the single-core CPU baseline (the same workload on a smaller box)")
     return ap.parse_args()
 
 
-def cpu_baseline(prim, grid, params, threads):
-    """One full RK3 step of the same workload through the CPU checker (the
-    restatement of the reference's pass structure), on `threads` host cores."""
+def cpu_threads(requested):
+    if requested > 0:
+        return requested
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(share, os.cpu_count() or 1))
+
+
+def cpu_baseline(prim, grid, params, threads, what="workload"):
+    """One full RK3 step (CFL + 3 stages) through the CPU checker (the
+    restatement of the reference's pass structure, which the reference runs
+    on one MPI rank per core), on `threads` host threads."""
     from oracle.ns3d_oracle import OracleField, set_threads
 
     used = set_threads(threads)
@@ -105,8 +132,8 @@ def cpu_baseline(prim, grid, params, threads):
     el = time.perf_counter() - t0
     cells = grid.global_nx * grid.global_ny * grid.global_nz
     ref.close()
-    return {"value": cells / el / 1e6, "unit": "Mcell-RK3-steps/s", "cores": used, "kind": "port",
-            "sample": f"1 RK3 step (CFL + 3 stages) of the {grid.global_nx}^3 workload, {el:.2f} s wall"}
+    return {"value": round(cells / el / 1e6, 4), "unit": "Mcell-RK3-steps/s", "cores": used, "kind": "port",
+            "sample": f"1 RK3 step (CFL + 3 stages) of the {grid.global_nx}^3 {what}, {el:.2f} s wall"}
 
 
 def mixed_report(args, prim, G, P, ms_plain, timed_run):
@@ -184,8 +211,9 @@ def adjoint_report(args):
     Reports the backward/forward cost ratio of the whole sweep (recomputation
     included; the uniform policy keeps the stage states of the recomputed
     segment) and per step (adjoint step with its stages recomputed, and
-    with them stored), central-difference checks (eps = 1e-6) on 8
-    seeded unit-norm directions in (rho u, rho v, rho w, E), and the
+    with them stored), central-difference checks on 8 seeded random
+    directions in (rho u, rho v, rho w, E) (unit Euclidean norm at eps =
+    1e-6 and 1e-4, unit RMS per entry at eps = 1e-6), and the
     tangent/adjoint dot test of one step."""
     import torch
 
@@ -249,26 +277,30 @@ def adjoint_report(args):
     torch.cuda.synchronize()
     adj_stored_ms = e0.elapsed_time(e1) / 10
     del S1, S2
-    # central differences
+    # central differences along 8 seeded random directions in (rho u, rho v, rho w, E): of unit Euclidean
+    # norm over all entries (the configuration's statement) at eps = 1e-6 and 1e-4, and of unit RMS per
+    # entry at eps = 1e-6 (a unit-norm direction is nearly orthogonal to the gradient and perturbs each
+    # entry by ~3.5e-10 eps, so at eps = 1e-6 its difference sits on the rounding floor of the perturbed
+    # trajectories, ~1e-16 J / eps; unit RMS scales the derivative by sqrt(entries))
     g = torch.Generator(device="cpu").manual_seed(7)
-    worst, checks, cs_worst, wide = 0.0, [], 0.0, []
     gnorm = float(torch.linalg.vector_norm(grad[1:, 3:-3, 3:-3, 3:-3]))
-    eps = 1e-6
+    entries = 4 * n ** 3
+    rel6, rel4, rms6, cs6 = [], [], [], []
     for k in range(8):
         d = torch.zeros_like(U)
         d[1:, 3:-3, 3:-3, 3:-3] = torch.randn((4, n, n, n), generator=g, dtype=torch.float64).cuda()
         d /= torch.linalg.vector_norm(d)
-        with torch.no_grad():
-            fd = float(central_difference(f, U, d, dts, eps))
         ad = float((grad * d).sum())
-        rel = abs(fd - ad) / abs(ad)
-        worst = max(worst, rel)
-        checks.append(float(f"{rel:.2e}"))
-        # error relative to |grad J| |d| (Cauchy-Schwarz scale of any direction)
-        cs_worst = max(cs_worst, abs(fd - ad) / gnorm)
-        if k < 2:  # the rounding floor: the same check at eps = 1e-4
+        with torch.no_grad():
+            fd6 = float(central_difference(f, U, d, dts, 1e-6))
             fd4 = float(central_difference(f, U, d, dts, 1e-4))
-            wide.append(float(f"{abs(fd4 - ad) / abs(ad):.2e}"))
+            dr = d * math.sqrt(entries)
+            fdr = float(central_difference(f, U, dr, dts, 1e-6))
+        adr = ad * math.sqrt(entries)
+        rel6.append(float(f"{abs(fd6 - ad) / abs(ad):.2e}"))
+        cs6.append(abs(fd6 - ad) / gnorm)
+        rel4.append(float(f"{abs(fd4 - ad) / abs(ad):.2e}"))
+        rms6.append(float(f"{abs(fdr - adr) / abs(adr):.2e}"))
     # dot test of one step
     gg = torch.Generator(device="cpu").manual_seed(8)
     dU = torch.zeros_like(U)
@@ -296,11 +328,12 @@ def adjoint_report(args):
         "adjoint_step_stored_stages_ms": round(adj_stored_ms, 4),
         "step_cost_ratio_stored_stages": round(adj_stored_ms / fwd_ms, 3),
         "recomputed_steps": stats["recomputed_steps"],
-        "fd_eps": eps,
-        "fd_rel_err": checks,
-        "fd_max_rel_err": float(f"{worst:.3e}"),
-        "fd_max_err_over_grad_norm": float(f"{cs_worst:.3e}"),
-        "fd_rel_err_eps_1e-4_first2": wide,
+        "fd_unit_norm_eps_1e-6_rel_err": rel6,
+        "fd_unit_norm_eps_1e-6_max_err_over_grad_norm": float(f"{max(cs6):.3e}"),
+        "fd_unit_norm_eps_1e-4_rel_err": rel4,
+        "fd_unit_rms_eps_1e-6_rel_err": rms6,
+        "fd_max_rel_err": {"unit_norm_eps_1e-6": max(rel6), "unit_norm_eps_1e-4": max(rel4),
+                           "unit_rms_eps_1e-6": max(rms6)},
         "dot_test_rel_err": float(f"{abs(a - b) / abs(a):.3e}"),
     }
 
@@ -595,8 +628,9 @@ def main():
     fl = kernel_flops_per_cell(flops, dom) * cells
     achieved = fl / (per_launch[dom] * 1e-3) / 1e12
     traffic = profiled_traffic(dom, n)
+    valu = profiled_valu(dom, n)
     roofline = {
-        "bound": "mfma",
+        "bound": "fp64_valu",
         "achieved": round(achieved, 3),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -608,6 +642,11 @@ def main():
         "compute": "FP64 vector FMA/add/mul (v_fma_f64); no MFMA instruction is issued; the MI355X FP64 "
                    "vector and FP64 matrix peaks are both 78.6 TFLOP/s",
         "hbm_GBps_measured_traffic": round(traffic / (per_launch[dom] * 1e-3) / 1e9, 1) if traffic else None,
+        # rocprofv3 counters of the same kernel (profiles/<kernel>_valu_<n>.md): share of cycles the VALU
+        # is busy, fraction of the one-wave-instruction-per-CU-per-cycle issue rate, counted FP64 rate
+        "valu_busy_pct": valu.get("valu_busy_pct"),
+        "valu_issue_frac": valu.get("valu_issue"),
+        "fp64_TFLOPs_counted": valu.get("fp64_tflops_counted"),
     }
     step_tflops = flops["per_step"] * cells / (ms * 1e-3) / 1e12
     out = {
@@ -649,7 +688,12 @@ def main():
     if args.scale_adjoint:
         out["adjoint_scale"] = scale_adjoint_report(args)
     if args.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(prim, G, P, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(prim, G, P, cpu_threads(args.cpu_threads))
+        # one core (the analogue of one MPI rank of the reference) on a smaller box of the same workload:
+        # the per-cell work of a step does not depend on the box size
+        m = args.cpu_1core_n
+        prim1, _ = isotropic_state(m, mach_t=args.mach_t, seed=12345)
+        out["cpu_baseline_1core"] = cpu_baseline(prim1, periodic_box(m), P, 1, "box of the same workload")
     print(json.dumps(out))
 
 
