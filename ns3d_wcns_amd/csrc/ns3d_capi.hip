@@ -9,9 +9,11 @@
 //    9 compact viscous-flux fields, nx*ny*nz each;
 //   12 compact gradient fields, allocated on first use of compute_gradients;
 //   reduction partials (one per 64x4 block) and a few device scalars.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <utility>
@@ -72,6 +74,12 @@ struct ns3d_ctx {
     // (ns3d_bounded.h), with the reference-layout face arrays fref
     bool bounded;
     int per[3];  // axis periodic
+    // second stream of the fused stage: the z pencil runs on it beside the y
+    // pencil (both read the stage's state and write partials of their own);
+    // forked from and joined back into `stream` by the two events.  Null:
+    // serial pencils (NS3D_SERIAL_PENCILS=1 in the environment at create).
+    hipStream_t side;
+    hipEvent_t ev_fork, ev_join;
     double *fref_slab;
     double *fref[3][5];
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;  // (class, (start, stop))
@@ -86,21 +94,35 @@ thread_local std::string g_err;
 struct Timed {
     ns3d_ctx *c;
     int cls;
+    hipStream_t s;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(ns3d_ctx *ctx, int k) : c(ctx), cls(k) {
+    Timed(ns3d_ctx *ctx, int k, hipStream_t stream) : c(ctx), cls(k), s(stream) {
         if (!c->timing) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
             a = b = nullptr;
             return;
         }
-        (void)hipEventRecord(a, c->stream);
+        (void)hipEventRecord(a, s);
     }
+    Timed(ns3d_ctx *ctx, int k) : Timed(ctx, k, ctx->stream) {}
     ~Timed() {
         if (!a) return;
-        (void)hipEventRecord(b, c->stream);
+        (void)hipEventRecord(b, s);
         c->events.push_back({cls, {a, b}});
     }
 };
+
+// The side stream and its events (those created so far).
+void destroy_side(ns3d_ctx *c) {
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    c->side = nullptr;
+    c->ev_fork = c->ev_join = nullptr;
+}
 
 int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -415,16 +437,39 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     const SchemeCoef<float> c32 = coef32(c->P);
     const int interp = c->P.interpolation;
     const bool chr = c->P.char_recon != 0;
-    for (int dir = 2; dir >= 0; --dir) {  // z, y, x
-        Timed t(c, NS3D_K_PENCIL_X - dir);
+    auto pencils = [&](int dir, hipStream_t st) {
         switch (c->P.riemann_solver) {
-            case R_ROE: launch_pencils<R_ROE>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
-            case R_RUSANOV: launch_pencils<R_RUSANOV>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
-            case R_HLLC: launch_pencils<R_HLLC>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
-            case R_HLL: launch_pencils<R_HLL>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
-            case R_HLLC_P: launch_pencils<R_HLLC_P>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
-            default: launch_pencils<R_AUSM>(dir, interp, chr, c->precision, a, c64, c32, c->stream); break;
+            case R_ROE: launch_pencils<R_ROE>(dir, interp, chr, c->precision, a, c64, c32, st); break;
+            case R_RUSANOV: launch_pencils<R_RUSANOV>(dir, interp, chr, c->precision, a, c64, c32, st); break;
+            case R_HLLC: launch_pencils<R_HLLC>(dir, interp, chr, c->precision, a, c64, c32, st); break;
+            case R_HLL: launch_pencils<R_HLL>(dir, interp, chr, c->precision, a, c64, c32, st); break;
+            case R_HLLC_P: launch_pencils<R_HLLC_P>(dir, interp, chr, c->precision, a, c64, c32, st); break;
+            default: launch_pencils<R_AUSM>(dir, interp, chr, c->precision, a, c64, c32, st); break;
         }
+    };
+    // z on the side stream beside y on the context's stream (independent:
+    // both read U and the viscous fluxes, each writes its own partials); the
+    // x pencil, which reads both partials, waits for the join
+    {
+        hipStream_t zs = c->side ? c->side : c->stream;
+        if (c->side) {
+            NS3D_HIP(hipEventRecord(c->ev_fork, c->stream));
+            NS3D_HIP(hipStreamWaitEvent(zs, c->ev_fork, 0));
+        }
+        Timed t(c, NS3D_K_PENCIL_Z, zs);
+        pencils(2, zs);
+    }
+    {
+        Timed t(c, NS3D_K_PENCIL_Y);
+        pencils(1, c->stream);
+    }
+    if (c->side) {
+        NS3D_HIP(hipEventRecord(c->ev_join, c->side));
+        NS3D_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    }
+    {
+        Timed t(c, NS3D_K_PENCIL_X);
+        pencils(0, c->stream);
     }
     NS3D_LAUNCHED("pencils");
     if (stage >= 1) c->prims_fp32 = a.Q32[0] != nullptr;
@@ -1080,6 +1125,8 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     c->timing = false;
     c->visc_fp32 = false;
     c->prims_fp32 = false;
+    c->side = nullptr;
+    c->ev_fork = c->ev_join = nullptr;
     c->bounded = any_physical(*params);
     for (int a = 0; a < 3; ++a) c->per[a] = params->bc[2 * a] == NS3D_BC_PERIODIC;
     c->fref_slab = nullptr;
@@ -1107,6 +1154,18 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
         (void)hipFree(c->slab);
         delete c;
         return fail(NS3D_E_HIP, "initialisation failed");
+    }
+    // the fused stage's second stream (default priority), unless serial
+    // kernels are asked for; without it the kernels run in order on `stream`
+    const char *serial = std::getenv("NS3D_SERIAL_PENCILS");
+    if (!(serial && serial[0] == '1')) {
+        const bool ok = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess &&
+                        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess &&
+                        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            destroy_side(c);
+        }
     }
     *out = c;
     return NS3D_OK;
@@ -1151,6 +1210,7 @@ int ns3d_destroy(ns3d_ctx *c) {
     if (c->obs_work) (void)hipFree(c->obs_work);
     destroy_fft_plan(c->fft_plan);
     if (c->fref_slab) (void)hipFree(c->fref_slab);
+    destroy_side(c);
     (void)hipFree(c->slab);
     delete c;
     return NS3D_OK;
