@@ -17,14 +17,30 @@
  * vectors are refreshed by ns3d_sync_host(F) (before diagnostics or output)
  * and pushed by ns3d_sync_device(F).
  *
- * One rank, periodic in all directions: CartDecomp describes the single
- * rank and HaloRequests has nothing to wait for.  Errors are printed to
- * std::cerr with an "ns3d:" prefix and the function returns, as the
- * reference's functions do (InviscousFlux.cpp:19-31).
+ * One rank: CartDecomp describes the single rank (periodic along an axis
+ * whose two faces are Periodic) and HaloRequests has nothing to wait for;
+ * the six boundary types of SolverParams select the device's boundary
+ * fills.  Errors are printed to std::cerr with an "ns3d:" prefix and the
+ * function returns, as the reference's functions do
+ * (InviscousFlux.cpp:19-31); interpolate_select throws std::runtime_error on
+ * a stencil of the wrong size, as the reference's does
+ * (interpolation.cpp:18-24).
+ *
+ * The scalar functions of ns3d_func.h:73-131 (interpolate_select,
+ * weno5_interpolate, the MDCD interpolations, weno5_reconstruction, the six
+ * X_Riemann_solver and WCNS_Riemann_InviscidFlux) evaluate one stencil or
+ * one face on the host with the numerics the kernels run
+ * (ns3d_scalar_* in include/ns3d_hip.h); the solvers take the unit normals
+ * along x, y or z that the reference's flux loops pass.
+ *
+ * include/compat/ holds field_structures.h and ns3d_func.h, which include
+ * this header under the reference's file names, so that a translation unit
+ * written against them (maincycle.cpp) compiles unchanged.
  */
 #ifndef NS3D_DROPIN_H
 #define NS3D_DROPIN_H
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -69,11 +85,22 @@ struct SolverParams {  // numerical and control fields of field_structures.h:86-
     double Cp = Cv * gamma;
     double Rgas = 1.0 / (Ma * Ma * gamma);
     double get_mu(double T) const;  // Sutherland
+    bool use_periodic = false;      // carried, not read (as in the reference)
+    // carried, not read: the flux-vector-splitting, reconstruction and
+    // viscous-scheme selectors of the reference's unused FD paths
+    enum class FVS_Type { StegerWarming, LaxFriedrichs, Rusanov, VanLeer };
+    FVS_Type fvs_type = FVS_Type::StegerWarming;
+    enum class Reconstruction { WENO5, WCNS, LINEAR, MDCD };
+    enum class ViscousScheme { C6th, C4th };
+    // WENO5_JS (weno5_reconstruction's Jiang-Shu weights) is this library's
+    // addition to the reference's four
     enum class Interpolation { WENO5, ZERO, MDCD_LINEAR, MDCD_HYBRID, WENO5_JS };
     enum class RiemannSolver { Roe, Rusanov, HLLC, HLL, HLLC_p, AUSM };
     enum class BCType { Periodic, Wall, Symmetry, Outflow, Inflow };
     RiemannSolver riemann_solver = RiemannSolver::Roe;
+    Reconstruction recon = Reconstruction::WENO5;
     Interpolation interpolation = Interpolation::WENO5;
+    ViscousScheme vis_scheme = ViscousScheme::C6th;
     double mdcd_diss = 0.01;
     double mdcd_disp = 0.0463783;
     int stencil = 6;
@@ -98,33 +125,86 @@ struct HaloRequests {};  // nothing outstanding on one GPU
 
 struct ns3d_mirror;  // device state of one Field3D (ns3d_dropin.cpp)
 
-struct Field3D {  // the hot-path members of field_structures.h:196-509
+// The data members of field_structures.h:196-245 (Field3D adds the device
+// mirror and the member functions).
+struct ns3d_field_data {
     LocalDesc L;
     std::vector<double> rho, rhou, rhov, rhow, E;
     std::vector<double> rho0, rhou0, rhov0, rhow0, E0;
+    std::vector<double> res_rho, res_rhou, res_rhov, res_rhow, res_E;  // U - U0 (updateResiduals)
+    double global_res_rho = 0.0, global_res_rhou = 0.0, global_res_rhov = 0.0, global_res_rhow = 0.0,
+           global_res_E = 0.0;  // compute_diagnostics
+    double global_Etot = 0.0;   // compute_total_energy
     std::vector<double> u, v, w, p, T;
     std::vector<double> rhs_rho, rhs_rhou, rhs_rhov, rhs_rhow, rhs_E;
-    double global_Etot = 0.0;
-    // filled on demand by the stepwise functions (empty until then)
-    std::vector<double> du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz, dT_dx, dT_dy, dT_dz;
+    // cell-centre flux arrays of the reference's FD path (allocated, not written)
+    std::vector<double> Fflux_mass, Fflux_momx, Fflux_momy, Fflux_momz, Fflux_E;
+    std::vector<double> Hflux_mass, Hflux_momx, Hflux_momy, Hflux_momz, Hflux_E;
+    std::vector<double> Gflux_mass, Gflux_momx, Gflux_momy, Gflux_momz, Gflux_E;
+    // written by compute_viscous_flux and compute_gradients
     std::vector<double> Fvflux_mass, Fvflux_momx, Fvflux_momy, Fvflux_momz, Fvflux_E;
     std::vector<double> Hvflux_mass, Hvflux_momx, Hvflux_momy, Hvflux_momz, Hvflux_E;
     std::vector<double> Gvflux_mass, Gvflux_momx, Gvflux_momy, Gvflux_momz, Gvflux_E;
+    std::vector<double> du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz, dT_dx, dT_dy, dT_dz;
+    // face arrays (idx_fx / idx_fy / idx_fz), written by compute_invis_flux
+    // and apply_boundary_halfnode_flux
     std::vector<double> flux_fx_mass, flux_fx_momx, flux_fx_momy, flux_fx_momz, flux_fx_E;
     std::vector<double> flux_fy_mass, flux_fy_momx, flux_fy_momy, flux_fy_momz, flux_fy_E;
     std::vector<double> flux_fz_mass, flux_fz_momx, flux_fz_momy, flux_fz_momz, flux_fz_E;
+    // half-node interpolated values of the reference's FD path (allocated, not written)
+    std::vector<double> interp_fx_mass, interp_fx_momx, interp_fx_momy, interp_fx_momz, interp_fx_E;
+    std::vector<double> interp_fy_mass, interp_fy_momx, interp_fy_momy, interp_fy_momz, interp_fy_E;
+    std::vector<double> interp_fz_mass, interp_fz_momx, interp_fz_momy, interp_fz_momz, interp_fz_E;
+};
 
+struct Field3D : ns3d_field_data {  // field_structures.h:196-509
     Field3D() = default;
-    Field3D(const Field3D &o);  // host copy; the copy has no device mirror
+    Field3D(const Field3D &o) : ns3d_field_data(o) {}  // host copy; the copy has no device mirror
     Field3D &operator=(const Field3D &) = delete;
     ~Field3D();
 
-    // sx = nx + 2 ngx, ...; rho = 1, everything else 0 (Field3D::allocate)
+    // sx = nx + 2 ngx, ...; every array of the reference's allocate at its
+    // size: rho = 1, everything else 0 (field_structures.h:247-372)
     void allocate(const LocalDesc &Ld);
     int I(int i, int j, int k) const noexcept { return idx3(i, j, k, L); }
     void conservedToPrimitive(const SolverParams &par);
     void primitiveToConserved(const SolverParams &par);
     void recordConservedTo0();
+    // res = U - U0 over every cell (field_structures.h:500-508); a resident
+    // field first refreshes its host conserved and recorded fields
+    void updateResiduals();
+
+    // element accessors (field_structures.h:374-437)
+    double &Rho(int i, int j, int k) noexcept { return rho[I(i, j, k)]; }
+    double &RhoU(int i, int j, int k) noexcept { return rhou[I(i, j, k)]; }
+    double &RhoV(int i, int j, int k) noexcept { return rhov[I(i, j, k)]; }
+    double &RhoW(int i, int j, int k) noexcept { return rhow[I(i, j, k)]; }
+    double &Eint(int i, int j, int k) noexcept { return E[I(i, j, k)]; }
+    double &U(int i, int j, int k) noexcept { return u[I(i, j, k)]; }
+    double &V(int i, int j, int k) noexcept { return v[I(i, j, k)]; }
+    double &W(int i, int j, int k) noexcept { return w[I(i, j, k)]; }
+    double &P(int i, int j, int k) noexcept { return p[I(i, j, k)]; }
+    double &Temp(int i, int j, int k) noexcept { return T[I(i, j, k)]; }
+    double &RHS_Rho(int i, int j, int k) noexcept { return rhs_rho[I(i, j, k)]; }
+    double &RHS_RhoU(int i, int j, int k) noexcept { return rhs_rhou[I(i, j, k)]; }
+    double &RHS_RhoV(int i, int j, int k) noexcept { return rhs_rhov[I(i, j, k)]; }
+    double &RHS_RhoW(int i, int j, int k) noexcept { return rhs_rhow[I(i, j, k)]; }
+    double &RHS_E(int i, int j, int k) noexcept { return rhs_E[I(i, j, k)]; }
+    double &FX_mass(int f, int j, int k) noexcept { return flux_fx_mass[idx_fx(f, j, k, L)]; }
+    double &FX_momx(int f, int j, int k) noexcept { return flux_fx_momx[idx_fx(f, j, k, L)]; }
+    double &FX_momy(int f, int j, int k) noexcept { return flux_fx_momy[idx_fx(f, j, k, L)]; }
+    double &FX_momz(int f, int j, int k) noexcept { return flux_fx_momz[idx_fx(f, j, k, L)]; }
+    double &FX_E(int f, int j, int k) noexcept { return flux_fx_E[idx_fx(f, j, k, L)]; }
+    double &FY_mass(int i, int f, int k) noexcept { return flux_fy_mass[idx_fy(i, f, k, L)]; }
+    double &FY_momx(int i, int f, int k) noexcept { return flux_fy_momx[idx_fy(i, f, k, L)]; }
+    double &FY_momy(int i, int f, int k) noexcept { return flux_fy_momy[idx_fy(i, f, k, L)]; }
+    double &FY_momz(int i, int f, int k) noexcept { return flux_fy_momz[idx_fy(i, f, k, L)]; }
+    double &FY_E(int i, int f, int k) noexcept { return flux_fy_E[idx_fy(i, f, k, L)]; }
+    double &FZ_mass(int i, int j, int f) noexcept { return flux_fz_mass[idx_fz(i, j, f, L)]; }
+    double &FZ_momx(int i, int j, int f) noexcept { return flux_fz_momx[idx_fz(i, j, f, L)]; }
+    double &FZ_momy(int i, int j, int f) noexcept { return flux_fz_momy[idx_fz(i, j, f, L)]; }
+    double &FZ_momz(int i, int j, int f) noexcept { return flux_fz_momz[idx_fz(i, j, f, L)]; }
+    double &FZ_E(int i, int j, int f) noexcept { return flux_fz_E[idx_fz(i, j, f, L)]; }
 
     ns3d_mirror *dev = nullptr;
 };
@@ -149,7 +229,54 @@ void compute_rhs(Field3D &F, CartDecomp &C, GridDesc &G, SolverParams &P, HaloRe
 void runge_kutta_3(Field3D &F, CartDecomp &C, GridDesc &G, SolverParams &P, HaloRequests &reqs, double dt);
 double compute_timestep(Field3D &F, const GridDesc &G, const SolverParams &P);
 // sum 0.5 rho |u|^2 dV over the interior into F.global_Etot (postprocess.cpp:103-128)
-void compute_total_energy(Field3D &F, const GridDesc &G, CartDecomp &C, const SolverParams &P);
+void compute_total_energy(Field3D &F, const GridDesc &G, const CartDecomp &C, const SolverParams &P);
+
+// Scalar functions (ns3d_func.h:73-131): one stencil or one face on the host.
+void WCNS_Riemann_InviscidFlux(std::vector<double> &Fface, const std::vector<std::vector<double>> &Ut,
+                               const std::vector<std::vector<double>> &ut, const SolverParams &P, int dim);
+void Roe_Riemann_solver(std::vector<double> &Fface, const std::vector<double> &UL, const std::vector<double> &UR,
+                        double nx, double ny, double nz, double gamma);
+void Rusanov_Riemann_solver(std::vector<double> &Fface, const std::vector<double> &UL,
+                            const std::vector<double> &UR, double nx, double ny, double nz, double gamma);
+void HLLC_Riemann_solver(std::vector<double> &Fface, const std::vector<double> &UL, const std::vector<double> &UR,
+                         double nx, double ny, double nz, double gamma);
+void HLLC_p_Riemann_solver(std::vector<double> &Fface, const std::vector<double> &UL,
+                           const std::vector<double> &UR, double nx, double ny, double nz, double gamma);
+void HLL_Riemann_solver(std::vector<double> &Fface, const std::vector<double> &UL, const std::vector<double> &UR,
+                        double nx, double ny, double nz, double gamma);
+void AUSM_Riemann_solver(std::vector<double> &Fface, const std::vector<double> &UL, const std::vector<double> &UR,
+                         double nx, double ny, double nz, double gamma);
+double interpolate_select(const std::vector<double> &vstencil, double flag, const SolverParams P);
+double weno5_interpolate(const std::array<double, 6> &stencil);
+double zero_interpolate(const std::array<double, 2> &stencil);
+double mdcd_linear_interpolate(const std::array<double, 6> &stencil, SolverParams P);
+double mdcd_hybrid_interpolate(const std::array<double, 6> &stencil, SolverParams P);
+double weno5_reconstruction(const std::array<double, 6> &stencil);
+
+// Monitoring and post-processing (postprocess.cpp, output.cpp:208-242,
+// EnergySpectrum.cpp, ana_isotropic.cpp:196-340).
+// updateResiduals, then global_res_X = sqrt(sum over the interior of
+// res_X^2 dx^3) / max |X| (postprocess.cpp:7-101).
+void compute_diagnostics(Field3D &F, const SolverParams &P, const GridDesc &G);
+// output/<filename>: "step res_rho res_rhou res_rhov res_rhow res_E Etot",
+// the title and variable lines first and the file truncated at step 1.
+void write_residuals_tecplot(const Field3D &F, int step, const std::string &filename = "residuals.dat");
+// E(k), k = 0 .. 2 global_nx, of the velocity field (forward FFT over the
+// box normalised by 1/N, 0.5 (|U|^2 + |V|^2 + |W|^2) summed over shells
+// lround(|k|)), one "k E" line each at 16 significant digits; computed on
+// the device.
+void compute_energy_spectrum(const Field3D &F, const GridDesc &G, const CartDecomp &C,
+                             const std::string &filename = "Energy-spectrum.dat");
+// compute_turbulence_statistics (ana_isotropic.cpp:196-340): kinetic energy,
+// TKE, u_rms, dissipation 2 nu sum k^2 E(k), Taylor, Kolmogorov and
+// Taylor-Li scales, Re_lambda and Mach_t, appended to
+// output/turbulence_stats.dat (header when the file is new) and printed.
+void compute_turbulence_statistics(Field3D &F, const GridDesc &G, const SolverParams &P, const CartDecomp &C,
+                                   const double current_time);
+// output/output_time_<t, 5 decimals>_spectrum.dat, then the statistics
+// (postprocess.cpp:133-145).
+void isotropic_post_process(Field3D &F, const GridDesc &G, const CartDecomp &C, const SolverParams &P,
+                            const double current_time);
 
 // field.h5 output and restart with serial HDF5 on the one rank: attributes
 // global_nx/ny/nz (int), time, dx, dy, dz (double); datasets X Y Z rho u v w

@@ -257,6 +257,30 @@ int ns3d_interpolate_batch(const ns3d_params *params, const double *d_stencils, 
 int ns3d_riemann_batch(const ns3d_params *params, const double *d_UL, const double *d_UR, int n, int dim,
                        double *d_F, void *stream);
 
+/* ---- single stencils on the host (the scalar functions of ns3d_func.h:73-131) ----
+ * The kernels' own numerics (ns3d_device.h) compiled for the host, for
+ * callers that evaluate one stencil or one face at a time; no device is
+ * used.  Divisions and square roots are the host's IEEE operations, where
+ * the kernels refine the hardware estimates: the two agree to a few ulp.
+ *   scalar_interpolate: interpolate_select (interpolation.cpp:13-77), the
+ *     scheme params->interpolation on stencil[0..n-1]; n must be 6 (2 for
+ *     ZERO, zero_interpolate :80-83), NS3D_E_ARG otherwise, with the
+ *     reference's message; flag < 0 reverses the stencil.  WENO5 is
+ *     weno5_interpolate (:86-115), MDCD_LINEAR / MDCD_HYBRID
+ *     mdcd_linear_interpolate / mdcd_hybrid_interpolate (:118-209),
+ *     WENO5_JS weno5_reconstruction (reconstruction.cpp:69-100).
+ *   scalar_riemann: X_Riemann_solver (InviscousFlux.cpp:270-846) of
+ *     params->riemann_solver, conserved UL, UR -> F (5 each), with the unit
+ *     face normal along axis dim (0, 1, 2: the normals that
+ *     WCNS_Riemann_InviscidFlux passes, :156-160).
+ *   scalar_face_flux: WCNS_Riemann_InviscidFlux (InviscousFlux.cpp:146-268):
+ *     Ut (conserved rho, rho u, rho v, rho w, E) and ut (primitive rho, u,
+ *     v, w, p) of cells i-2..i+3, 5 x 6 row-major; ut is read only for
+ *     characteristic-wise interpolation (cells i and i+1). */
+int ns3d_scalar_interpolate(const ns3d_params *params, const double *stencil, int n, double flag, double *out);
+int ns3d_scalar_riemann(const ns3d_params *params, const double *UL, const double *UR, int dim, double *F);
+int ns3d_scalar_face_flux(const ns3d_params *params, const double *Ut, const double *ut, int dim, double *F);
+
 /* ---- isotropic-turbulence observables on the device ----
  * energy_spectrum: E(k) of the velocity fields (compute_energy_spectrum_rank0,
  * EnergySpectrum.cpp:63-156): forward 3D FFT normalised by 1/N, 0.5 (|U|^2 +

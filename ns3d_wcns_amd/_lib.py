@@ -120,6 +120,9 @@ _SIGS = {
     "ns3d_check_weno_pair_adjoint": [ctypes.POINTER(CParams), _DP, ctypes.c_double, ctypes.c_double, _DP, _DP],
     "ns3d_interpolate_batch": [ctypes.POINTER(CParams), _VP, ctypes.c_int, ctypes.c_double, _VP, _VP],
     "ns3d_riemann_batch": [ctypes.POINTER(CParams), _VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP],
+    "ns3d_scalar_interpolate": [ctypes.POINTER(CParams), _DP, ctypes.c_int, ctypes.c_double, _DP],
+    "ns3d_scalar_riemann": [ctypes.POINTER(CParams), _DP, _DP, ctypes.c_int, _DP],
+    "ns3d_scalar_face_flux": [ctypes.POINTER(CParams), _DP, _DP, ctypes.c_int, _DP],
 }
 
 

@@ -22,6 +22,7 @@
 
 #include "../../include/ns3d_hip.h"
 #include "ns3d_bounded.h"
+#include "ns3d_internal.h"
 #include "ns3d_kernels.h"
 #include "ns3d_observe.h"
 
@@ -1570,6 +1571,9 @@ int spectrum(ns3d_ctx *c, int nshell, double *host) {
 }
 
 }  // namespace
+
+int ns3d::set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+SchemeCoef<double> ns3d::scheme_coef64(const ns3d_params &P) { return coef64(P); }
 
 extern "C" {
 

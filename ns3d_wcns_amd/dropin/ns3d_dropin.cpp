@@ -12,8 +12,12 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <filesystem>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
+#include <sstream>
+#include <stdexcept>
 
 #include "../../include/ns3d_hip.h"
 
@@ -165,12 +169,6 @@ std::string trim(const std::string &s) {
 double SolverParams::get_mu(double T) const { return mu * std::pow(T, 1.5) * (1.0 + S_ref) / (T + S_ref); }
 
 // ------------------------------------------------------------ Field3D
-Field3D::Field3D(const Field3D &o)
-    : L(o.L), rho(o.rho), rhou(o.rhou), rhov(o.rhov), rhow(o.rhow), E(o.E), rho0(o.rho0), rhou0(o.rhou0),
-      rhov0(o.rhov0), rhow0(o.rhow0), E0(o.E0), u(o.u), v(o.v), w(o.w), p(o.p), T(o.T), rhs_rho(o.rhs_rho),
-      rhs_rhou(o.rhs_rhou), rhs_rhov(o.rhs_rhov), rhs_rhow(o.rhs_rhow), rhs_E(o.rhs_E), global_Etot(o.global_Etot),
-      dev(nullptr) {}
-
 Field3D::~Field3D() {
     if (dev) {
         if (dev->ctx) ns3d_destroy(dev->ctx);
@@ -185,9 +183,26 @@ void Field3D::allocate(const LocalDesc &Ld) {
     L.sz = L.nz + 2 * L.ngz;
     const size_t n = cells(L);
     rho.assign(n, 1.0);
-    for (auto *f : {&rhou, &rhov, &rhow, &E, &rho0, &rhou0, &rhov0, &rhow0, &E0, &u, &v, &w, &p, &T, &rhs_rho,
-                    &rhs_rhou, &rhs_rhov, &rhs_rhow, &rhs_E})
+    for (auto *f : {&rhou, &rhov, &rhow, &E, &rho0, &rhou0, &rhov0, &rhow0, &E0, &res_rho, &res_rhou, &res_rhov,
+                    &res_rhow, &res_E, &u, &v, &w, &p, &T, &rhs_rho, &rhs_rhou, &rhs_rhov, &rhs_rhow, &rhs_E,
+                    &Fflux_mass, &Fflux_momx, &Fflux_momy, &Fflux_momz, &Fflux_E, &Hflux_mass, &Hflux_momx,
+                    &Hflux_momy, &Hflux_momz, &Hflux_E, &Gflux_mass, &Gflux_momx, &Gflux_momy, &Gflux_momz,
+                    &Gflux_E, &Fvflux_mass, &Fvflux_momx, &Fvflux_momy, &Fvflux_momz, &Fvflux_E, &Hvflux_mass,
+                    &Hvflux_momx, &Hvflux_momy, &Hvflux_momz, &Hvflux_E, &Gvflux_mass, &Gvflux_momx, &Gvflux_momy,
+                    &Gvflux_momz, &Gvflux_E, &du_dx, &du_dy, &du_dz, &dv_dx, &dv_dy, &dv_dz, &dw_dx, &dw_dy,
+                    &dw_dz, &dT_dx, &dT_dy, &dT_dz})
         f->assign(n, 0.0);
+    const size_t nf[3] = {(size_t)(L.sx - 1) * L.sy * L.sz, (size_t)L.sx * (L.sy - 1) * L.sz,
+                          (size_t)L.sx * L.sy * (L.sz - 1)};
+    std::vector<double> *faces[3][10] = {
+        {&flux_fx_mass, &flux_fx_momx, &flux_fx_momy, &flux_fx_momz, &flux_fx_E, &interp_fx_mass, &interp_fx_momx,
+         &interp_fx_momy, &interp_fx_momz, &interp_fx_E},
+        {&flux_fy_mass, &flux_fy_momx, &flux_fy_momy, &flux_fy_momz, &flux_fy_E, &interp_fy_mass, &interp_fy_momx,
+         &interp_fy_momy, &interp_fy_momz, &interp_fy_E},
+        {&flux_fz_mass, &flux_fz_momx, &flux_fz_momy, &flux_fz_momz, &flux_fz_E, &interp_fz_mass, &interp_fz_momx,
+         &interp_fz_momy, &interp_fz_momz, &interp_fz_E}};
+    for (int d = 0; d < 3; ++d)
+        for (auto *f : faces[d]) f->assign(nf[d], 0.0);
     if (dev && dev->ctx) {
         ns3d_destroy(dev->ctx);
         dev->ctx = nullptr;
@@ -251,6 +266,20 @@ void Field3D::recordConservedTo0() {
     rhov0 = rhov;
     rhow0 = rhow;
     E0 = E;
+}
+
+void Field3D::updateResiduals() {
+    if (resident(*this) && dev->ctx)
+        for (int n = 0; n < 5; ++n)
+            if (!get(dev->ctx, NS3D_RHO + n, *conserved(*this, n)) || !get(dev->ctx, NS3D_RHO0 + n, *recorded(*this, n)))
+                return;
+    std::vector<double> *res[5] = {&res_rho, &res_rhou, &res_rhov, &res_rhow, &res_E};
+    for (int n = 0; n < 5; ++n) {
+        const std::vector<double> &a = *conserved(*this, n), &b = *recorded(*this, n);
+        std::vector<double> &r = *res[n];
+        r.resize(a.size());
+        for (size_t i = 0; i < a.size(); ++i) r[i] = a[i] - b[i];
+    }
 }
 
 // ------------------------------------------------------------ set-up
@@ -378,27 +407,44 @@ void apply_boundary(Field3D &F, GridDesc &G, CartDecomp &, const SolverParams &P
     for (int n = 0; n < 4; ++n) get(c, NS3D_U + n, *prims(F, n));
 }
 
-void apply_boundary_halfnode_flux(Field3D &, const GridDesc &, CartDecomp &, const SolverParams &) {
-    // on a periodic box the faces computed by compute_invis_flux already are
-    // the periodic images the reference exchanges
-}
+namespace {
 
-void exchange_halos_viscous_flux(Field3D &, CartDecomp &, LocalDesc &, HaloRequests &) {
-    // likewise for the viscous fluxes
-}
-
-void compute_invis_flux(Field3D &F, const SolverParams &P) {
-    ns3d_ctx *c = mirror(F, &P, nullptr);
-    if (!c || !push(F, c, false, false) || !ok(ns3d_compute_invis_flux(c), "compute_invis_flux")) return;
+// the device's face arrays into the host's (faithful mode)
+void pull_faces(Field3D &F, ns3d_ctx *c) {
+    if (resident(F)) return;
     std::vector<double> *ff[3][5] = {
         {&F.flux_fx_mass, &F.flux_fx_momx, &F.flux_fx_momy, &F.flux_fx_momz, &F.flux_fx_E},
         {&F.flux_fy_mass, &F.flux_fy_momx, &F.flux_fy_momy, &F.flux_fy_momz, &F.flux_fy_E},
         {&F.flux_fz_mass, &F.flux_fz_momx, &F.flux_fz_momy, &F.flux_fz_momz, &F.flux_fz_E}};
     for (int d = 0; d < 3; ++d)
         for (int n = 0; n < 5; ++n) {
-            ff[d][n]->assign(ns3d_face_count(c, d), 0.0);
+            ff[d][n]->resize(ns3d_face_count(c, d));
             if (!ok(ns3d_download_face_flux(c, d, n, ff[d][n]->data()), "face flux")) return;
         }
+}
+
+}  // namespace
+
+void apply_boundary_halfnode_flux(Field3D &F, const GridDesc &G, CartDecomp &, const SolverParams &P) {
+    // on a periodic axis the faces compute_invis_flux wrote already are the
+    // periodic images the reference exchanges; physical faces set their
+    // ghost faces on the device
+    ns3d_ctx *c = mirror(F, &P, &G);
+    if (!c || !ok(ns3d_apply_boundary_halfnode_flux(c), "apply_boundary_halfnode_flux")) return;
+    pull_faces(F, c);
+}
+
+void exchange_halos_viscous_flux(Field3D &F, CartDecomp &, LocalDesc &, HaloRequests &) {
+    // the viscous-flux consumers read periodic images by wrapped indexing and
+    // the one-sided stencils of physical faces, so no data moves
+    ns3d_ctx *c = mirror(F, nullptr, nullptr);
+    if (c) ok(ns3d_exchange_halos_viscous_flux(c), "exchange_halos_viscous_flux");
+}
+
+void compute_invis_flux(Field3D &F, const SolverParams &P) {
+    ns3d_ctx *c = mirror(F, &P, nullptr);
+    if (!c || !push(F, c, false, false) || !ok(ns3d_compute_invis_flux(c), "compute_invis_flux")) return;
+    pull_faces(F, c);
 }
 
 void compute_invis_dflux(Field3D &F, const SolverParams &P, const GridDesc &G) {
@@ -459,7 +505,7 @@ double compute_timestep(Field3D &F, const GridDesc &G, const SolverParams &P) {
     return dt;
 }
 
-void compute_total_energy(Field3D &F, const GridDesc &G, CartDecomp &, const SolverParams &P) {
+void compute_total_energy(Field3D &F, const GridDesc &G, const CartDecomp &, const SolverParams &P) {
     ns3d_ctx *c = mirror(F, &P, &G);
     double e = 0.0;
     if (!c || !push(F, c, false, false) || !ok(ns3d_total_kinetic_energy(c, &e), "compute_total_energy")) return;
@@ -540,4 +586,250 @@ void ns3d_sync_device(Field3D &F) {
         if (!put(c, NS3D_RHO + n, *conserved(F, n)) || !put(c, NS3D_RHO0 + n, *recorded(F, n)) ||
             !put(c, NS3D_U + n, *prims(F, n)) || !put(c, NS3D_RHS_RHO + n, *rhs(F, n)))
             return;
+}
+
+// ------------------------------------------------------------ scalar functions
+// One stencil or one face on the host (ns3d_func.h:73-131), through
+// ns3d_scalar_* of libns3d_hip.
+namespace {
+
+SolverParams with_interp(SolverParams P, SolverParams::Interpolation i) {
+    P.interpolation = i;
+    return P;
+}
+
+double interp_one(const double *st, int n, double flag, const SolverParams &P) {
+    const ns3d_params c = to_c(P);
+    double out = 0.0;
+    if (ns3d_scalar_interpolate(&c, st, n, flag, &out) != NS3D_OK)
+        throw std::runtime_error(ns3d_last_error());  // a stencil of the wrong size (interpolation.cpp:18-24)
+    return out;
+}
+
+// the axis of a unit coordinate normal, or -1
+int axis_of(double nx, double ny, double nz) {
+    if (nx == 1.0 && ny == 0.0 && nz == 0.0) return 0;
+    if (nx == 0.0 && ny == 1.0 && nz == 0.0) return 1;
+    if (nx == 0.0 && ny == 0.0 && nz == 1.0) return 2;
+    return -1;
+}
+
+void riemann_one(SolverParams::RiemannSolver rs, const char *name, std::vector<double> &F,
+                 const std::vector<double> &UL, const std::vector<double> &UR, double nx, double ny, double nz,
+                 double gamma) {
+    const int dim = axis_of(nx, ny, nz);
+    if (dim < 0) {
+        std::cerr << "ns3d: " << name << ": the face normal (" << nx << ", " << ny << ", " << nz
+                  << ") is not (1,0,0), (0,1,0) or (0,0,1)\n";
+        return;
+    }
+    if (UL.size() < 5 || UR.size() < 5) {
+        std::cerr << "ns3d: " << name << ": UL and UR need 5 conserved values\n";
+        return;
+    }
+    SolverParams P;
+    P.gamma = gamma;
+    P.riemann_solver = rs;
+    const ns3d_params c = to_c(P);
+    if (F.size() < 5) F.resize(5);
+    ok(ns3d_scalar_riemann(&c, UL.data(), UR.data(), dim, F.data()), name);
+}
+
+}  // namespace
+
+double interpolate_select(const std::vector<double> &vstencil, double flag, const SolverParams P) {
+    return interp_one(vstencil.data(), (int)vstencil.size(), flag, P);
+}
+
+double weno5_interpolate(const std::array<double, 6> &s) {
+    return interp_one(s.data(), 6, 1.0, with_interp(SolverParams(), SolverParams::Interpolation::WENO5));
+}
+
+double zero_interpolate(const std::array<double, 2> &s) { return s[0]; }
+
+double mdcd_linear_interpolate(const std::array<double, 6> &s, SolverParams P) {
+    return interp_one(s.data(), 6, 1.0, with_interp(P, SolverParams::Interpolation::MDCD_LINEAR));
+}
+
+double mdcd_hybrid_interpolate(const std::array<double, 6> &s, SolverParams P) {
+    return interp_one(s.data(), 6, 1.0, with_interp(P, SolverParams::Interpolation::MDCD_HYBRID));
+}
+
+double weno5_reconstruction(const std::array<double, 6> &s) {
+    return interp_one(s.data(), 6, 1.0, with_interp(SolverParams(), SolverParams::Interpolation::WENO5_JS));
+}
+
+void Roe_Riemann_solver(std::vector<double> &F, const std::vector<double> &UL, const std::vector<double> &UR,
+                        double nx, double ny, double nz, double gamma) {
+    riemann_one(SolverParams::RiemannSolver::Roe, "Roe_Riemann_solver", F, UL, UR, nx, ny, nz, gamma);
+}
+void Rusanov_Riemann_solver(std::vector<double> &F, const std::vector<double> &UL, const std::vector<double> &UR,
+                            double nx, double ny, double nz, double gamma) {
+    riemann_one(SolverParams::RiemannSolver::Rusanov, "Rusanov_Riemann_solver", F, UL, UR, nx, ny, nz, gamma);
+}
+void HLLC_Riemann_solver(std::vector<double> &F, const std::vector<double> &UL, const std::vector<double> &UR,
+                         double nx, double ny, double nz, double gamma) {
+    riemann_one(SolverParams::RiemannSolver::HLLC, "HLLC_Riemann_solver", F, UL, UR, nx, ny, nz, gamma);
+}
+void HLLC_p_Riemann_solver(std::vector<double> &F, const std::vector<double> &UL, const std::vector<double> &UR,
+                           double nx, double ny, double nz, double gamma) {
+    riemann_one(SolverParams::RiemannSolver::HLLC_p, "HLLC_p_Riemann_solver", F, UL, UR, nx, ny, nz, gamma);
+}
+void HLL_Riemann_solver(std::vector<double> &F, const std::vector<double> &UL, const std::vector<double> &UR,
+                        double nx, double ny, double nz, double gamma) {
+    riemann_one(SolverParams::RiemannSolver::HLL, "HLL_Riemann_solver", F, UL, UR, nx, ny, nz, gamma);
+}
+void AUSM_Riemann_solver(std::vector<double> &F, const std::vector<double> &UL, const std::vector<double> &UR,
+                         double nx, double ny, double nz, double gamma) {
+    riemann_one(SolverParams::RiemannSolver::AUSM, "AUSM_Riemann_solver", F, UL, UR, nx, ny, nz, gamma);
+}
+
+void WCNS_Riemann_InviscidFlux(std::vector<double> &Fface, const std::vector<std::vector<double>> &Ut,
+                               const std::vector<std::vector<double>> &ut, const SolverParams &P, int dim) {
+    // 5 x 6 row-major stencils (cells i-2..i+3); ut is read for
+    // characteristic-wise interpolation only
+    double st[30], pt[30] = {};
+    if (Ut.size() < 5 || (P.char_recon && ut.size() < 5)) {
+        std::cerr << "ns3d: WCNS_Riemann_InviscidFlux: Ut and ut need 5 rows\n";
+        return;
+    }
+    for (int n = 0; n < 5; ++n) {
+        if (Ut[n].size() < 6 || (P.char_recon && ut[n].size() < 6)) {
+            std::cerr << "ns3d: WCNS_Riemann_InviscidFlux: stencils of 6 cells are needed\n";
+            return;
+        }
+        for (int m = 0; m < 6; ++m) {
+            st[6 * n + m] = Ut[n][m];
+            if (P.char_recon) pt[6 * n + m] = ut[n][m];
+        }
+    }
+    const ns3d_params c = to_c(P);
+    if (Fface.size() < 5) Fface.resize(5);
+    ok(ns3d_scalar_face_flux(&c, st, pt, dim, Fface.data()), "WCNS_Riemann_InviscidFlux");
+}
+
+// ------------------------------------------------------------ monitoring and post-processing
+void compute_diagnostics(Field3D &F, const SolverParams &, const GridDesc &G) {
+    const LocalDesc &L = F.L;
+    const double dx = G.dx, dx3 = dx * dx * dx;  // dx only, as the reference
+    F.updateResiduals();
+    const std::vector<double> *U[5] = {&F.rho, &F.rhou, &F.rhov, &F.rhow, &F.E};
+    const std::vector<double> *R[5] = {&F.res_rho, &F.res_rhou, &F.res_rhov, &F.res_rhow, &F.res_E};
+    double sq[5] = {0, 0, 0, 0, 0}, mx[5] = {0, 0, 0, 0, 0};
+    for (int k = L.ngz; k < L.ngz + L.nz; ++k)
+        for (int j = L.ngy; j < L.ngy + L.ny; ++j)
+            for (int i = L.ngx; i < L.ngx + L.nx; ++i) {
+                const int id = F.I(i, j, k);
+                for (int n = 0; n < 5; ++n) {
+                    mx[n] = std::max(mx[n], std::abs((*U[n])[id]));
+                    const double r = (*R[n])[id];
+                    sq[n] += r * r * dx3;
+                }
+            }
+    double *out[5] = {&F.global_res_rho, &F.global_res_rhou, &F.global_res_rhov, &F.global_res_rhow, &F.global_res_E};
+    for (int n = 0; n < 5; ++n) *out[n] = std::sqrt(sq[n]) / mx[n];
+}
+
+void write_residuals_tecplot(const Field3D &F, int step, const std::string &filename) {
+    std::error_code ec;
+    std::filesystem::create_directories("output", ec);
+    if (ec) std::cerr << "Warning: could not create output directory 'output': " << ec.message() << "\n";
+    const std::filesystem::path path = std::filesystem::path("output") / filename;
+    std::ofstream ofs(path.string(), step == 1 ? std::ofstream::out : std::ofstream::out | std::ofstream::app);
+    if (!ofs) {
+        std::cerr << "Failed to open residuals file " << path.string() << "\n";
+        return;
+    }
+    if (step == 1) {
+        ofs << "TITLE = \"Residuals vs Step\"\n";
+        ofs << "VARIABLES = \"Step\" \"Res_rho\" \"Res_rhou\" \"Res_rhov\" \"Res_rhow\" \"Res_E\" \"Etot\"\n";
+    }
+    ofs << std::scientific << std::setprecision(8);
+    ofs << step << " " << F.global_res_rho << " " << F.global_res_rhou << " " << F.global_res_rhov << " "
+        << F.global_res_rhow << " " << F.global_res_E << " " << F.global_Etot << "\n";
+}
+
+namespace {
+
+// E(k), k = 0 .. 2 global_nx, on the device
+bool spectrum(Field3D &F, const GridDesc &G, std::vector<double> &Ek) {
+    ns3d_ctx *c = mirror(F, nullptr, &G);
+    if (!c || !push(F, c, false, false)) return false;
+    Ek.assign(2 * (size_t)G.global_nx + 1, 0.0);
+    return ok(ns3d_energy_spectrum(c, Ek.data(), (int)Ek.size()), "compute_energy_spectrum");
+}
+
+std::string time_prefix(double t) {
+    std::stringstream ss;
+    ss << "output/output_time_" << std::fixed << std::setprecision(5) << t;
+    return ss.str();
+}
+
+}  // namespace
+
+void compute_energy_spectrum(const Field3D &Fc, const GridDesc &G, const CartDecomp &, const std::string &filename) {
+    Field3D &F = const_cast<Field3D &>(Fc);  // the device mirror is a cache of the field
+    std::vector<double> Ek;
+    if (!spectrum(F, G, Ek)) return;
+    std::ofstream fout(filename);
+    fout << std::scientific << std::setprecision(16);
+    for (size_t k = 0; k < Ek.size(); ++k) fout << k << " " << Ek[k] << "\n";
+}
+
+void compute_turbulence_statistics(Field3D &F, const GridDesc &G, const SolverParams &P, const CartDecomp &,
+                                   const double current_time) {
+    // kinetic energy, TKE, u_rms, nu = <mu(T)> / <rho> and the dissipation
+    // 2 nu sum k^2 E(k) on the device; <(du/dx)^2> from compute_gradients
+    // and the mean sound speed sqrt(gamma Rgas T) over the interior
+    ns3d_ctx *c = mirror(F, &P, &G);
+    if (!c || !push(F, c, false, false)) return;
+    ns3d_turbulence st{};
+    if (!ok(ns3d_turbulence_statistics(c, &st), "compute_turbulence_statistics") ||
+        !ok(ns3d_compute_gradients(c), "compute_gradients"))
+        return;
+    const size_t n = cells(F.L);
+    std::vector<double> dudx(n), T(n);
+    if (!ok(ns3d_download_gradient(c, 0, dudx.data()), "gradient") || !get(c, NS3D_T, T)) return;
+    const LocalDesc &L = F.L;
+    double dudx2 = 0.0, sound = 0.0;
+    for (int k = 0; k < L.nz; ++k)
+        for (int j = 0; j < L.ny; ++j)
+            for (int i = 0; i < L.nx; ++i) {
+                const int id = F.I(i + L.ngx, j + L.ngy, k + L.ngz);
+                dudx2 += dudx[id] * dudx[id];
+                sound += std::sqrt(P.gamma * P.Rgas * T[id]);
+            }
+    const double N = (double)G.global_nx * G.global_ny * G.global_nz;
+    const double nu = st.nu, eps = st.dissipation;
+    const double taylor_Li = st.u_rms / std::sqrt(dudx2 / N);
+    const double mach_t = std::sqrt(st.kinetic_energy * 2) / (sound / N);
+    const double re_lambda = st.u_rms * taylor_Li / nu;
+    const double eta = std::pow(nu * nu * nu / eps, 0.25), u_eta = std::pow(nu * eps, 0.25),
+                 tau_eta = std::pow(nu / eps, 0.5);
+    const double taylor = std::sqrt(15.0 * nu * st.u_rms * st.u_rms / eps);
+    std::cout << "[Post] nu = " << nu << ", u_rms = " << st.u_rms << ", dudx = " << dudx2 / N
+              << ", Taylor_Li = " << taylor_Li << ", Mach_t = " << mach_t << "\n";
+    std::error_code ec;
+    std::filesystem::create_directories("output", ec);
+    const std::filesystem::path fpath("output/turbulence_stats.dat");
+    const bool exists = std::filesystem::exists(fpath);
+    std::ofstream fout(fpath, std::ios::app);
+    if (!fout) {
+        std::cout << "[Error] Cannot open output/turbulence_stats.dat for writing.\n";
+        return;
+    }
+    if (!exists)
+        fout << "current_time KineticEnergy TKE u_rms Dissipation Taylor Kol_scale Kol_vel Kol_time Taylor_Li "
+                "Re_lambda Mach_t\n";
+    fout << std::scientific << std::setprecision(8) << current_time << " " << st.kinetic_energy << " " << st.tke
+         << " " << st.u_rms << " " << eps << " " << taylor << " " << eta << " " << u_eta << " " << tau_eta << " "
+         << taylor_Li << " " << re_lambda << " " << mach_t << "\n";
+}
+
+void isotropic_post_process(Field3D &F, const GridDesc &G, const CartDecomp &C, const SolverParams &P,
+                            const double current_time) {
+    std::error_code ec;
+    std::filesystem::create_directories("output", ec);
+    compute_energy_spectrum(F, G, C, time_prefix(current_time) + "_spectrum.dat");
+    compute_turbulence_statistics(F, G, P, C, current_time);
 }
