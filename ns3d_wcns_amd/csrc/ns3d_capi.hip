@@ -75,10 +75,12 @@ struct ns3d_ctx {
     // (ns3d_bounded.h), with the reference-layout face arrays fref
     bool bounded;
     int per[3];  // axis periodic
-    // second stream of the fused stage: the z pencil runs on it beside the y
-    // pencil (both read the stage's state and write partials of their own);
-    // forked from and joined back into `stream` by the two events.  Null:
-    // serial pencils (NS3D_SERIAL_PENCILS=1 in the environment at create).
+    // second stream of the fused stage, with NS3D_CONCURRENT_PENCILS=1 in the
+    // environment at create: the z pencil runs on it beside the y pencil
+    // (both read the stage's state and write partials of their own), forked
+    // from and joined back into `stream` by the two events.  Null (the
+    // default): the pencils run in order on `stream`, so each launch's time
+    // is its own.
     hipStream_t side;
     hipEvent_t ev_fork, ev_join;
     double *fref_slab;
@@ -1156,10 +1158,10 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
         delete c;
         return fail(NS3D_E_HIP, "initialisation failed");
     }
-    // the fused stage's second stream (default priority), unless serial
-    // kernels are asked for; without it the kernels run in order on `stream`
-    const char *serial = std::getenv("NS3D_SERIAL_PENCILS");
-    if (!(serial && serial[0] == '1')) {
+    // the fused stage's second stream (default priority), when asked for;
+    // without it the kernels run in order on `stream`
+    const char *concurrent = std::getenv("NS3D_CONCURRENT_PENCILS");
+    if (concurrent && concurrent[0] == '1') {
         const bool ok = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess &&
                         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess &&
                         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) == hipSuccess;

@@ -387,7 +387,8 @@ def test_isotropic_post_process_files(gpu, probe, tmp_path):
     rgas = 1.0 / (ma * ma * g)
     rho, u, v, w, p = post_field(n, 1.0 / (g * ma * ma))
     T = p / (rho * rgas)
-    assert abs(float(out[0]) - T[0, 0, 0]) <= 1e-15 * T[0, 0, 0]
+    assert out[0].startswith("[Post] nu = ")  # the reference's console line (ana_isotropic.cpp:296-301)
+    assert abs(float(out[-1]) - T[0, 0, 0]) <= 1e-15 * T[0, 0, 0]
     # spectrum
     N = n ** 3
     E = sum(0.5 * np.abs(np.fft.fftn(q) / N) ** 2 for q in (u, v, w))
