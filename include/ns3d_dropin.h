@@ -252,6 +252,18 @@ double zero_interpolate(const std::array<double, 2> &stencil);
 double mdcd_linear_interpolate(const std::array<double, 6> &stencil, SolverParams P);
 double mdcd_hybrid_interpolate(const std::array<double, 6> &stencil, SolverParams P);
 double weno5_reconstruction(const std::array<double, 6> &stencil);
+// reconstruction.cpp:13-124, not on the reference's path (no caller): the
+// P.recon selector with the reference's size checks (std::runtime_error),
+// reversal for flag < 0 and fallbacks; 2-point linear (stencil[0]); the
+// 6-point MDCD form as the reference evaluates it, with C++ integer
+// division in its constants (1/2, 1/12, 7/12 -> 0; 3/2 -> 1; 5/2 -> 2); the
+// 6-point (1, -8, 37, 37, -8, 1)/60 and 4-point (-1, 7, 7, -1)/12 central
+// interpolations.
+double reconstruct_select(const std::vector<double> &vstencil, double flag, const SolverParams P);
+double linear_reconstruction(const std::array<double, 2> &stencil);
+double mdcd_reconstruction(const std::array<double, 6> &stencil, SolverParams P);
+double c6th_reconstruction(const std::array<double, 6> &stencil);
+double c4th_reconstruction(const std::array<double, 4> &stencil);
 
 // Monitoring and post-processing (postprocess.cpp, output.cpp:208-242,
 // EnergySpectrum.cpp, ana_isotropic.cpp:196-340).

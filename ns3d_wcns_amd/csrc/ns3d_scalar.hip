@@ -141,7 +141,7 @@ int ns3d_scalar_interpolate(const ns3d_params *p, const double *st, int n, doubl
         return error(NS3D_E_ARG, "unknown interpolation %d", p->interpolation);
     const int want = p->interpolation == I_ZERO ? 2 : 6;
     if (n != want) {
-        return error(NS3D_E_ARG, "interpolate_select: interpolation '%s' requires stencil size %d, got %d",
+        return error(NS3D_E_ARG, "interpolate_select: interpolation '%s requires stencil size %d, got %d",
                      interp_name(p->interpolation), want, n);
     }
     if (p->interpolation == I_ZERO) {

@@ -659,6 +659,51 @@ double weno5_reconstruction(const std::array<double, 6> &s) {
     return interp_one(s.data(), 6, 1.0, with_interp(SolverParams(), SolverParams::Interpolation::WENO5_JS));
 }
 
+// reconstruction.cpp:13-124 (host; nothing on the path calls these)
+double reconstruct_select(const std::vector<double> &v, double flag, const SolverParams P) {
+    const int n = (int)v.size();
+    auto require = [&](int want, const char *name) {
+        if (n != want)
+            throw std::runtime_error(std::string("reconstruct_select: reconstruction '") + name +
+                                     " requires stencil size " + std::to_string(want) + ", got " +
+                                     std::to_string(n));
+    };
+    using R = SolverParams::Reconstruction;
+    if (P.recon == R::WENO5 || P.recon == R::MDCD) {
+        require(6, P.recon == R::WENO5 ? "WENO5" : "MDCD");
+        std::array<double, 6> a;
+        for (int i = 0; i < 6; ++i) a[i] = v[flag < 0.0 ? 5 - i : i];
+        return P.recon == R::WENO5 ? weno5_reconstruction(a) : mdcd_reconstruction(a, P);
+    }
+    if (P.recon == R::LINEAR) {
+        require(2, "LINEAR");
+        return linear_reconstruction({v[flag < 0.0 ? 1 : 0], v[flag < 0.0 ? 0 : 1]});
+    }
+    // WCNS and anything else: the mean of the two cells beside the middle
+    if (n >= 2) {
+        const int mid = n / 2;
+        return flag >= 0.0 ? 0.5 * (v[mid] + v[mid - 1]) : 0.5 * (v[mid] + v[mid + 1]);
+    }
+    return v.empty() ? 0.0 : v[0];
+}
+
+double linear_reconstruction(const std::array<double, 2> &s) { return s[0]; }
+
+double mdcd_reconstruction(const std::array<double, 6> &s, SolverParams P) {
+    // the reference's constants are integer quotients: 1/2, 1/12 and 7/12
+    // are 0, 3/2 is 1 and 5/2 is 2, so the form is
+    // (-disp - 2 diss) s1 + (disp + 5 diss) s2 + (disp - 5 diss) s3 + (-disp + 2 diss) s4
+    const double diss = P.mdcd_diss, disp = P.mdcd_disp;
+    return (0 * disp + 0 * diss) * s[0] + (-1 * disp - 2 * diss - 0) * s[1] + (0 + disp + 5 * diss) * s[2] +
+           (0 + disp - 5 * diss) * s[3] + (-1 * disp + 2 * diss - 0) * s[4] + (0 * disp - 0 * diss) * s[5];
+}
+
+double c6th_reconstruction(const std::array<double, 6> &s) {
+    return (1.0 / 60.0) * (s[0] - 8 * s[1] + 37 * s[2] + 37 * s[3] - 8 * s[4] + s[5]);
+}
+
+double c4th_reconstruction(const std::array<double, 4> &s) { return (1.0 / 12.0) * (-s[0] + 7 * s[1] + 7 * s[2] - s[3]); }
+
 void Roe_Riemann_solver(std::vector<double> &F, const std::vector<double> &UL, const std::vector<double> &UR,
                         double nx, double ny, double nz, double gamma) {
     riemann_one(SolverParams::RiemannSolver::Roe, "Roe_Riemann_solver", F, UL, UR, nx, ny, nz, gamma);

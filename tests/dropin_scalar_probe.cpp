@@ -11,10 +11,14 @@
 //   riemann <rs> <nx> <ny> <nz> <gamma> UL0..UL4 UR0..UR4      X_Riemann_solver
 //   face <interp> <rs> <char> <dim> <gamma> Ut[5][6] ut[5][6]   WCNS_Riemann_InviscidFlux
 //   diag <nx> <ny> <nz> <ng> <dx> <step> <seed>                 compute_diagnostics + write_residuals_tecplot
+//   recon <kind> <recon> <flag> <diss> <disp> <n> s0 .. s(n-1)  reconstruct_select (kind 0, P.recon = recon),
+//                                                             linear_ (1), mdcd_ (2), c6th_ (3), c4th_ (4)
+//                                                             reconstruction on s0 ..
 //   post <n> <time>                                             isotropic_post_process of the analytic field
 //                                                               post_field() on a periodic n^3 box of side
 //                                                               2 pi (device; files under ./output)
 // An exception prints "throw <what>".
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <iostream>
@@ -111,6 +115,20 @@ int main() {
                 std::vector<double> F(5, 0.0);
                 WCNS_Riemann_InviscidFlux(F, Ut, ut, P, dim);
                 for (double f : F) put(f);
+            } else if (kind == "recon") {
+                int which, rc, n;
+                double flag;
+                SolverParams P;
+                in >> which >> rc >> flag >> P.mdcd_diss >> P.mdcd_disp >> n;
+                P.recon = static_cast<SolverParams::Reconstruction>(rc);
+                std::vector<double> v(n);
+                for (auto &x : v) in >> x;
+                v.resize(std::max(n, 6), 0.0);
+                if (which == 0) put(reconstruct_select(std::vector<double>(v.begin(), v.begin() + n), flag, P));
+                else if (which == 1) put(linear_reconstruction({v[0], v[1]}));
+                else if (which == 2) put(mdcd_reconstruction({v[0], v[1], v[2], v[3], v[4], v[5]}, P));
+                else if (which == 3) put(c6th_reconstruction({v[0], v[1], v[2], v[3], v[4], v[5]}));
+                else put(c4th_reconstruction({v[0], v[1], v[2], v[3]}));
             } else if (kind == "post") {
                 int n;
                 double t;

@@ -141,10 +141,10 @@ def test_scalar_interpolate_zero_and_sizes(lib):
     assert out.value == 7.0
     six = np.arange(6.0)
     assert lib.ns3d_scalar_interpolate(ctypes.byref(cp), dp(six), 6, 1.0, ctypes.byref(out)) == E_ARG
-    assert b"'ZERO' requires stencil size 2, got 6" in lib.ns3d_last_error()
+    assert b"'ZERO requires stencil size 2, got 6" in lib.ns3d_last_error()
     cp = P(Interpolation.WENO5).to_c()
     assert lib.ns3d_scalar_interpolate(ctypes.byref(cp), dp(six), 5, 1.0, ctypes.byref(out)) == E_ARG
-    assert b"'WENO5' requires stencil size 6, got 5" in lib.ns3d_last_error()
+    assert b"'WENO5 requires stencil size 6, got 5" in lib.ns3d_last_error()
 
 
 @pytest.mark.parametrize("rs", SOLVERS, ids=lambda r: r.name)
@@ -268,10 +268,68 @@ def test_reference_signature_errors(probe):
     out, err = run_probe(probe, ["interp 0 1.0 0.02 0.04 5 1 2 3 4 5",
                                  "interp 1 1.0 0.02 0.04 6 1 2 3 4 5 6",
                                  "riemann 2 0.6 0.8 0 1.4 1 0 0 0 2.5 1 0 0 0 2.5"])
-    assert out[0] == "throw interpolate_select: interpolation 'WENO5' requires stencil size 6, got 5"
-    assert out[1] == "throw interpolate_select: interpolation 'ZERO' requires stencil size 2, got 6"
+    assert out[0] == "throw interpolate_select: interpolation 'WENO5 requires stencil size 6, got 5"
+    assert out[1] == "throw interpolate_select: interpolation 'ZERO requires stencil size 2, got 6"
     assert [float(x) for x in out[2:7]] == [0.0] * 5  # Fface untouched
     assert "is not (1,0,0), (0,1,0) or (0,0,1)" in err
+
+
+def cell_averages(coef, cells, lo):
+    """Averages of the polynomial sum coef[m] x^m over the unit cells
+    [lo + j, lo + j + 1], j = 0 .. cells - 1."""
+    out = []
+    for j in range(cells):
+        a, b = lo + j, lo + j + 1
+        out.append(sum(c * (b ** (m + 1) - a ** (m + 1)) / (m + 1) for m, c in enumerate(coef)))
+    return out
+
+
+def test_reconstruction_functions(probe):
+    """reconstruction.cpp:13-124 (not on the reference's path): c6th / c4th
+    reproduce the interface value p(0) of polynomials of degree 5 / 3 from
+    their cell averages (and miss degree 6 / 4, so the orders are pinned);
+    mdcd_reconstruction keeps the reference's integer-quotient constants, so
+    a constant stencil gives 0 and its coefficients are (0, -disp - 2 diss,
+    disp + 5 diss, disp - 5 diss, -disp + 2 diss, 0); reconstruct_select
+    dispatches on P.recon with the reference's reversal, size errors and
+    fallback mean."""
+    rng = np.random.Generator(np.random.PCG64(77))
+    lines, want = [], []
+    for deg, exact in ((5, True), (6, False)):
+        coef = rng.uniform(-1, 1, deg + 1)
+        lines.append(f"recon 3 0 1 0 0 6 {fmt(cell_averages(coef, 6, -3))}")
+        want.append((coef[0], exact))
+    for deg, exact in ((3, True), (4, False)):
+        coef = rng.uniform(-1, 1, deg + 1)
+        lines.append(f"recon 4 0 1 0 0 4 {fmt(cell_averages(coef, 4, -2))}")
+        want.append((coef[0], exact))
+    out, _ = run_probe(probe, lines)
+    for got, (p0, exact) in zip(out, want):
+        assert (abs(float(got) - p0) < 1e-13) == exact, (got, p0, exact)
+    diss, disp = 0.02, 0.0463783
+    expect_c = [0.0, -disp - 2 * diss, disp + 5 * diss, disp - 5 * diss, -disp + 2 * diss, 0.0]
+    lines = [f"recon 2 0 1 {diss!r} {disp!r} 6 {fmt(np.eye(6)[m])}" for m in range(6)]
+    lines.append(f"recon 2 0 1 {diss!r} {disp!r} 6 {fmt(np.full(6, 3.7))}")
+    out, _ = run_probe(probe, lines)
+    np.testing.assert_allclose([float(x) for x in out[:6]], expect_c, rtol=0, atol=1e-17)
+    assert abs(float(out[6])) < 1e-15
+    st = rng.uniform(-1, 1, 6)
+    WENO5, WCNS, LINEAR, MDCD = 0, 1, 2, 3  # SolverParams::Reconstruction
+    out, _ = run_probe(probe, [f"recon 0 {WENO5} -1 0.01 0.0463783 6 {fmt(st)}",
+                               f"named 3 0 0 {fmt(st[::-1])}",
+                               f"recon 0 {MDCD} 1 {diss!r} {disp!r} 6 {fmt(st)}",
+                               f"recon 2 0 1 {diss!r} {disp!r} 6 {fmt(st)}",
+                               f"recon 0 {LINEAR} -1 0 0 2 {fmt([4.0, 9.0])}",
+                               f"recon 0 {WCNS} 1 0 0 6 {fmt(st)}",
+                               f"recon 0 {WCNS} -1 0 0 6 {fmt(st)}",
+                               f"recon 0 {WENO5} 1 0 0 5 {fmt(st[:5])}",
+                               f"recon 0 {LINEAR} 1 0 0 6 {fmt(st)}"])
+    assert out[0] == out[1]  # flag -1 reverses, then weno5_reconstruction
+    assert out[2] == out[3]
+    assert float(out[4]) == 9.0
+    assert float(out[5]) == 0.5 * (st[3] + st[2]) and float(out[6]) == 0.5 * (st[3] + st[4])
+    assert out[7] == "throw reconstruct_select: reconstruction 'WENO5 requires stencil size 6, got 5"
+    assert out[8] == "throw reconstruct_select: reconstruction 'LINEAR requires stencil size 2, got 6"
 
 
 # ------------------------------------------------------------------ monitoring
@@ -329,6 +387,11 @@ def test_dropin_exports_scalar_and_monitor_signatures():
             "mdcd_linear_interpolate(std::array<double, 6ul> const&, SolverParams)",
             "mdcd_hybrid_interpolate(std::array<double, 6ul> const&, SolverParams)",
             "weno5_reconstruction(std::array<double, 6ul> const&)",
+            f"reconstruct_select({vec} const&, double, SolverParams)",
+            "linear_reconstruction(std::array<double, 2ul> const&)",
+            "mdcd_reconstruction(std::array<double, 6ul> const&, SolverParams)",
+            "c6th_reconstruction(std::array<double, 6ul> const&)",
+            "c4th_reconstruction(std::array<double, 4ul> const&)",
             "compute_diagnostics(Field3D&, SolverParams const&, GridDesc const&)",
             "compute_total_energy(Field3D&, GridDesc const&, CartDecomp const&, SolverParams const&)",
             f"write_residuals_tecplot(Field3D const&, int, {string} const&)",
