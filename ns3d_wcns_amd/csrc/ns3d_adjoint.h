@@ -9,10 +9,12 @@
 // ties), d|x| = sign(x) with sign(0) = +1, and every branch condition (the
 // HLLC wave pattern) is frozen at the forward values.
 //
-// Supported: component-wise WCNS5 and WENO5-JS interpolation with the HLLC
-// and Rusanov fluxes.
+// Supported: component-wise WCNS5 and WENO5-JS interpolation with every
+// Riemann solver: HLLC and Rusanov by hand, Roe, HLL, HLLC_p and AUSM by
+// transposing their forward-mode tangent (riemann_vjp_dual).
 #pragma once
 #include "ns3d_common.h"
+#include "ns3d_dual.h"
 #include "ns3d_visc.h"
 
 namespace ns3d {
@@ -329,19 +331,66 @@ NS3D_HD void rusanov_vjp(const SchemeCoef<double> &c, const double *UL, const do
     unpack_vjp(c.gamma, UR, Rr, Rb, URb);
 }
 
+// The scheme coefficients on Dual (constants: zero tangents).
+NS3D_HD SchemeCoef<Dual> dual_coef(const SchemeCoef<double> &c) {
+    SchemeCoef<Dual> d;
+    d.gamma = Dual(c.gamma);
+    d.p_ref = Dual(c.p_ref);
+    d.H_ref = Dual(c.H_ref);
+    d.E_ref = Dual(c.E_ref);
+    for (int q = 0; q < 6; ++q) d.mdcd_lin[q] = Dual(c.mdcd_lin[q]);
+    for (int q = 0; q < 4; ++q) d.mdcd_d[q] = Dual(c.mdcd_d[q]);
+    return d;
+}
+
+// Reverse mode of riemann<RS, XS> by transposing its tangent: one
+// forward-mode evaluation per input state component (ten), each contracted
+// with Fb.  The same code and the same subgradients as the tangent-linear
+// model, so the two stay exact transposes; it costs about ten Dual
+// evaluations of the solver, for the solvers without a hand-written
+// reverse (Roe with its entropy fix, HLL, HLLC_p, AUSM).
+template <int RS, int XS>
+NS3D_HD void riemann_vjp_dual(const SchemeCoef<double> &c, const double *UL, const double *UR, const double *Fb,
+                              double *ULb, double *URb) {
+    const SchemeCoef<Dual> cd = dual_coef(c);
+#pragma unroll 1
+    for (int i = 0; i < 10; ++i) {
+        Dual L[5], R[5], F[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            L[n] = Dual(UL[n], i == n ? 1.0 : 0.0);
+            R[n] = Dual(UR[n], i == 5 + n ? 1.0 : 0.0);
+        }
+        riemann<RS, XS>(cd, L, R, F);
+        double sum = 0.0;
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sum += F[n].d * Fb[n];
+        if (i < 5) ULb[i] += sum;
+        else URb[i - 5] += sum;
+    }
+}
+
+// Reverse mode of riemann<RS, XS>: ULb += (dF/dUL)^T Fb, URb likewise.
+template <int RS, int XS>
+NS3D_HD void riemann_vjp(const SchemeCoef<double> &c, const double *UL, const double *UR, const double *Fb,
+                         double *ULb, double *URb) {
+    if constexpr (RS == R_HLLC) hllc_vjp(c, UL, UR, Fb, ULb, URb);
+    else if constexpr (RS == R_RUSANOV) rusanov_vjp(c, UL, UR, Fb, ULb, URb);
+    else riemann_vjp_dual<RS, XS>(c, UL, UR, Fb, ULb, URb);
+}
+
 // Reverse mode of the component-wise face flux face_flux<INTERP, RS, false>
 // on the rotated stencil st[var][m] (cells i-2..i+3): stb += (dF/dst)^T Fb.
-template <int INTERP, int RS>
+// XS: slot of the x velocity in the rotated frame (AUSM).
+template <int INTERP, int RS, int XS = 1>
 NS3D_HD void face_vjp(const SchemeCoef<double> &c, const double (&st)[5][6], const double *Fb, double (&stb)[5][6]) {
     static_assert(INTERP == I_WENO5 || INTERP == I_WENO5_JS, "adjoint: WCNS5 or WENO5-JS interpolation");
-    static_assert(RS == R_HLLC || RS == R_RUSANOV, "adjoint: HLLC or Rusanov flux");
     constexpr bool JS = INTERP == I_WENO5_JS;
     double UL[5], UR[5];
 #pragma unroll
     for (int n = 0; n < 5; ++n) interp_lr<INTERP>(c, st[n], UL[n], UR[n]);
     double ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
-    if constexpr (RS == R_HLLC) hllc_vjp(c, UL, UR, Fb, ULb, URb);
-    else rusanov_vjp(c, UL, UR, Fb, ULb, URb);
+    riemann_vjp<RS, XS>(c, UL, UR, Fb, ULb, URb);
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
         double fb[5] = {0, 0, 0, 0, 0};

@@ -262,8 +262,7 @@ __global__ void __launch_bounds__(256) k_adj_riemann(AdjArgs a) {
             double Fb[5], ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
 #pragma unroll
             for (int n = 0; n < 5; ++n) Fb[n] = a.FB[DIR][var[n]][of];
-            if constexpr (RS == R_HLLC) hllc_vjp(a.c, ULp, UR, Fb, ULb, URb);
-            else rusanov_vjp(a.c, ULp, UR, Fb, ULb, URb);
+            riemann_vjp<RS, DIR == 0 ? 1 : (DIR == 1 ? 3 : 2)>(a.c, ULp, UR, Fb, ULb, URb);
 #pragma unroll
             for (int n = 0; n < 5; ++n) {
                 a.SB[var[n]][of] = ULb[n];
@@ -449,8 +448,7 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a, LinTail tl) {
             UR[n] = A[(5 + n) * N + tp];
             Fb[n] = a.FB[0][n][o];
         }
-        if constexpr (RS == R_HLLC) hllc_vjp(a.c, UL, UR, Fb, ULb, URb);
-        else rusanov_vjp(a.c, UL, UR, Fb, ULb, URb);
+        riemann_vjp<RS, 1>(a.c, UL, UR, Fb, ULb, URb);
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
             A[n * N + t] = ULb[n];          // adjoint of the left state of face t + 1/2
@@ -532,8 +530,7 @@ __global__ void __launch_bounds__(XSEG) k_adj_xseg(AdjArgs a, LinTail tl, int ns
             UR[n] = A[(5 + n) * R + q + 1];
             Fb[n] = a.FB[0][n][of];
         }
-        if constexpr (RS == R_HLLC) hllc_vjp(a.c, UL, UR, Fb, ULb, URb);
-        else rusanov_vjp(a.c, UL, UR, Fb, ULb, URb);
+        riemann_vjp<RS, 1>(a.c, UL, UR, Fb, ULb, URb);
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
             A[n * R + q] = ULb[n];
@@ -813,41 +810,65 @@ void tl_face_i(const AdjArgs &a, int interp, int rs, bool chr, hipStream_t s) {
     }
 }
 
-template <int DIR>
-void adj_dir(const AdjArgs &a, int interp, int rs, hipStream_t s) {
-    const bool js = interp == I_WENO5_JS, rus = rs == R_RUSANOV;
-    if (js && rus) launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5_JS, R_RUSANOV>>(a, s);
-    else if (js) launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5_JS, R_HLLC>>(a, s);
-    else if (rus) launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5, R_RUSANOV>>(a, s);
-    else launch_adj_march<DIR, k_adj_riemann<DIR, I_WENO5, R_HLLC>>(a, s);
-    if (js) launch_adj_march<DIR, k_adj_weno<DIR, true, DIR == 1>>(a, s);
-    else launch_adj_march<DIR, k_adj_weno<DIR, false, DIR == 1>>(a, s);
+template <int DIR, bool JS, int RS>
+void adj_dir_t(const AdjArgs &a, hipStream_t s) {
+    launch_adj_march<DIR, k_adj_riemann<DIR, JS ? I_WENO5_JS : I_WENO5, RS>>(a, s);
+    launch_adj_march<DIR, k_adj_weno<DIR, JS, DIR == 1>>(a, s);
 }
 
-void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s, const LinTail &t) {
-    const bool js = interp == I_WENO5_JS, rus = rs == R_RUSANOV;
+template <int DIR, bool JS>
+void adj_dir_r(const AdjArgs &a, int rs, hipStream_t s) {
+    switch (rs) {
+        case R_ROE: adj_dir_t<DIR, JS, R_ROE>(a, s); break;
+        case R_RUSANOV: adj_dir_t<DIR, JS, R_RUSANOV>(a, s); break;
+        case R_HLLC: adj_dir_t<DIR, JS, R_HLLC>(a, s); break;
+        case R_HLL: adj_dir_t<DIR, JS, R_HLL>(a, s); break;
+        case R_HLLC_P: adj_dir_t<DIR, JS, R_HLLC_P>(a, s); break;
+        default: adj_dir_t<DIR, JS, R_AUSM>(a, s); break;
+    }
+}
+
+template <int DIR>
+void adj_dir(const AdjArgs &a, int interp, int rs, hipStream_t s) {
+    if (interp == I_WENO5_JS) adj_dir_r<DIR, true>(a, rs, s);
+    else adj_dir_r<DIR, false>(a, rs, s);
+}
+
+template <bool JS, int RS>
+void adj_x_t(const AdjArgs &a, hipStream_t s, const LinTail &t) {
     const int N = a.b.nx;
     if (N > XSEG) {  // segments of at most XSEG cells, balanced over the line
         const int nseg = (N + XSEG - 1) / XSEG, S = (N + nseg - 1) / nseg;
         const dim3 g((unsigned)((long)a.b.ny * a.b.nz * nseg)), bl(XSEG);
         const size_t shm = (size_t)xseg_doubles(S) * sizeof(double);
-        if (js && rus) hipLaunchKernelGGL((k_adj_xseg<true, R_RUSANOV>), g, bl, shm, s, a, t, nseg, S);
-        else if (js) hipLaunchKernelGGL((k_adj_xseg<true, R_HLLC>), g, bl, shm, s, a, t, nseg, S);
-        else if (rus) hipLaunchKernelGGL((k_adj_xseg<false, R_RUSANOV>), g, bl, shm, s, a, t, nseg, S);
-        else hipLaunchKernelGGL((k_adj_xseg<false, R_HLLC>), g, bl, shm, s, a, t, nseg, S);
+        hipLaunchKernelGGL((k_adj_xseg<JS, RS>), g, bl, shm, s, a, t, nseg, S);
         return;
     }
     const dim3 g((unsigned)((long)a.b.ny * a.b.nz)), bl((unsigned)(((N + 63) / 64) * 64));
     const size_t shm = (size_t)(5 * (N + 4) + 15 * N) * sizeof(double);
-    if (js && rus) hipLaunchKernelGGL((k_adj_xline<true, R_RUSANOV>), g, bl, shm, s, a, t);
-    else if (js) hipLaunchKernelGGL((k_adj_xline<true, R_HLLC>), g, bl, shm, s, a, t);
-    else if (rus) hipLaunchKernelGGL((k_adj_xline<false, R_RUSANOV>), g, bl, shm, s, a, t);
-    else hipLaunchKernelGGL((k_adj_xline<false, R_HLLC>), g, bl, shm, s, a, t);
+    hipLaunchKernelGGL((k_adj_xline<JS, RS>), g, bl, shm, s, a, t);
+}
+
+template <bool JS>
+void adj_x_r(const AdjArgs &a, int rs, hipStream_t s, const LinTail &t) {
+    switch (rs) {
+        case R_ROE: adj_x_t<JS, R_ROE>(a, s, t); break;
+        case R_RUSANOV: adj_x_t<JS, R_RUSANOV>(a, s, t); break;
+        case R_HLLC: adj_x_t<JS, R_HLLC>(a, s, t); break;
+        case R_HLL: adj_x_t<JS, R_HLL>(a, s, t); break;
+        case R_HLLC_P: adj_x_t<JS, R_HLLC_P>(a, s, t); break;
+        default: adj_x_t<JS, R_AUSM>(a, s, t); break;
+    }
+}
+
+void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s, const LinTail &t) {
+    if (interp == I_WENO5_JS) adj_x_r<true>(a, rs, s, t);
+    else adj_x_r<false>(a, rs, s, t);
 }
 }  // namespace
 
 bool adjoint_supported(int interp, int rs, bool chr) {
-    return !chr && (interp == I_WENO5 || interp == I_WENO5_JS) && (rs == R_HLLC || rs == R_RUSANOV);
+    return !chr && (interp == I_WENO5 || interp == I_WENO5_JS) && rs >= R_ROE && rs <= R_AUSM;
 }
 
 

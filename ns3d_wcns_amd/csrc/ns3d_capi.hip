@@ -910,7 +910,7 @@ int check_ptrs(const void *const *p, int n) {
 int check_adjoint(ns3d_ctx *c) {
     if (!adjoint_supported(c->P.interpolation, c->P.riemann_solver, c->P.char_recon != 0))
         return fail(NS3D_E_UNSUPPORTED,
-                    "the adjoint covers component-wise WCNS5 / WENO5-JS interpolation with HLLC or Rusanov flux "
+                    "the adjoint covers component-wise WCNS5 / WENO5-JS interpolation with any Riemann solver "
                     "(got interpolation %d, Riemann solver %d, char_recon %d)",
                     c->P.interpolation, c->P.riemann_solver, c->P.char_recon);
     return NS3D_OK;

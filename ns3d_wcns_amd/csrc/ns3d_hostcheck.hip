@@ -80,6 +80,19 @@ int face_tangent_all(const ns3d_params *p, const double *st, const double *dst, 
     }
 }
 
+template <int INTERP>
+void face_vjp_rs(int rs, const SchemeCoef<double> &c, const double (&s)[5][6], const double *Fb,
+                 double (&sb)[5][6]) {
+    switch (rs) {
+        case R_ROE: face_vjp<INTERP, R_ROE>(c, s, Fb, sb); break;
+        case R_RUSANOV: face_vjp<INTERP, R_RUSANOV>(c, s, Fb, sb); break;
+        case R_HLLC: face_vjp<INTERP, R_HLLC>(c, s, Fb, sb); break;
+        case R_HLL: face_vjp<INTERP, R_HLL>(c, s, Fb, sb); break;
+        case R_HLLC_P: face_vjp<INTERP, R_HLLC_P>(c, s, Fb, sb); break;
+        default: face_vjp<INTERP, R_AUSM>(c, s, Fb, sb); break;
+    }
+}
+
 }  // namespace
 
 extern "C" int ns3d_check_face_tangent(const ns3d_params *p, const double *st, const double *dst, double *F,
@@ -95,11 +108,8 @@ extern "C" int ns3d_check_face_adjoint(const ns3d_params *p, const double *st, c
     double s[5][6], sb[5][6] = {};
     for (int n = 0; n < 5; ++n)
         for (int m = 0; m < 6; ++m) s[n][m] = st[6 * n + m];
-    const bool js = p->interpolation == I_WENO5_JS, rus = p->riemann_solver == R_RUSANOV;
-    if (js && rus) face_vjp<I_WENO5_JS, R_RUSANOV>(c, s, Fb, sb);
-    else if (js) face_vjp<I_WENO5_JS, R_HLLC>(c, s, Fb, sb);
-    else if (rus) face_vjp<I_WENO5, R_RUSANOV>(c, s, Fb, sb);
-    else face_vjp<I_WENO5, R_HLLC>(c, s, Fb, sb);
+    if (p->interpolation == I_WENO5_JS) face_vjp_rs<I_WENO5_JS>(p->riemann_solver, c, s, Fb, sb);
+    else face_vjp_rs<I_WENO5>(p->riemann_solver, c, s, Fb, sb);
     for (int n = 0; n < 5; ++n)
         for (int m = 0; m < 6; ++m) stb[6 * n + m] = sb[n][m];
     return NS3D_OK;

@@ -7,7 +7,10 @@ device through the C-ABI, and the checkpointed autograd rollout.
   adjoint is hand-written reverse mode: independent implementations), on
   boxes of one and of several marching segments along every axis, and x
   lines held whole or in segments (512 cells: two, 517 cells: three, the
-  last shorter);
+  last shorter); the same for Roe, HLL, HLLC_p and AUSM, whose reverse
+  transposes their tangent solver by solver (the interpolation reverses,
+  the transposed differences and the viscous reverse stay independent of
+  the tangent), and the step dot test with Roe and AUSM;
 - tangent RHS vs central differences of compute_rhs: 1e-6;
 - gradient of J = mean 0.5 rho |u|^2 at the final time with respect to the
   initial conserved fields vs central differences (eps = 1e-7) on 8 seeded
@@ -32,6 +35,9 @@ pytestmark = pytest.mark.gpu
 
 ADJ = [(Interpolation.WENO5, RiemannSolver.HLLC), (Interpolation.WENO5, RiemannSolver.Rusanov),
        (Interpolation.WENO5_JS, RiemannSolver.HLLC), (Interpolation.WENO5_JS, RiemannSolver.Rusanov)]
+# the solvers whose reverse transposes their tangent (Roe, HLL, HLLC_p, AUSM)
+ADJ_DUAL = [(i, r) for i in (Interpolation.WENO5, Interpolation.WENO5_JS)
+            for r in (RiemannSolver.Roe, RiemannSolver.HLL, RiemannSolver.HLLC_p, RiemannSolver.AUSM)]
 NG = 3
 
 
@@ -93,6 +99,24 @@ def test_rhs_dot_product(gpu, interp, rs, n):
     f.close()
 
 
+@pytest.mark.parametrize("n", [SMALL, (40, 24, 20), (517, 6, 8)],
+                         ids=["one-segment", "several-segments", "ragged-x-segments"])
+@pytest.mark.parametrize("interp,rs", ADJ_DUAL, ids=lambda x: x.name)
+def test_rhs_dot_product_other_solvers(gpu, interp, rs, n):
+    import torch
+
+    P = params(0.17, interp, rs)
+    f, U = field_and_state(n, P)
+    dU = rand_interior(U.shape, 3) * 1e-3
+    lam = rand_interior(U.shape, 4)
+    tl, ad = torch.zeros_like(U), torch.zeros_like(U)
+    f.tangent_rhs(ptrs(dU), ptrs(tl))
+    f.adjoint_rhs(ptrs(lam), ptrs(ad))
+    torch.cuda.synchronize()
+    dot_agree(tl, lam, dU, ad, f"{interp.name}+{rs.name} {n}")
+    f.close()
+
+
 @pytest.mark.parametrize("interp,rs,char", [(Interpolation.WENO5, RiemannSolver.HLLC, False),
                                             (Interpolation.MDCD_LINEAR, RiemannSolver.Roe, False),
                                             (Interpolation.WENO5_JS, RiemannSolver.AUSM, False),
@@ -124,11 +148,12 @@ def test_tangent_rhs_matches_central_differences(gpu, interp, rs, char):
     f.close()
 
 
-@pytest.mark.parametrize("n", [16, 36])
-def test_step_dot_product(gpu, n):
+@pytest.mark.parametrize("n,rs", [(16, RiemannSolver.HLLC), (36, RiemannSolver.HLLC), (16, RiemannSolver.Roe),
+                                  (16, RiemannSolver.AUSM)], ids=lambda x: getattr(x, "name", str(x)))
+def test_step_dot_product(gpu, n, rs):
     import torch
 
-    P = params(0.17)
+    P = params(0.17, Interpolation.WENO5, rs)
     f, U = field_and_state(n, P)
     dt = 0.5 * f.compute_timestep()
     dU = rand_interior(U.shape, 5) * 1e-3
@@ -262,14 +287,16 @@ def rollout_grad_keep(f, U, dts, policy, keep, stats):
 def test_adjoint_rejects_unsupported_scheme(gpu):
     import torch
 
-    P = params(0.17, Interpolation.MDCD_HYBRID, RiemannSolver.HLLC)
-    f, U = field_and_state(SMALL, P)
-    lam = torch.zeros_like(U)
-    with pytest.raises(NS3DError):
-        f.adjoint_rhs(ptrs(lam), ptrs(torch.zeros_like(U)))
-    with pytest.raises(NS3DError):
-        f.adjoint_step(1e-3, ptrs(lam))
-    f.close()
+    for interp, char in ((Interpolation.MDCD_HYBRID, False), (Interpolation.MDCD_LINEAR, False),
+                         (Interpolation.WENO5, True)):
+        P = params(0.17, interp, RiemannSolver.HLLC, char=char)
+        f, U = field_and_state(SMALL, P)
+        lam = torch.zeros_like(U)
+        with pytest.raises(NS3DError):
+            f.adjoint_rhs(ptrs(lam), ptrs(torch.zeros_like(U)))
+        with pytest.raises(NS3DError):
+            f.adjoint_step(1e-3, ptrs(lam))
+        f.close()
 
 
 def integral_setup(precision=None, steps=8):
