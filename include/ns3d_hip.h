@@ -317,11 +317,11 @@ int ns3d_get_state(ns3d_ctx *ctx, double *const *d_U);
  * images in the ghosts of its array.  The non-smooth parts (min/max wave speeds, |.|, branches) use a
  * fixed subgradient, the same in the tangent and the adjoint.
  *   tangent_rhs:  out = (dRHS/dU) dU           any scheme combination
- *   adjoint_rhs:  out = (dRHS/dU)^T lam        component-wise WCNS5 or
- *                                              WENO5-JS, any Riemann solver
+ *   adjoint_rhs:  out = (dRHS/dU)^T lam        component-wise WCNS5,
+ *                                              WENO5-JS or MDCD, any solver
  *   tangent_step: advances the state one step and dU to its tangent
  *   adjoint_step: adj = (dU^{n+1}/dU^n)^T adj; the state stays U^n
- * The adjoint entry points return NS3D_E_UNSUPPORTED for MDCD and
+ * The adjoint entry points return NS3D_E_UNSUPPORTED for
  * characteristic-wise interpolation. */
 int ns3d_tangent_rhs(ns3d_ctx *ctx, const double *const *d_dU, double *const *d_out);
 int ns3d_adjoint_rhs(ns3d_ctx *ctx, const double *const *d_lam, double *const *d_out);

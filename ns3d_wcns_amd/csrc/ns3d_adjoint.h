@@ -9,9 +9,10 @@
 // ties), d|x| = sign(x) with sign(0) = +1, and every branch condition (the
 // HLLC wave pattern) is frozen at the forward values.
 //
-// Supported: component-wise WCNS5 and WENO5-JS interpolation with every
-// Riemann solver: HLLC and Rusanov by hand, Roe, HLL, HLLC_p and AUSM by
-// transposing their forward-mode tangent (riemann_vjp_dual).
+// Supported: component-wise WCNS5, WENO5-JS and MDCD (linear, hybrid)
+// interpolation with every Riemann solver: HLLC and Rusanov by hand, Roe,
+// HLL, HLLC_p and AUSM by transposing their forward-mode tangent
+// (riemann_vjp_dual).
 #pragma once
 #include "ns3d_common.h"
 #include "ns3d_dual.h"
@@ -149,6 +150,98 @@ NS3D_HD void weno_pair_vjp(const double *f, double ybL, double ybR, double *fb) 
 struct PrimBar {
     double rho, un, u1, u2, E, p, a, ir;
 };
+
+// The symmetric matrix of beta3_of's quadratic form times 120960 (the
+// diagonal doubled): d beta3 / d f_i = sum_j kB3[i][j] f_j / 120960.
+__device__ constexpr double kB3[6][6] = {
+    {2.0 * 271799, -2380800, 4086352, -3462252, 1458762, -245620},
+    {-2380800, 2.0 * 5653317, -20427884, 17905032, -7727988, 1325006},
+    {4086352, -20427884, 2.0 * 19510972, -35817664, 15929912, -2792660},
+    {-3462252, 17905032, -35817664, 2.0 * 17195652, -15880404, 2863984},
+    {1458762, -7727988, 15929912, -15880404, 2.0 * 3824847, -1429976},
+    {-245620, 1325006, -2792660, 2863984, -1429976, 2.0 * 139633}};
+
+// Reverse mode of the FP64 MDCD interpolations (interp6 of MDCD_LINEAR /
+// MDCD_HYBRID, off = 0) on f[0..5]: fb += yb dy/df; returns y.  The
+// hybrid's sensor branch is frozen at the forward values, and |.| takes
+// sign(0) = +1, as in the tangent.
+template <int INTERP>
+NS3D_HD double mdcd_vjp(const SchemeCoef<double> &c, const double *f, double yb, double *fb) {
+    static_assert(INTERP == I_MDCD_LINEAR || INTERP == I_MDCD_HYBRID, "MDCD interpolation");
+    const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
+    if constexpr (INTERP == I_MDCD_HYBRID) {
+        // the forward's sensor and branch (ns3d_device.h mdcd_hybrid)
+        const double eps = 0.9 * 0.4 / (1.0 - 0.9 * 0.4) * 1e-4;
+        const double a1 = ::fabs(f2 - f1) + ::fabs(f2 - 2.0 * f1 + f0);
+        const double b1 = ::fabs(f2 - f3) + ::fabs(f2 - 2.0 * f3 + f4);
+        const double a2 = ::fabs(f3 - f2) + ::fabs(f3 - 2.0 * f2 + f1);
+        const double b2 = ::fabs(f3 - f4) + ::fabs(f3 - 2.0 * f4 + f5);
+        const double sai1 = fdiv(2.0 * a1 * b1 + eps, a1 * a1 + b1 * b1 + eps);
+        const double sai2 = fdiv(2.0 * a2 * b2 + eps, a2 * a2 + b2 * b2 + eps);
+        const double sai = sai2 < sai1 ? sai2 : sai1;
+        if (!(sai > 0.4)) {
+            const double eps_small = 1e-40;
+            const double q[4] = {(3.0 * f0 - 10.0 * f1 + 15.0 * f2) * 0.125, (-f1 + 6.0 * f2 + 3.0 * f3) * 0.125,
+                                 (3.0 * f2 + 6.0 * f3 - f4) * 0.125, (15.0 * f3 - 10.0 * f4 + 3.0 * f5) * 0.125};
+            const double t0 = f0 - 2.0 * f1 + f2, s0 = f0 - 4.0 * f1 + 3.0 * f2;
+            const double t1 = f1 - 2.0 * f2 + f3, s1 = f1 - f3;
+            const double t2 = f2 - 2.0 * f3 + f4, s2 = 3.0 * f2 - 4.0 * f3 + f4;
+            double beta[4];
+            beta[0] = (13.0 / 12.0) * (t0 * t0) + 0.25 * (s0 * s0);
+            beta[1] = (13.0 / 12.0) * (t1 * t1) + 0.25 * (s1 * s1);
+            beta[2] = (13.0 / 12.0) * (t2 * t2) + 0.25 * (s2 * s2);
+            beta[3] = beta3_of<double>(f0, f1, f2, f3, f4, f5);
+            const double X = beta[3] - (beta[0] + 4.0 * beta[1] + beta[2]) * (1.0 / 6.0);
+            const double tau6 = ::fabs(X);
+            double r[4], z[4], w[4], N = 0.0, D = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                r[k] = frcp(beta[k] + eps_small);
+                z[k] = 20.0 + tau6 * r[k];
+                w[k] = c.mdcd_d[k] * (z[k] * z[k]);
+            }
+            N = w[0] * q[0] + w[1] * q[1] + w[2] * q[2] + w[3] * q[3];
+            D = w[0] + w[1] + w[2] + w[3];
+            const double iD = frcp(D), y = N * iD;
+            // y = N / D
+            const double Nb = yb * iD, Db = -yb * y * iD;
+            double qb[4], betab[4], tau6b = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                qb[k] = Nb * w[k];
+                const double zb = (Nb * q[k] + Db) * c.mdcd_d[k] * 2.0 * z[k];
+                tau6b += zb * r[k];
+                betab[k] = -zb * tau6 * r[k] * r[k];
+            }
+            const double Xb = X >= 0.0 ? tau6b : -tau6b;
+            betab[3] += Xb;
+            betab[0] -= Xb * (1.0 / 6.0);
+            betab[1] -= Xb * (4.0 / 6.0);
+            betab[2] -= Xb * (1.0 / 6.0);
+            const double tb0 = betab[0] * (13.0 / 6.0) * t0, sb0 = betab[0] * 0.5 * s0;
+            const double tb1 = betab[1] * (13.0 / 6.0) * t1, sb1 = betab[1] * 0.5 * s1;
+            const double tb2 = betab[2] * (13.0 / 6.0) * t2, sb2 = betab[2] * 0.5 * s2;
+            fb[0] += tb0 + sb0 + 0.375 * qb[0];
+            fb[1] += -2.0 * tb0 - 4.0 * sb0 + tb1 + sb1 - 1.25 * qb[0] - 0.125 * qb[1];
+            fb[2] += tb0 + 3.0 * sb0 - 2.0 * tb1 + tb2 + 3.0 * sb2 + 1.875 * qb[0] + 0.75 * qb[1] + 0.375 * qb[2];
+            fb[3] += tb1 - sb1 - 2.0 * tb2 - 4.0 * sb2 + 0.375 * qb[1] + 0.75 * qb[2] + 1.875 * qb[3];
+            fb[4] += tb2 + sb2 - 0.125 * qb[2] - 1.25 * qb[3];
+            fb[5] += 0.375 * qb[3];
+            const double b3 = betab[3] * (1.0 / 120960.0);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double g = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < 6; ++jj) g += kB3[i][jj] * f[jj];
+                fb[i] += b3 * g;
+            }
+            return y;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) fb[m] += c.mdcd_lin[m] * yb;
+    return mdcd_linear(c, f0, f1, f2, f3, f4, f5);
+}
 
 // Reverse mode of phys_flux: sb += (dF/ds)^T Fb.
 NS3D_HD void phys_flux_vjp(const Prim<double> &s, const double *Fb, PrimBar &sb) {
@@ -384,13 +477,27 @@ NS3D_HD void riemann_vjp(const SchemeCoef<double> &c, const double *UL, const do
 // XS: slot of the x velocity in the rotated frame (AUSM).
 template <int INTERP, int RS, int XS = 1>
 NS3D_HD void face_vjp(const SchemeCoef<double> &c, const double (&st)[5][6], const double *Fb, double (&stb)[5][6]) {
-    static_assert(INTERP == I_WENO5 || INTERP == I_WENO5_JS, "adjoint: WCNS5 or WENO5-JS interpolation");
-    constexpr bool JS = INTERP == I_WENO5_JS;
     double UL[5], UR[5];
 #pragma unroll
     for (int n = 0; n < 5; ++n) interp_lr<INTERP>(c, st[n], UL[n], UR[n]);
     double ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
     riemann_vjp<RS, XS>(c, UL, UR, Fb, ULb, URb);
+    if constexpr (INTERP == I_MDCD_LINEAR || INTERP == I_MDCD_HYBRID) {
+        // both states of the face from the same 6 cells, the right one mirrored
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            double fb[6] = {0, 0, 0, 0, 0, 0}, gb[6] = {0, 0, 0, 0, 0, 0};
+            const double fr[6] = {st[n][5], st[n][4], st[n][3], st[n][2], st[n][1], st[n][0]};
+            mdcd_vjp<INTERP>(c, st[n], ULb[n], fb);
+            mdcd_vjp<INTERP>(c, fr, URb[n], gb);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) stb[n][m] += fb[m] + gb[5 - m];
+        }
+        return;
+    } else {
+        static_assert(INTERP == I_WENO5 || INTERP == I_WENO5_JS, "adjoint: WCNS5, WENO5-JS or MDCD");
+    }
+    constexpr bool JS = INTERP == I_WENO5_JS;
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
         double fb[5] = {0, 0, 0, 0, 0};

@@ -338,6 +338,79 @@ __global__ void __launch_bounds__(256) k_adj_weno(AdjArgs a) {
     }
 }
 
+// The y / z direction of the MDCD adjoint (marching): per face f + 1/2 both
+// states from the six cells f-2..f+3 (the right one on the mirrored
+// stencil), the reverse of the Riemann flux with the face adjoint FB[DIR],
+// then the reverse of the two reconstructions, accumulated in a ring of the
+// six stencil cells held in LDS (one private column per thread, slot of cell
+// x: x mod 6).  A cell is complete once the six faces whose stencils hold it
+// are in, and is added to out (FIRST: written).
+template <int DIR, int INTERP, int RS, bool FIRST>
+__global__ void __launch_bounds__(256) k_adj_mdcd(AdjArgs a) {
+    __shared__ double ring[6][5][256];  // [cell mod 6][rotated var][thread]
+    const March<DIR> M(a.b, a.seg);
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    if (!M.ok) return;  // no barrier below: each thread owns its ring column
+    constexpr int P1 = 1 + DIR, P2 = 1 + (DIR + 1) % 3, P3 = 1 + (DIR + 2) % 3;
+    constexpr int XS = DIR == 0 ? 1 : (DIR == 1 ? 3 : 2);
+    const int var[5] = {0, P1, P2, P3, 4};
+    // faces c0-3 .. c1+1 hold the cells c0 .. c1-1; face f reads cells f-2..f+3
+    double w[5][6];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const long g = M.g(M.c0 - 5 + m);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) w[n][m] = a.U[var[n]][g];
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) ring[m][n][tid] = 0.0;
+    for (int f = M.c0 - 3; f <= M.c1 + 1; ++f) {
+        {
+            const long g = M.g(f + 3);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) w[n][5] = a.U[var[n]][g];
+        }
+        double UL[5], UR[5], Fb[5], ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
+        const long of = M.o(f);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            interp_lr<INTERP>(a.c, w[n], UL[n], UR[n]);
+            Fb[n] = a.FB[DIR][var[n]][of];
+        }
+        riemann_vjp<RS, XS>(a.c, UL, UR, Fb, ULb, URb);
+        const int base = f + 6;  // non-negative for the mod (f >= c0 - 3)
+#pragma unroll 1
+        for (int n = 0; n < 5; ++n) {
+            double fb[6] = {0, 0, 0, 0, 0, 0}, gb[6] = {0, 0, 0, 0, 0, 0};
+            const double fr[6] = {w[n][5], w[n][4], w[n][3], w[n][2], w[n][1], w[n][0]};
+            mdcd_vjp<INTERP>(a.c, w[n], ULb[n], fb);
+            mdcd_vjp<INTERP>(a.c, fr, URb[n], gb);
+            // only cells c0..c1-1 are accumulated: the slots of cells below c0
+            // are those of c0 + 1.. (uniform condition over the block)
+#pragma unroll
+            for (int m = 0; m < 6; ++m)
+                if (f - 2 + m >= M.c0 && f - 2 + m < M.c1) ring[(base - 2 + m) % 6][n][tid] += fb[m] + gb[5 - m];
+        }
+        const int done = f - 2;  // its faces done-3 .. done+2 are in
+        if (done >= M.c0) {
+            const int slot = (base - 2) % 6;
+            const long g = M.g(done);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                a.out[var[n]][g] = (FIRST ? 0.0 : a.out[var[n]][g]) + ring[slot][n][tid];
+                ring[slot][n][tid] = 0.0;  // reused by cell done + 6
+            }
+        }
+        if (done + 1 >= M.c1) break;  // cells c0..c1-1 are out
+#pragma unroll
+        for (int n = 0; n < 5; ++n)
+#pragma unroll
+            for (int m = 0; m < 5; ++m) w[n][m] = w[n][m + 1];
+    }
+}
+
 // alpha A + beta B + gamma p at ghosted cell g, component n (null operands
 // count as zero; p enters where l.P is set), in this order.
 __device__ __forceinline__ double lin_at(const LinComb &l, int n, long g, double p) {
@@ -478,6 +551,77 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a, LinTail tl) {
     }
     if (!own) return;
     xline_tail(a, tl, t, j, k, acc);
+}
+
+// The x direction of the MDCD adjoint, for x lines of any length: one block
+// of XSEG threads per segment of S <= XSEG cells (one segment when N <= XSEG;
+// cells beyond the line are periodic images).  Per face f + 1/2 of the
+// faces (s0-3)+1/2 .. (s0+S+1)+1/2, whose six-cell stencils reach the
+// segment: both states, the reverse of the Riemann flux; then per variable
+// the reverse of the two reconstructions of every face into LDS, and the
+// gather of the six contributions of each cell; then the viscous part, as in
+// k_adj_xline.
+__host__ __device__ inline long xmdcd_doubles(int S) { return 5L * (S + 10) + 16L * (S + 5); }
+
+template <int INTERP, int RS>
+__global__ void __launch_bounds__(XSEG) k_adj_mdcd_x(AdjArgs a, LinTail tl, int nseg, int S) {
+    extern __shared__ double lds[];
+    const Box &b = a.b;
+    const int N = b.nx, W = S + 10, F = S + 5;
+    const int t = threadIdx.x;
+    const int seg = blockIdx.x % nseg;
+    const long line = blockIdx.x / nseg;
+    const int j = (int)(line % b.ny), k = (int)(line / b.ny);
+    const int s0 = seg * S, Sl = min(S, N - s0);  // this segment: cells s0..s0+Sl-1
+    double *Ul = lds;          // [5][W]: slot x = cell s0 - 5 + x
+    double *A = Ul + 5 * W;    // [10][F]: slot q = face (s0 - 3 + q) + 1/2, left / right state adjoints
+    double *Cc = A + 10 * F;   // [6][F]: face q's contribution to cell s0 - 5 + q + m
+    const long grow = b.gidx(0, j, k);
+    for (int x = t; x < Sl + 10; x += XSEG) {
+        const long g = grow + wrap(s0 - 5 + x, N);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) Ul[n * W + x] = a.U[n][g];
+    }
+    __syncthreads();
+    for (int q = t; q < Sl + 5; q += XSEG) {  // stencil: state slots q..q+5
+        const long of = b.oidx(wrap(s0 - 3 + q, N), j, k);
+        double UL[5], UR[5], Fb[5], ULb[5] = {0, 0, 0, 0, 0}, URb[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            interp_lr<INTERP>(a.c, Ul + n * W + q, UL[n], UR[n]);
+            Fb[n] = a.FB[0][n][of];
+        }
+        riemann_vjp<RS, 1>(a.c, UL, UR, Fb, ULb, URb);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            A[n * F + q] = ULb[n];
+            A[(5 + n) * F + q] = URb[n];
+        }
+    }
+    __syncthreads();
+    double acc[5];
+#pragma unroll 1
+    for (int v = 0; v < 5; ++v) {
+        for (int q = t; q < Sl + 5; q += XSEG) {
+            const double *fl = Ul + v * W + q;
+            const double fr[6] = {fl[5], fl[4], fl[3], fl[2], fl[1], fl[0]};
+            double fb[6] = {0, 0, 0, 0, 0, 0}, gb[6] = {0, 0, 0, 0, 0, 0};
+            mdcd_vjp<INTERP>(a.c, fl, A[v * F + q], fb);
+            mdcd_vjp<INTERP>(a.c, fr, A[(5 + v) * F + q], gb);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) Cc[m * F + q] = fb[m] + gb[5 - m];
+        }
+        __syncthreads();
+        if (t < Sl) {  // cell s0+t gathers output m of face s0+t+2-m (slot t+5-m)
+            double sum = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) sum += Cc[m * F + t + 5 - m];
+            acc[v] = sum;
+        }
+        __syncthreads();  // Cc is rewritten by the next variable
+    }
+    if (t >= Sl) return;
+    xline_tail(a, tl, s0 + t, j, k, acc);
 }
 
 // The x direction for lines longer than XSEG cells: one block of XSEG
@@ -828,9 +972,24 @@ void adj_dir_r(const AdjArgs &a, int rs, hipStream_t s) {
     }
 }
 
+template <int DIR, int INTERP>
+void adj_mdcd_r(const AdjArgs &a, int rs, hipStream_t s) {
+    constexpr bool FIRST = DIR == 1;
+    switch (rs) {
+        case R_ROE: launch_adj_march<DIR, k_adj_mdcd<DIR, INTERP, R_ROE, FIRST>>(a, s); break;
+        case R_RUSANOV: launch_adj_march<DIR, k_adj_mdcd<DIR, INTERP, R_RUSANOV, FIRST>>(a, s); break;
+        case R_HLLC: launch_adj_march<DIR, k_adj_mdcd<DIR, INTERP, R_HLLC, FIRST>>(a, s); break;
+        case R_HLL: launch_adj_march<DIR, k_adj_mdcd<DIR, INTERP, R_HLL, FIRST>>(a, s); break;
+        case R_HLLC_P: launch_adj_march<DIR, k_adj_mdcd<DIR, INTERP, R_HLLC_P, FIRST>>(a, s); break;
+        default: launch_adj_march<DIR, k_adj_mdcd<DIR, INTERP, R_AUSM, FIRST>>(a, s); break;
+    }
+}
+
 template <int DIR>
 void adj_dir(const AdjArgs &a, int interp, int rs, hipStream_t s) {
-    if (interp == I_WENO5_JS) adj_dir_r<DIR, true>(a, rs, s);
+    if (interp == I_MDCD_LINEAR) adj_mdcd_r<DIR, I_MDCD_LINEAR>(a, rs, s);
+    else if (interp == I_MDCD_HYBRID) adj_mdcd_r<DIR, I_MDCD_HYBRID>(a, rs, s);
+    else if (interp == I_WENO5_JS) adj_dir_r<DIR, true>(a, rs, s);
     else adj_dir_r<DIR, false>(a, rs, s);
 }
 
@@ -861,14 +1020,38 @@ void adj_x_r(const AdjArgs &a, int rs, hipStream_t s, const LinTail &t) {
     }
 }
 
+template <int INTERP, int RS>
+void adj_mdcd_x_t(const AdjArgs &a, hipStream_t s, const LinTail &t) {
+    const int N = a.b.nx;
+    const int nseg = (N + XSEG - 1) / XSEG, S = (N + nseg - 1) / nseg;
+    const dim3 g((unsigned)((long)a.b.ny * a.b.nz * nseg)), bl(XSEG);
+    const size_t shm = (size_t)xmdcd_doubles(S) * sizeof(double);
+    hipLaunchKernelGGL((k_adj_mdcd_x<INTERP, RS>), g, bl, shm, s, a, t, nseg, S);
+}
+
+template <int INTERP>
+void adj_mdcd_x_r(const AdjArgs &a, int rs, hipStream_t s, const LinTail &t) {
+    switch (rs) {
+        case R_ROE: adj_mdcd_x_t<INTERP, R_ROE>(a, s, t); break;
+        case R_RUSANOV: adj_mdcd_x_t<INTERP, R_RUSANOV>(a, s, t); break;
+        case R_HLLC: adj_mdcd_x_t<INTERP, R_HLLC>(a, s, t); break;
+        case R_HLL: adj_mdcd_x_t<INTERP, R_HLL>(a, s, t); break;
+        case R_HLLC_P: adj_mdcd_x_t<INTERP, R_HLLC_P>(a, s, t); break;
+        default: adj_mdcd_x_t<INTERP, R_AUSM>(a, s, t); break;
+    }
+}
+
 void adj_x(const AdjArgs &a, int interp, int rs, hipStream_t s, const LinTail &t) {
-    if (interp == I_WENO5_JS) adj_x_r<true>(a, rs, s, t);
+    if (interp == I_MDCD_LINEAR) adj_mdcd_x_r<I_MDCD_LINEAR>(a, rs, s, t);
+    else if (interp == I_MDCD_HYBRID) adj_mdcd_x_r<I_MDCD_HYBRID>(a, rs, s, t);
+    else if (interp == I_WENO5_JS) adj_x_r<true>(a, rs, s, t);
     else adj_x_r<false>(a, rs, s, t);
 }
 }  // namespace
 
 bool adjoint_supported(int interp, int rs, bool chr) {
-    return !chr && (interp == I_WENO5 || interp == I_WENO5_JS) && rs >= R_ROE && rs <= R_AUSM;
+    return !chr && (interp == I_WENO5 || interp == I_WENO5_JS || interp == I_MDCD_LINEAR || interp == I_MDCD_HYBRID) &&
+           rs >= R_ROE && rs <= R_AUSM;
 }
 
 

@@ -108,8 +108,12 @@ extern "C" int ns3d_check_face_adjoint(const ns3d_params *p, const double *st, c
     double s[5][6], sb[5][6] = {};
     for (int n = 0; n < 5; ++n)
         for (int m = 0; m < 6; ++m) s[n][m] = st[6 * n + m];
-    if (p->interpolation == I_WENO5_JS) face_vjp_rs<I_WENO5_JS>(p->riemann_solver, c, s, Fb, sb);
-    else face_vjp_rs<I_WENO5>(p->riemann_solver, c, s, Fb, sb);
+    switch (p->interpolation) {
+        case I_WENO5_JS: face_vjp_rs<I_WENO5_JS>(p->riemann_solver, c, s, Fb, sb); break;
+        case I_MDCD_LINEAR: face_vjp_rs<I_MDCD_LINEAR>(p->riemann_solver, c, s, Fb, sb); break;
+        case I_MDCD_HYBRID: face_vjp_rs<I_MDCD_HYBRID>(p->riemann_solver, c, s, Fb, sb); break;
+        default: face_vjp_rs<I_WENO5>(p->riemann_solver, c, s, Fb, sb); break;
+    }
     for (int n = 0; n < 5; ++n)
         for (int m = 0; m < 6; ++m) stb[6 * n + m] = sb[n][m];
     return NS3D_OK;

@@ -6,7 +6,9 @@ stencils evaluated on the host (no GPU).
 - adjoint vs transposed tangent Jacobian: 1e-12 relative (two independent
   derivative implementations of one function for HLLC and Rusanov; for Roe,
   HLL, HLLC_p and AUSM the Riemann reverse is the transposed tangent itself,
-  so there the check pins the interpolation reverses and the plumbing);
+  so there the check pins the interpolation reverses and the plumbing), for
+  WCNS5, WENO5-JS, MDCD linear and MDCD hybrid (1e-11: the 6-point
+  indicator's large coefficients amplify rounding);
 - tangent vs central differences (h = 1e-6 of the input scale): 1e-6.
 """
 import ctypes
@@ -18,7 +20,8 @@ from cases import params
 from ns3d_wcns_amd import Interpolation, RiemannSolver
 from ns3d_wcns_amd._lib import load_library
 
-ADJ_COMBOS = [(i, r) for i in (Interpolation.WENO5, Interpolation.WENO5_JS) for r in RiemannSolver]
+ADJ_COMBOS = [(i, r) for i in (Interpolation.WENO5, Interpolation.WENO5_JS, Interpolation.MDCD_LINEAR,
+                               Interpolation.MDCD_HYBRID) for r in RiemannSolver]
 
 
 def _p(a):
@@ -60,7 +63,10 @@ def test_face_adjoint_is_transposed_tangent(interp, rs, mean_u):
         stb = np.zeros(30)
         assert lib.ns3d_check_face_adjoint(ctypes.byref(cp), _p(st), _p(Fb), _p(stb)) == 0
         ref = J.T @ Fb
-        assert np.max(np.abs(stb - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), (stb, ref)
+        # MDCD hybrid: beta3's coefficients (up to 3.6e7 / 120960) amplify
+        # the rounding of the two derivative orders
+        bar = 1e-11 if interp == Interpolation.MDCD_HYBRID else 1e-12
+        assert np.max(np.abs(stb - ref)) <= bar * max(1.0, np.max(np.abs(ref))), (stb, ref)
 
 
 @pytest.mark.parametrize("interp", [Interpolation.WENO5, Interpolation.WENO5_JS, Interpolation.MDCD_LINEAR,
@@ -87,8 +93,8 @@ def test_face_adjoint_rejects_unsupported_schemes():
     lib = load_library()
     st = stencil(np.random.Generator(np.random.PCG64(1)), 0.3)
     stb = np.zeros(30)
-    for interp, rs, char in [(Interpolation.MDCD_HYBRID, RiemannSolver.HLLC, False),
-                             (Interpolation.MDCD_LINEAR, RiemannSolver.Roe, False),
+    for interp, rs, char in [(Interpolation.MDCD_HYBRID, RiemannSolver.HLLC, True),
+                             (Interpolation.MDCD_LINEAR, RiemannSolver.Roe, True),
                              (Interpolation.WENO5, RiemannSolver.HLLC, True)]:
         cp = params(0.2, interp, rs, char=char).to_c()
         assert lib.ns3d_check_face_adjoint(ctypes.byref(cp), _p(st), _p(np.ones(5)), _p(stb)) == -2

@@ -10,8 +10,10 @@ device through the C-ABI, and the checkpointed autograd rollout.
   last shorter); the same for Roe, HLL, HLLC_p and AUSM, whose reverse
   transposes their tangent solver by solver (the interpolation reverses,
   the transposed differences and the viscous reverse stay independent of
-  the tangent), and the step dot test with Roe and AUSM;
-- tangent RHS vs central differences of compute_rhs: 1e-6;
+  the tangent), for MDCD linear and hybrid interpolation with four solvers,
+  and the step dot test with Roe, AUSM and MDCD hybrid;
+- tangent RHS vs central differences of compute_rhs: 1e-6 (step 1e-6 of
+  each field's scale, 1e-7 for MDCD hybrid);
 - gradient of J = mean 0.5 rho |u|^2 at the final time with respect to the
   initial conserved fields vs central differences (eps = 1e-7) on 8 seeded
   directions with unit-variance entries: |fd - ad| <= 1e-6 |grad J|_2 (the
@@ -38,6 +40,9 @@ ADJ = [(Interpolation.WENO5, RiemannSolver.HLLC), (Interpolation.WENO5, RiemannS
 # the solvers whose reverse transposes their tangent (Roe, HLL, HLLC_p, AUSM)
 ADJ_DUAL = [(i, r) for i in (Interpolation.WENO5, Interpolation.WENO5_JS)
             for r in (RiemannSolver.Roe, RiemannSolver.HLL, RiemannSolver.HLLC_p, RiemannSolver.AUSM)]
+# MDCD interpolation (six-cell face stencils; fused march and segmented x kernels)
+ADJ_MDCD = [(i, r) for i in (Interpolation.MDCD_LINEAR, Interpolation.MDCD_HYBRID)
+            for r in (RiemannSolver.HLLC, RiemannSolver.Rusanov, RiemannSolver.Roe, RiemannSolver.AUSM)]
 NG = 3
 
 
@@ -101,8 +106,8 @@ def test_rhs_dot_product(gpu, interp, rs, n):
 
 @pytest.mark.parametrize("n", [SMALL, (40, 24, 20), (517, 6, 8)],
                          ids=["one-segment", "several-segments", "ragged-x-segments"])
-@pytest.mark.parametrize("interp,rs", ADJ_DUAL, ids=lambda x: x.name)
-def test_rhs_dot_product_other_solvers(gpu, interp, rs, n):
+@pytest.mark.parametrize("interp,rs", ADJ_DUAL + ADJ_MDCD, ids=lambda x: x.name)
+def test_rhs_dot_product_other_schemes(gpu, interp, rs, n):
     import torch
 
     P = params(0.17, interp, rs)
@@ -119,6 +124,7 @@ def test_rhs_dot_product_other_solvers(gpu, interp, rs, n):
 
 @pytest.mark.parametrize("interp,rs,char", [(Interpolation.WENO5, RiemannSolver.HLLC, False),
                                             (Interpolation.MDCD_LINEAR, RiemannSolver.Roe, False),
+                                            (Interpolation.MDCD_HYBRID, RiemannSolver.Rusanov, False),
                                             (Interpolation.WENO5_JS, RiemannSolver.AUSM, False),
                                             (Interpolation.WENO5, RiemannSolver.HLLC_p, True)],
                          ids=lambda x: getattr(x, "name", str(x)))
@@ -131,7 +137,9 @@ def test_tangent_rhs_matches_central_differences(gpu, interp, rs, char):
     d *= U.abs().amax(dim=(1, 2, 3), keepdim=True)  # per-field scale
     tl = torch.zeros_like(U)
     f.tangent_rhs(ptrs(d), ptrs(tl))
-    h = 1e-6
+    # MDCD hybrid: the weights 20 + tau6 / beta_k curve strongly where a
+    # stencil is smooth, so the difference step is 10 times shorter
+    h = 1e-7 if interp == Interpolation.MDCD_HYBRID else 1e-6
     rhs = []
     for sgn in (1.0, -1.0):
         Us = (U + sgn * h * d).contiguous()
@@ -148,12 +156,17 @@ def test_tangent_rhs_matches_central_differences(gpu, interp, rs, char):
     f.close()
 
 
-@pytest.mark.parametrize("n,rs", [(16, RiemannSolver.HLLC), (36, RiemannSolver.HLLC), (16, RiemannSolver.Roe),
-                                  (16, RiemannSolver.AUSM)], ids=lambda x: getattr(x, "name", str(x)))
-def test_step_dot_product(gpu, n, rs):
+@pytest.mark.parametrize("n,interp,rs", [(16, Interpolation.WENO5, RiemannSolver.HLLC),
+                                         (36, Interpolation.WENO5, RiemannSolver.HLLC),
+                                         (16, Interpolation.WENO5, RiemannSolver.Roe),
+                                         (16, Interpolation.WENO5, RiemannSolver.AUSM),
+                                         (16, Interpolation.MDCD_HYBRID, RiemannSolver.HLLC),
+                                         (36, Interpolation.MDCD_HYBRID, RiemannSolver.Rusanov)],
+                         ids=lambda x: getattr(x, "name", str(x)))
+def test_step_dot_product(gpu, n, interp, rs):
     import torch
 
-    P = params(0.17, Interpolation.WENO5, rs)
+    P = params(0.17, interp, rs)
     f, U = field_and_state(n, P)
     dt = 0.5 * f.compute_timestep()
     dU = rand_interior(U.shape, 5) * 1e-3
@@ -287,7 +300,7 @@ def rollout_grad_keep(f, U, dts, policy, keep, stats):
 def test_adjoint_rejects_unsupported_scheme(gpu):
     import torch
 
-    for interp, char in ((Interpolation.MDCD_HYBRID, False), (Interpolation.MDCD_LINEAR, False),
+    for interp, char in ((Interpolation.MDCD_HYBRID, True), (Interpolation.MDCD_LINEAR, True),
                          (Interpolation.WENO5, True)):
         P = params(0.17, interp, RiemannSolver.HLLC, char=char)
         f, U = field_and_state(SMALL, P)
