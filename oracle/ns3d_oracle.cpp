@@ -16,9 +16,10 @@
 //   - characteristic-wise interpolation: which cell's eigenvectors project the
 //     stencil (cell i for the left state, cell i+1 for the right);
 //   - the MDCD-WENO linear weights' slopes in (disp, diss) (3/2 and 9/2), beyond
-//     their consistency, central value, mirror symmetry and linearity;
-//   - AUSM's pressure- and velocity-diffusion terms beyond their vanishing on
-//     equal states (Kp = 0.25, Ku = 0.75 and the f_a scaling).
+//     their consistency, central value, mirror symmetry and linearity.
+// AUSM is pinned as Liou's AUSM+-up (J. Comput. Phys. 214, 2006) with the
+// reference's choices: mean interface sound speed, M_inf^2 = 0.5, f_a on M_o^2,
+// beta = alpha = 0, the x velocity in the pressure diffusion.
 // Two reference quirks are reproduced: the 6-cell MDCD indicator's f0^2
 // coefficient 271799 (the indicator's definition gives 271779), and MDCD-WENO
 // linear weights whose smooth limit is exact only through quadratics.
