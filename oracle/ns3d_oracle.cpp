@@ -12,11 +12,11 @@
 // includes <fftw3-mpi.h>, which this image lacks), so each function is pinned by
 // the closed forms, invariants, symbolic derivations and brute-force checks in
 // tests/test_oracle_*.py; the orc_*_parts hooks expose scheme internals to them.
-// Parity unpinned (read from the reference's text, no mathematics fixes them):
+// Parity unpinned (read from the reference's text, no mathematics fixes it):
 //   - characteristic-wise interpolation: which cell's eigenvectors project the
-//     stencil (cell i for the left state, cell i+1 for the right);
-//   - the MDCD-WENO linear weights' slopes in (disp, diss) (3/2 and 9/2), beyond
-//     their consistency, central value, mirror symmetry and linearity.
+//     stencil (cell i for the left state, cell i+1 for the right).
+// The MDCD-WENO linear weights are pinned as the published ones (with the
+// finite-volume candidates they reproduce the six-point linear MDCD scheme).
 // AUSM is pinned as Liou's AUSM+-up (J. Comput. Phys. 214, 2006) with the
 // reference's choices: mean interface sound speed, M_inf^2 = 0.5, f_a on M_o^2,
 // beta = alpha = 0, the x velocity in the pressure diffusion.

@@ -18,7 +18,10 @@ MDCD hybrid (interpolation.cpp:135-209):
 - the linear weights: sum d_k = 1 for any (disp, diss), central
   (0, 1/2, 1/2, 0) at disp = diss = 0, mirror symmetric (d3(disp, diss) =
   d0(disp, -diss), d2(disp, diss) = d1(disp, -diss)) and linear in (disp,
-  diss).  Their slopes (3/2 and 9/2) are read from the reference's text:
+  diss).  Their slopes (3/2 and 9/2) are the published MDCD-WENO weights:
+  with the finite-volume (WENO-JS) candidates they reproduce the six-point
+  linear MDCD reconstruction identically (Sun et al. 2011), which has no
+  other solution.  With the WCNS candidates the reference uses,
   the smooth limit sum d_k q_k is then exact only through quadratics
   (cubic error -9 diss / 4), unlike the linear branch mdcd_linear (exact
   through cubics); the weights that would reproduce mdcd_linear are
@@ -190,6 +193,41 @@ def test_linear_weights_structure():
         d0 = mdcd_parts(f, 0.0, 0.0)["d"]
         assert np.allclose(d1 - d0, 2 * (d - d0), rtol=0, atol=1e-15)
     assert np.array_equal(mdcd_parts(f, 0.0, 0.0)["d"], [0.0, 0.5, 0.5, 0.0])
+
+
+def test_linear_weights_are_the_published_mdcd_weno_weights():
+    """The slopes of d_k in (disp, diss) are fixed by the scheme's definition
+    (Sun, Ren, Zhang, Jiang, J. Comput. Phys. 230 (2011) 4616): the MDCD-WENO
+    linear weights are those for which the four finite-volume (WENO-JS)
+    candidates, combined with them, give the six-point linear MDCD
+    reconstruction
+      (disp/2 + diss/2) f0 + (-3 disp/2 - 5 diss/2 - 1/12) f1 + (7/12 + disp + 5 diss) f2
+      + (7/12 + disp - 5 diss) f3 + (-3 disp/2 + 5 diss/2 - 1/12) f4 + (disp/2 - diss/2) f5
+    for every stencil and every (disp, diss).  That identity has the one
+    solution slopes (3/2, 9/2) among the consistent, central, mirror
+    symmetric linear weights (sympy, below), so checking it on the oracle's
+    d_k, exactly in rational arithmetic, pins them.  (The reference then pairs
+    them with the WCNS point-value candidates, which is the quirk recorded
+    with the smooth-limit test.)"""
+    R = sp.Rational
+    disp_s, diss_s, a, b = sp.symbols("disp diss a b")
+    q_fv = [(2 * F[0] - 7 * F[1] + 11 * F[2]) / 6, (-F[1] + 5 * F[2] + 2 * F[3]) / 6,
+            (2 * F[2] + 5 * F[3] - F[4]) / 6, (11 * F[3] - 7 * F[4] + 2 * F[5]) / 6]
+    linear = ((disp_s / 2 + diss_s / 2) * F[0] + (-R(3, 2) * disp_s - R(5, 2) * diss_s - R(1, 12)) * F[1]
+              + (R(7, 12) + disp_s + 5 * diss_s) * F[2] + (R(7, 12) + disp_s - 5 * diss_s) * F[3]
+              + (-R(3, 2) * disp_s + R(5, 2) * diss_s - R(1, 12)) * F[4] + (disp_s / 2 - diss_s / 2) * F[5])
+    family = [a * (disp_s + diss_s), R(1, 2) - a * disp_s + b * diss_s, R(1, 2) - a * disp_s - b * diss_s,
+              a * (disp_s - diss_s)]
+    eqs = sp.Poly(sp.expand(sum(w * q for w, q in zip(family, q_fv)) - linear), *F, disp_s, diss_s).coeffs()
+    assert sp.solve(eqs, [a, b], dict=True) == [{a: R(3, 2), b: R(9, 2)}]
+    f = np.arange(6.0)
+    for disp, diss in ((0.0463783, 0.02), (0.03125, 0.015625), (-0.0625, 0.0078125)):
+        d = [sp.nsimplify(float(x), rational=True) for x in mdcd_parts(f, disp, diss)["d"]]
+        dv, sv = sp.nsimplify(disp, rational=True), sp.nsimplify(diss, rational=True)
+        got = sp.expand(sum(w * q for w, q in zip(d, q_fv)))
+        want = sp.expand(linear.subs({disp_s: dv, diss_s: sv}))
+        for m in range(6):  # coefficient of each cell, to the weights' rounding
+            assert abs(float(got.coeff(F[m]) - want.coeff(F[m]))) < 1e-15, (disp, diss, m)
 
 
 def test_smooth_limit_order_of_the_branch():
