@@ -168,6 +168,7 @@ def mixed_report(args, prim, G, P, ms_plain, timed_run):
             "tke": float(f"{rep['tke']:.3e}"),
             "dissipation": float(f"{rep['dissipation']:.3e}"),
             "spectrum_max_shell": float(f"{rep['spectrum_max']:.3e}"),
+            "spectrum_max_shell_over": "shells k >= 1 holding at least 1e-6 of the largest shell energy",
             "spectrum_per_shell": [float(f"{e:.2e}") for e in rep["spectrum_per_shell"]],
         },
     }
@@ -411,6 +412,7 @@ def scale_mixed_report(args):
             "tke": rel(a["stats"]["tke"], b["stats"]["tke"]),
             "dissipation": rel(a["stats"]["dissipation"], b["stats"]["dissipation"]),
             "spectrum_max_shell": max(big) if big else 0.0,
+            "spectrum_max_shell_over": "shells k >= 1 holding at least 1e-6 of the largest shell energy",
             "spectrum_per_shell": shells,
         },
     }

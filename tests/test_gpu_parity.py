@@ -2,7 +2,15 @@
 
 Tolerances: index maps, conversions and the CFL time step are bit-exact;
 per-field RHS, face fluxes and one RK3 step within 1e-12 norm-wise relative
-(max|gpu - ref| / max|ref|), the FP64 parity bar for the plain method.
+(max|gpu - ref| / max|ref|), the FP64 parity bar for the plain method (a
+per-cell relative bound has no meaning where a field's RHS crosses zero).
+
+On a periodic box the half-node and viscous-flux exchanges move no data:
+the consuming kernels read periodic images by wrapped index, and
+download_face_flux fills the reference layout's ghost faces from them on
+the host.  test_face_flux therefore checks the ghost faces of the download;
+the device's wrapped reads are checked by every RHS, step and trajectory
+test below, whose results depend on them.
 """
 import itertools
 
