@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ns3d_device.h"
 
 namespace ns3d {
@@ -53,14 +55,19 @@ enum { VXX = 0, VYY, VZZ, VXY, VXZ, VYZ, VEX, VEY, VEZ, NVISC };
 // marches, with fewer cells of lead-in, go down to ADJ_SEG_MIN.
 constexpr int PENCIL_SEG_MIN = 32;
 constexpr int ADJ_SEG_MIN = 16;
+// Shortest segment of a box too small to give every CU a block with
+// segments of min_len cells.
+constexpr int SEG_FINE = 8;
 
 // Segments per column of a marching launch: the blocks the device holds at
 // once (blocks per CU at the kernel's occupancy x CUs) over the blocks of one
 // segment layer, rounded, so that the launch fills the resident slots a whole
 // number of times with the fewest segments (each segment repeats its
-// stencil's lead-in); segments of at least min_len cells.  E.g. 256^3: a
-// layer is 1024 blocks; two waves per SIMD hold 2048 blocks -> 2 segments of
-// 128, three hold 3072 -> 3 segments of 86.
+// stencil's lead-in); segments of at least min_len cells, or of at least
+// SEG_FINE cells as far as a box needs them to give every CU one block.
+// E.g. 256^3: a layer is 4 x 64 blocks of 256 threads; two blocks per CU
+// hold 512 -> 2 segments of 128 (three per CU: 3 of 86).  128^3: 4 segments
+// of 32 (one block per CU).  64^3: a layer is 16 blocks -> 8 segments of 8.
 inline int march_segments(int blocks_per_cu, long layer_blocks, int N, int min_len) {
     static const int cus = [] {
         int dev = 0, n = 0;
@@ -69,11 +76,14 @@ inline int march_segments(int blocks_per_cu, long layer_blocks, int N, int min_l
             n = 0;
         return n;
     }();
-    const int most = N / min_len > 1 ? N / min_len : 1;
-    if (cus <= 0 || blocks_per_cu <= 0 || layer_blocks <= 0) return (N + 63) / 64 < most ? (N + 63) / 64 : most;
+    const int coarse = N / min_len > 1 ? N / min_len : 1;
+    if (cus <= 0 || blocks_per_cu <= 0 || layer_blocks <= 0) return (N + 63) / 64 < coarse ? (N + 63) / 64 : coarse;
+    const long fine = N / SEG_FINE > 1 ? N / SEG_FINE : 1;
+    const long fill = (cus + layer_blocks - 1) / layer_blocks;  // segments for one block per CU
+    const long most = std::min(fine, std::max<long>(coarse, fill));
     long S = ((long)blocks_per_cu * cus + layer_blocks / 2) / layer_blocks;
     if (S < 1) S = 1;
-    return S < most ? (int)S : most;
+    return (int)std::min(S, most);
 }
 
 // Cells per segment for kernel K (256-thread blocks) marching columns of

@@ -15,6 +15,7 @@ struct ViscArgs {
     double i60[3];       // 1 / (60 h)
     double mu, S_ref, cp_pr;
     const float *Q32[4];  // mixed: compact FP32 u, v, w, T - 1 read instead of Q (else null)
+    int zseg;             // visc march: planes per z segment (set by the launch)
 };
 
 enum { ASM_INVISCID = 1, ASM_VISCOUS = 2, ASM_ACCUMULATE = 4 };

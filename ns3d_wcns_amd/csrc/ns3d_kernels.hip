@@ -51,8 +51,11 @@ __global__ void __launch_bounds__(256) k_visc(ViscArgs a) {
 // 64 x 8 tiles marching 32 planes: the plane tile with its halo is 70 x 14
 // cells per 512 outputs (1.91 loads per cell against 2.73 for 64 x 4), and
 // the 6-plane window prologue is paid once per 32 planes.  The FP64 kernel is
-// held to 128 registers, four waves per SIMD: two blocks per CU.
+// held to 128 registers, four waves per SIMD: two blocks per CU.  A box with
+// too few tiles per plane for two blocks per CU marches shorter segments,
+// down to VZ_SEG_MIN planes (64^3: 4 planes, 128 blocks instead of 16).
 constexpr int VZ_SEG = 32;
+constexpr int VZ_SEG_MIN = 4;
 constexpr int VT_Y = 8;      // tile rows (64 x VT_Y threads per block)
 constexpr int VT_WAVES = 4;  // minimum waves per SIMD asked of the register allocation
 
@@ -78,7 +81,7 @@ __global__ void __launch_bounds__(64 * VT_Y, VT_WAVES) k_visc_march(ViscArgs a) 
     // read); slots beyond the first three columns / rows past the box are
     // staged but never read
     auto per = [](int x, int n) { return ((x % n) + n) % n; };
-    const int k0 = blockIdx.z * VZ_SEG, k1 = min(k0 + VZ_SEG, b.nz);
+    const int k0 = blockIdx.z * a.zseg, k1 = min(k0 + a.zseg, b.nz);
     // plane stride and (i, j, 0) index of the primitives' layout
     const long sxy = Q32 ? (long)b.nx * b.ny : (long)b.sx * b.sy;
     auto at = [&](int x, int y) { return Q32 ? b.oidx(x, y, 0) : b.gidx(x, y, 0); };
@@ -495,8 +498,22 @@ static unsigned reduce_grid(long rows) { return (unsigned)(rows < REDUCE_BLOCKS 
 void launch_visc(const ViscArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_visc, tile_grid(a.b), cell_block(), 0, s, a);
 }
-void launch_visc_march(const ViscArgs &a, int precision, hipStream_t s) {
-    const dim3 g((a.b.nx + 63) / 64, (a.b.ny + VT_Y - 1) / VT_Y, (a.b.nz + VZ_SEG - 1) / VZ_SEG);
+void launch_visc_march(const ViscArgs &args, int precision, hipStream_t s) {
+    // VZ_SEG planes per segment where the box fills the device; smaller boxes
+    // take shorter segments (down to VZ_SEG_MIN) for the two blocks per CU
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            n = 0;
+        return n;
+    }();
+    ViscArgs a = args;
+    const long xy = (long)((a.b.nx + 63) / 64) * ((a.b.ny + VT_Y - 1) / VT_Y);
+    const long want = 2L * (cus > 0 ? cus : 256);
+    const long segs = (want + xy - 1) / xy;
+    a.zseg = (int)std::min<long>(VZ_SEG, std::max<long>(VZ_SEG_MIN, (a.b.nz + segs - 1) / segs));
+    const dim3 g((a.b.nx + 63) / 64, (a.b.ny + VT_Y - 1) / VT_Y, (a.b.nz + a.zseg - 1) / a.zseg);
     if (precision == 1 && a.Q32[0]) hipLaunchKernelGGL((k_visc_march<float, true>), g, dim3(64, VT_Y), 0, s, a);
     else if (precision == 1) hipLaunchKernelGGL(k_visc_march<float>, g, dim3(64, VT_Y), 0, s, a);
     else hipLaunchKernelGGL(k_visc_march<double>, g, dim3(64, VT_Y), 0, s, a);
