@@ -425,7 +425,9 @@ __device__ __forceinline__ double lin_at(const LinComb &l, int n, long g, double
 // transposed gradient stencil and the reverse of the conserved -> primitive
 // map added to the cell's face part acc, and the total added to out, or
 // handed to the tail's combinations.
-__device__ __forceinline__ void xline_tail(const AdjArgs &a, const LinTail &tl, int i, int j, int k, double *acc) {
+// Uc: the cell's state (read from the kernel's LDS copy of the line).
+__device__ __forceinline__ void xline_tail(const AdjArgs &a, const LinTail &tl, int i, int j, int k, double *acc,
+                                           const double *Uc) {
     const Box &b = a.b;
     const long g = b.gidx(i, j, k), o = b.oidx(i, j, k);
     double qb[4];
@@ -445,13 +447,9 @@ __device__ __forceinline__ void xline_tail(const AdjArgs &a, const LinTail &tl, 
             qb[q] -= sum * a.i60[d];
         }
     }
-    double U[5];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-        U[n] = a.U[n][g];
-        acc[n] += a.out[n][g];
-    }
-    c2p_vjp(a.gamma, a.Rgas, U, qb, acc);
+    for (int n = 0; n < 5; ++n) acc[n] += a.out[n][g];
+    c2p_vjp(a.gamma, a.Rgas, Uc, qb, acc);
     if (tl.n == 0) {
 #pragma unroll
         for (int n = 0; n < 5; ++n) a.out[n][g] = acc[n];
@@ -480,10 +478,11 @@ constexpr int XSEG = 256;
 // reconstructions of every cell; the reverse of the Riemann flux of every face (state adjoints
 // written over the states they replace: each slot is read and written by
 // one thread); per variable the reverse of the reconstructions into LDS and
-// the gather of the five contributions of each cell; then the viscous part
-// (transposed gradient stencil and the reverse of the conserved ->
-// primitive map), added to out; with a tail, the cell's total goes into the
-// tail's combinations instead of out.
+// the gather of the five contributions of each cell (two contribution
+// buffers, used in turn, so that one barrier per variable orders the
+// writes and reads); then the viscous part (transposed gradient stencil and
+// the reverse of the conserved -> primitive map), added to out; with a tail,
+// the cell's total goes into the tail's combinations instead of out.
 template <bool JS, int RS>
 __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a, LinTail tl) {
     extern __shared__ double lds[];
@@ -493,7 +492,7 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a, LinTail tl) {
     const int j = blockIdx.x % b.ny, k = blockIdx.x / b.ny;
     double *Ul = lds;           // [5][W]: cells -2..N+1
     double *A = Ul + 5 * W;     // [10][N]: left / right states, then their adjoints
-    double *Cc = A + 10 * N;    // [5][N]: contributions of one variable
+    double *Cc = A + 10 * N;    // [2][5][N]: contributions of one variable, buffer v % 2
     const long row = b.gidx(0, j, k) - NG;  // ghosted index of x = -NG
     for (int x = t; x < W; x += blockDim.x) {
         const long g = row + NG + wrap(x - 2, N);
@@ -532,25 +531,27 @@ __global__ void __launch_bounds__(512) k_adj_xline(AdjArgs a, LinTail tl) {
     double acc[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
+        // buffer v % 2 was last read (variable v - 2) before the previous barrier
+        double *C = Cc + (v & 1) * 5 * N;
         if (own) {
             const double f[5] = {Ul[v * W + t], Ul[v * W + t + 1], Ul[v * W + t + 2], Ul[v * W + t + 3],
                                  Ul[v * W + t + 4]};
             double fb[5] = {0, 0, 0, 0, 0};
             weno_pair_vjp<JS>(f, A[v * N + t], A[(5 + v) * N + t], fb);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) Cc[m * N + t] = fb[m];
+            for (int m = 0; m < 5; ++m) C[m * N + t] = fb[m];
         }
         __syncthreads();
         if (own) {
             double sum = 0.0;
 #pragma unroll
-            for (int m = 0; m < 5; ++m) sum += Cc[m * N + wrap(t + 2 - m, N)];
+            for (int m = 0; m < 5; ++m) sum += C[m * N + wrap(t + 2 - m, N)];
             acc[v] = sum;
         }
-        __syncthreads();  // Cc is rewritten by the next variable
     }
     if (!own) return;
-    xline_tail(a, tl, t, j, k, acc);
+    const double Uc[5] = {Ul[t + 2], Ul[W + t + 2], Ul[2 * W + t + 2], Ul[3 * W + t + 2], Ul[4 * W + t + 2]};
+    xline_tail(a, tl, t, j, k, acc, Uc);
 }
 
 // The x direction of the MDCD adjoint, for x lines of any length: one block
@@ -621,7 +622,8 @@ __global__ void __launch_bounds__(XSEG) k_adj_mdcd_x(AdjArgs a, LinTail tl, int 
         __syncthreads();  // Cc is rewritten by the next variable
     }
     if (t >= Sl) return;
-    xline_tail(a, tl, s0 + t, j, k, acc);
+    const double Uc[5] = {Ul[t + 5], Ul[W + t + 5], Ul[2 * W + t + 5], Ul[3 * W + t + 5], Ul[4 * W + t + 5]};
+    xline_tail(a, tl, s0 + t, j, k, acc, Uc);
 }
 
 // The x direction for lines longer than XSEG cells: one block of XSEG
@@ -703,7 +705,8 @@ __global__ void __launch_bounds__(XSEG) k_adj_xseg(AdjArgs a, LinTail tl, int ns
         __syncthreads();  // Cc is rewritten by the next variable
     }
     if (t >= Sl) return;
-    xline_tail(a, tl, s0 + t, j, k, acc);
+    const double Uc[5] = {Ul[t + 5], Ul[W + t + 5], Ul[2 * W + t + 5], Ul[3 * W + t + 5], Ul[4 * W + t + 5]};
+    xline_tail(a, tl, s0 + t, j, k, acc, Uc);
 }
 
 // The face and viscous adjoints of one adjoint RHS, as two z-marching
@@ -1004,7 +1007,7 @@ void adj_x_t(const AdjArgs &a, hipStream_t s, const LinTail &t) {
         return;
     }
     const dim3 g((unsigned)((long)a.b.ny * a.b.nz)), bl((unsigned)(((N + 63) / 64) * 64));
-    const size_t shm = (size_t)(5 * (N + 4) + 15 * N) * sizeof(double);
+    const size_t shm = (size_t)(5 * (N + 4) + 20 * N) * sizeof(double);
     hipLaunchKernelGGL((k_adj_xline<JS, RS>), g, bl, shm, s, a, t);
 }
 
