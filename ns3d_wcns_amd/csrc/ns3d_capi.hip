@@ -56,6 +56,7 @@ struct ns3d_ctx {
     double *xg[5];   // ghosted copy of the tangent / adjoint input, ghosts filled
     double *dq[4];   // ghosted (u, v, w, T) tangents
     double *s1[5];   // stage-1 state of the step being differentiated
+    double *q2[4];   // ghosted u, v, w, T of its stage-2 state (adjoint step)
     double *a0[5], *a1[5], *pr[5];  // stage accumulators and the RHS derivative
     double *cw[25];  // compact: face adjoints and state adjoints, or face and viscous tangents
     double *gb[12], *qb[4];
@@ -387,12 +388,17 @@ int halo(ns3d_ctx *c, bool all) {
 }
 
 // State arrays of a fused stage other than the context's own: the stage reads
-// Uin (its primitives in the context), writes Uout and takes U0 as the
-// recorded state (stage 1 without record, stages 2 and 3).
+// Uin (its primitives in the context, or Qin when set), writes Uout (its
+// primitives into the context's, or into Qout when set) and takes U0 as the
+// recorded state (stage 1 without record, stages 2 and 3).  Qin / Qout hold
+// u, v, w, p, T in slots 0..4; slot 3 (p) is neither read nor written by
+// stages 1 and 2 and may be null.
 struct StageIO {
     const double *const *Uin;
     double *const *Uout;
     double *const *U0;
+    const double *const *Qin = nullptr;
+    double *const *Qout = nullptr;
 };
 
 // One fused stage (stage 0: RHS only): viscous fluxes, then the z, y and x
@@ -407,6 +413,10 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     const bool mixed = c->precision == NS3D_PRECISION_MIXED;
     {
         ViscArgs va = visc_args(c, false, true);
+        if (io && io->Qin) {
+            const int slot[4] = {0, 1, 2, 4};
+            for (int q = 0; q < 4; ++q) va.Q[q] = io->Qin[slot[q]];
+        }
         for (int q = 0; q < 4; ++q) va.Q32[q] = mixed && c->prims_fp32 ? prims32(c, q) : nullptr;
         Timed t(c, NS3D_K_VISCOUS);
         launch_visc_march(va, c->precision, c->stream);
@@ -419,7 +429,7 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
         a.RHS[n] = rhs(c)[n];
         a.U0[n] = io ? io->U0[n] : recorded(c)[n];
         a.Unew[n] = io ? io->Uout[n] : conserved(c)[n];  // in place by default (the x pencil owns its line)
-        a.Q[n] = prims(c)[n];
+        a.Q[n] = io && io->Qout ? io->Qout[n] : prims(c)[n];
         a.Rz[n] = c->rz[n];
         a.Ry[n] = c->ry[n];
     }
@@ -731,7 +741,7 @@ int ensure_grad(ns3d_ctx *c) {
 int ensure_adj(ns3d_ctx *c) {
     if (c->adj_slab) return NS3D_OK;
     const size_t cs = field_stride(c->ncell), is = field_stride(c->nint);
-    constexpr size_t kGhosted = 5 + 4 + 5 + 5 + 5 + 5;  // xg, dq, s1, a0, a1, pr
+    constexpr size_t kGhosted = 5 + 4 + 5 + 4 + 5 + 5 + 5;  // xg, dq, s1, q2, a0, a1, pr
     constexpr size_t kCompact = 25 + 12 + 4;            // cw, gb, qb
     const size_t total = kGhosted * cs + kCompact * is;
     if (hipMalloc(&c->adj_slab, total * sizeof(double)) != hipSuccess) {
@@ -743,6 +753,7 @@ int ensure_adj(ns3d_ctx *c) {
     for (int n = 0; n < 5; ++n, p += cs) c->xg[n] = p;
     for (int n = 0; n < 4; ++n, p += cs) c->dq[n] = p;
     for (int n = 0; n < 5; ++n, p += cs) c->s1[n] = p;
+    for (int n = 0; n < 4; ++n, p += cs) c->q2[n] = p;
     for (int n = 0; n < 5; ++n, p += cs) c->a0[n] = p;
     for (int n = 0; n < 5; ++n, p += cs) c->a1[n] = p;
     for (int n = 0; n < 5; ++n, p += cs) c->pr[n] = p;
@@ -856,12 +867,15 @@ int tangent_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out) {
 // primitives the context holds.
 // With a tail, the RHS is not stored: its value at each cell goes into the
 // tail's combinations.
+// U, Q: the state and its primitives (slots 0..4; p is not read) when not
+// the context's.
 int adjoint_rhs_impl(ns3d_ctx *c, const double *const *X, double *const *out, const double *const *U = nullptr,
-                     const LinTail *tail = nullptr) {
+                     const LinTail *tail = nullptr, const double *const *Q = nullptr) {
     AdjArgs a = adj_args(c, out);
     for (int n = 0; n < 5; ++n) {
         a.X[n] = X[n];
         if (U) a.U[n] = U[n];
+        if (Q) a.Q[n] = Q[n];
     }
     LinTail t{};
     if (tail) t = *tail;
@@ -916,13 +930,17 @@ int check_adjoint(ns3d_ctx *c) {
     return NS3D_OK;
 }
 
-// The reverse of one RK3 step: the context holds U^n, untouched, and the
-// primitives of U^(2); U1, U2: U^(1), U^(2) (interior cells are read; every
-// adjoint kernel reads periodic images, not ghost layers).  adj: Ub^3 on
-// entry, Ub^n on return (ghosts zero); the context's primitives are left
-// those of U^n, so the context is at U^n as on entry.
+// The reverse of one RK3 step: the context holds U^n, untouched; U1, U2:
+// U^(1), U^(2) (interior cells are read; every adjoint kernel reads periodic
+// images, not ghost layers).  Without Q1 / Q2 the context holds the
+// primitives of U^(2) and the reverse forms those of U^(1) and U^n itself;
+// with them (the primitives of U^(1) and U^(2) as the forward stages wrote
+// them) the context holds the primitives of U^n.  adj: Ub^3 on entry, Ub^n on
+// return (ghosts zero); the context's primitives are left those of U^n, so
+// the context is at U^n as on entry.
 int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1,
-                    const double *const *U2) {
+                    const double *const *U2, const double *const *Q1 = nullptr,
+                    const double *const *Q2 = nullptr) {
     int rc;
     // Each adjoint RHS P hands its value to the stage combinations in its
     // last kernel (the P operand marks where it enters).
@@ -931,15 +949,18 @@ int adjoint_reverse(ns3d_ctx *c, double dt, double *const *adj, const double *co
     t.n = 2;
     t.c[0] = lc(c, c->a1, 2.0 / 3.0, adj, 0.0, nullptr, 2.0 / 3.0 * dt, c->pr);
     t.c[1] = lc(c, c->a0, 1.0 / 3.0, adj, 0.0, nullptr, 0.0, nullptr);
-    if ((rc = adjoint_rhs_impl(c, adj, c->pr, U2, &t))) return rc;
+    if ((rc = adjoint_rhs_impl(c, adj, c->pr, U2, &t, Q2))) return rc;
     // stage 2 at U^(1): Ub0 += 3/4 Ub2, Ub1 = 1/4 (Ub2 + dt P)
     t.c[0] = lc(c, c->a0, 1.0, c->a0, 0.75, c->a1, 0.0, nullptr);
     t.c[1] = lc(c, c->a1, 0.25, c->a1, 0.0, nullptr, 0.25 * dt, c->pr);
-    if ((rc = prims_from(c, U1)) || (rc = adjoint_rhs_impl(c, c->a1, c->pr, U1, &t))) return rc;
+    if (!Q1 && (rc = prims_from(c, U1))) return rc;
+    if ((rc = adjoint_rhs_impl(c, c->a1, c->pr, U1, &t, Q1))) return rc;
     // stage 1 at U^n: Ub^n = Ub0 + Ub1 + dt P (the primitives of U^n over
     // every cell, ghosts included, as the context had them on entry)
-    launch_c2p(conv_args(c), c->stream);
-    NS3D_LAUNCHED("c2p");
+    if (!Q1) {
+        launch_c2p(conv_args(c), c->stream);
+        NS3D_LAUNCHED("c2p");
+    }
     t.n = 1;
     t.c[0] = lc(c, adj, 1.0, c->a0, 1.0, c->a1, dt, c->pr);
     if ((rc = adjoint_rhs_impl(c, c->a1, c->pr, nullptr, &t))) return rc;
@@ -1037,14 +1058,19 @@ int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
     if ((rc = check_adjoint(c)) || (rc = ensure_adj(c))) return rc;
     launch_c2p(conv_args(c), c->stream);  // the primitives of U^n
     NS3D_LAUNCHED("c2p");
-    // forward stages 1 and 2 from U^n, which stays in the context: U^(1) into
-    // s1, U^(2) into xg (free during a step's adjoint); the ghost layers of
-    // the stage states are not formed (nothing reads them)
-    const StageIO st1{conserved(c), c->s1, conserved(c)}, st2{c->s1, c->xg, conserved(c)};
+    // forward stages 1 and 2 from U^n, which stays in the context with its
+    // primitives: U^(1) into s1 with its u, v, w, T into dq (the tangent's,
+    // free during an adjoint), U^(2) into xg (free during a step's adjoint)
+    // with its u, v, w, T into q2; the reverse takes those primitives as they
+    // are.  The ghost layers of the stage states are not formed (nothing
+    // reads them).
+    double *q1[5] = {c->dq[0], c->dq[1], c->dq[2], nullptr, c->dq[3]};
+    double *q2[5] = {c->q2[0], c->q2[1], c->q2[2], nullptr, c->q2[3]};
+    const StageIO st1{conserved(c), c->s1, conserved(c), nullptr, q1}, st2{c->s1, c->xg, conserved(c), q1, q2};
     if ((rc = fused_stage(c, 1, nullptr, dt, false, false, &st1)) ||
         (rc = fused_stage(c, 2, nullptr, dt, false, false, &st2)))
         return rc;
-    return adjoint_reverse(c, dt, adj, c->s1, c->xg);
+    return adjoint_reverse(c, dt, adj, c->s1, c->xg, q1, q2);
 }
 
 int ns3d_step_stages(ns3d_ctx *c, double dt, double *const *U1, double *const *U2) {
