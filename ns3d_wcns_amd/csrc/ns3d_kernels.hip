@@ -374,34 +374,40 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// The block's maximum CFL speed over the interior cells, flat over the box
-// with four cells of each thread in flight (a maximum does not depend on the
-// order of the cells).
+// The block's maximum CFL speed over the interior cells: block-strided rows
+// (y, z) of the interior with the threads along x, four rows of each thread
+// in flight (a maximum does not depend on the order of the cells).  The row
+// index is uniform over the block, so no per-cell index division is formed.
 __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
     const Box &b = a.b;
-    const long n = (long)b.nx * b.ny * b.nz, stride = (long)gridDim.x * 256;
+    const int rows = b.ny * b.nz, G = (int)gridDim.x;
     double s = 0.0;
-    for (long id0 = (long)blockIdx.x * 256 + threadIdx.x; id0 < n; id0 += 4 * stride) {
-        double rho[4], rhou[4], E[4];
+    for (int r0 = blockIdx.x; r0 < rows; r0 += 4 * G) {
+        long row[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const long id = id0 + u * stride;
-            rho[u] = rhou[u] = E[u] = 0.0;
-            if (id < n) {
-                const int i = (int)(id % b.nx);
-                const long r = id / b.nx;
-                const long c = b.gidx(i, (int)(r % b.ny), (int)(r / b.ny));
-                rho[u] = a.U[0][c];
-                rhou[u] = a.U[1][c];
-                E[u] = a.U[4][c];
-            }
+            const int r = r0 + u * G;
+            row[u] = r < rows ? b.gidx(0, r % b.ny, r / b.ny) : -1;
         }
+        for (int x = threadIdx.x; x < b.nx; x += 256) {
+            double rho[4], rhou[4], E[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (id0 + u * stride < n) {
-                const double v = cfl_speed(rho[u], rhou[u], E[u], a.gamma);
-                if (v == v) s = fmax(s, v);  // skip NaN
+            for (int u = 0; u < 4; ++u) {
+                rho[u] = rhou[u] = E[u] = 0.0;
+                if (row[u] >= 0) {
+                    const long c = row[u] + x;
+                    rho[u] = a.U[0][c];
+                    rhou[u] = a.U[1][c];
+                    E[u] = a.U[4][c];
+                }
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (row[u] >= 0) {
+                    const double v = cfl_speed(rho[u], rhou[u], E[u], a.gamma);
+                    if (v == v) s = fmax(s, v);  // skip NaN
+                }
+        }
     }
     s = wave_max(s);
     __shared__ double part[4];
@@ -537,7 +543,8 @@ void launch_halo(const HaloArgs &h, hipStream_t s) {
     const long m = (long)std::max(std::max(h.b.ny * h.b.nz, h.b.nx * h.b.nz), h.b.nx * h.b.ny) * 2 * NG;
     hipLaunchKernelGGL(k_halo, dim3((unsigned)((m + 255) / 256), 3), dim3(256), 0, s, h);
 }
-long cfl_block_count(const Box &b) { return reduce_grid(((long)b.nx * b.ny * b.nz + 1023) / 1024); }
+// blocks of the first CFL pass: one per four interior rows, at most REDUCE_BLOCKS
+long cfl_block_count(const Box &b) { return reduce_grid(((long)b.ny * b.nz + 3) / 4); }
 void launch_cfl(const CflArgs &a, hipStream_t s) {
     const unsigned g = (unsigned)cfl_block_count(a.b);
     hipLaunchKernelGGL(k_cfl, dim3(g), dim3(256), 0, s, a);
