@@ -56,10 +56,15 @@ def test_initial_state_bit_exact(gpu):
     dev.close()
 
 
-@pytest.mark.parametrize("seed", [1, 2, 3])
-def test_timestep_bit_exact(gpu, seed):
-    prim, ma = state(SMALL, 0.6, noise=0.02, seed=seed)
-    dev, ref = pair(SMALL, params(ma), prim)
+@pytest.mark.parametrize("seed,n", [(1, SMALL), (2, SMALL), (3, SMALL), (4, (64, 72, 66)), (5, (600, 6, 8)),
+                                    (6, (520, 6, 6))],
+                         ids=["small-1", "small-2", "small-3", "64x72x66", "x-line-600", "x-line-520"])
+def test_timestep_bit_exact(gpu, seed, n):
+    """dt of the device's row-wise maximum equals the checker's: rows shorter
+    than a block, several blocks of rows, rows longer than a block (a thread
+    takes two or three cells of its row)."""
+    prim, ma = state(n, 0.6, noise=0.02, seed=seed)
+    dev, ref = pair(n, params(ma), prim)
     assert dev.compute_timestep() == ref.compute_timestep()
     dev.close()
 
