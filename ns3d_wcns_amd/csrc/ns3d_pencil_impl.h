@@ -131,12 +131,11 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o
     __builtin_nontemporal_store(T, &a.Q[4][g]);
 }
 
-// FP64 component-wise WCNS5 / WENO5-JS: the marches form both
-// reconstructions of a cell at once (recon_pair).  The FP32 faces of the
-// mixed method keep the per-face form, which measured faster at three waves
-// per SIMD.
+// Component-wise WCNS5 / WENO5-JS: the marches form both reconstructions of a
+// cell at once (recon_pair), in FP64 and in the FP32 faces of the mixed
+// method (which then still fits three waves per SIMD without scratch).
 template <int INTERP, bool CHAR, class R>
-constexpr bool kPairRecon = sizeof(R) == 8 && !CHAR && (INTERP == I_WENO5 || INTERP == I_WENO5_JS);
+constexpr bool kPairRecon = !CHAR && (INTERP == I_WENO5 || INTERP == I_WENO5_JS);
 
 // The x pencil forms both reconstructions of a cell at once for
 // component-wise WCNS5 / WENO5-JS when one thread owns one face and one cell.
