@@ -70,7 +70,8 @@ struct ns3d_ctx {
     bool visc_fp32;  // visc[] holds the mixed method's FP32 fluxes (last writer: a mixed fused stage)
     // the primitives of the state are the compact FP32 u, v, w, T - 1 of
     // prims32(): only between the stages of one rk3 / step_stages call of
-    // the mixed method (stage 3 stores the FP64 primitives)
+    // the mixed method, and between the steps of one advance_steps call
+    // (the last stage that call runs stores the FP64 primitives)
     bool prims_fp32;
     // a face is not periodic: the stepwise path of the bounded box runs
     // (ns3d_bounded.h), with the reference-layout face arrays fref
@@ -405,9 +406,10 @@ struct StageIO {
 // pencils; stages 1..3 update the state in place, or from io->Uin into
 // io->Uout.  The kernels read no ghost layer (periodic images instead); with
 // ghosts the layers of the new state (the context's) are refilled for the
-// callers that read them.  q32 (mixed method, stage 1 or 2, set only by
-// callers whose next step is the following stage): the x pencil hands the
-// primitives to the next stage's viscous kernel in FP32.
+// callers that read them.  q32 (mixed method, set only by callers whose next
+// step is another fused stage of the same state: stages 1 and 2 of a step,
+// and stage 3 of a device time loop's steps other than the last): the x
+// pencil hands the primitives to the next stage's viscous kernel in FP32.
 int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false, bool ghosts = true,
                 const StageIO *io = nullptr, bool q32 = false) {
     const bool mixed = c->precision == NS3D_PRECISION_MIXED;
@@ -445,7 +447,7 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     a.dt = d_dt;
     a.dtv = dt;
     a.stage = stage;
-    for (int q = 0; q < 4; ++q) a.Q32[q] = q32 && mixed && (stage == 1 || stage == 2) ? prims32(c, q) : nullptr;
+    for (int q = 0; q < 4; ++q) a.Q32[q] = q32 && mixed && stage >= 1 ? prims32(c, q) : nullptr;
     const SchemeCoef<double> c64 = coef64(c->P);
     const SchemeCoef<float> c32 = coef32(c->P);
     const int interp = c->P.interpolation;
@@ -496,18 +498,21 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
 // refills the ghost layers, and only with ghosts set: the device time loops
 // run their steps without (no kernel of a step reads a ghost layer) and
 // refill once before they return.  The mixed method hands stages 1 and 2's
-// primitives to the next viscous kernel in FP32.
+// primitives to the next viscous kernel in FP32, and stage 3's too with
+// q32_last (a device time loop's step that another step follows).
 int record0(ns3d_ctx *c);
 int bnd_rk3(ns3d_ctx *c, const double *d_dt, double dt);
 
-int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool ghosts = true) {
+int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool ghosts = true,
+        bool q32_last = false) {
     int rc;
     if (c->bounded) {
         if (record && (rc = record0(c))) return rc;
         return bnd_rk3(c, d_dt, dt);
     }
     for (int stage = 1; stage <= 3; ++stage)
-        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, ghosts && stage == 3, nullptr, stage < 3))) {
+        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, ghosts && stage == 3, nullptr,
+                              stage < 3 || q32_last))) {
             c->prims_fp32 = false;
             return rc;
         }
@@ -1543,9 +1548,11 @@ int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
     for (int s = 0; s < nsteps; ++s) {
         double *d_dt = d_dt_hist ? d_dt_hist + s : c->scal;
         // recordConservedTo0 fused into the first stage; ghost layers
-        // refilled after the last step only
+        // refilled after the last step only, which alone hands out FP64
+        // primitives (the mixed method's steps before it pass theirs on in
+        // FP32; the CFL reduction reads the conserved state)
         if ((rc = cfl(c, d_dt))) return rc;
-        if ((rc = rk3(c, d_dt, 0.0, true, s == nsteps - 1))) return rc;
+        if ((rc = rk3(c, d_dt, 0.0, true, s == nsteps - 1, s < nsteps - 1))) return rc;
     }
     return NS3D_OK;
 }

@@ -112,10 +112,11 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o
     // which stops after stage 2, recomputes its primitives by C2P).
     const double T = p / (rr * a.Rgas);
     if constexpr (kPert<R>) {
-        // mixed stages 1 and 2: the next viscous kernel reads u, v, w and
-        // T - T_ref (T_ref = 1) in FP32, which it would round from these
-        // same FP64 values; stored compact as FP32 at compact cell o (stage
-        // 3 stores the FP64 primitives that are handed out)
+        // mixed stages whose next kernel is another stage's viscous kernel:
+        // it reads u, v, w and T - T_ref (T_ref = 1) in FP32, which it would
+        // round from these same FP64 values; stored compact as FP32 at
+        // compact cell o (a stage whose primitives are handed out stores
+        // them in FP64)
         if (a.Q32[0]) {
             __builtin_nontemporal_store((float)u, a.Q32[0] + o);
             __builtin_nontemporal_store((float)v, a.Q32[1] + o);
