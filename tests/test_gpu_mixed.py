@@ -134,8 +134,12 @@ def test_mixed_viscous_buffer_guard(gpu):
     dev.close()
 
 
-def test_mixed_time_advance_matches_advance_steps(gpu):
-    n = 16
+@pytest.mark.parametrize("n", [16, (72, 40, 66)], ids=["16", "72x40x66"])
+def test_mixed_time_advance_matches_advance_steps(gpu, n):
+    """time_advance hands each step's primitives on in FP64 (stage 3), while
+    advance_steps hands them to the next step's viscous kernel in FP32 (all
+    but the last step): the states agree bit for bit, on one tile and on a
+    non-cubic box of several viscous tiles and marching segments."""
     prim, ma = state(n, 0.3, seed=8)
     P = params(ma)
     G = box(n)
