@@ -215,7 +215,10 @@ int ns3d_time_advance(ns3d_ctx *ctx, int max_steps, double dt_fixed, double tota
 /* nsteps of (record U0, CFL dt, RK3) entirely on the device; d_dt_hist (device,
  * nsteps doubles, may be NULL) receives each step's dt.  No clipping: the
  * caller's total time and output interval must lie beyond the run.  Ghost
- * layers as for time_advance. */
+ * layers as for time_advance.  Afterwards U0 holds the state at the start of
+ * the last step, as after time_advance; the steps before it keep that state
+ * in place as their U0 and pass their stage states through the RHS arrays,
+ * whose contents are unspecified afterwards. */
 int ns3d_advance_steps(ns3d_ctx *ctx, int nsteps, double *d_dt_hist);
 
 /* ---- observables (postprocess.cpp:103-128) ---- */

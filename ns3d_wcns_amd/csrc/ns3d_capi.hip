@@ -391,7 +391,8 @@ int halo(ns3d_ctx *c, bool all) {
 // State arrays of a fused stage other than the context's own: the stage reads
 // Uin (its primitives in the context, or Qin when set), writes Uout (its
 // primitives into the context's, or into Qout when set) and takes U0 as the
-// recorded state (stage 1 without record, stages 2 and 3).  Qin / Qout hold
+// recorded state (stage 1 with U0_READ, stages 2 and 3; a stage 3 may write
+// Uout = U0: each thread reads U0 of a cell before it writes that cell).  Qin / Qout hold
 // u, v, w, p, T in slots 0..4; slot 3 (p) is neither read nor written by
 // stages 1 and 2 and may be null.
 struct StageIO {
@@ -410,7 +411,7 @@ struct StageIO {
 // step is another fused stage of the same state: stages 1 and 2 of a step,
 // and stage 3 of a device time loop's steps other than the last): the x
 // pencil hands the primitives to the next stage's viscous kernel in FP32.
-int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool record = false, bool ghosts = true,
+int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, int u0mode = U0_READ, bool ghosts = true,
                 const StageIO *io = nullptr, bool q32 = false) {
     const bool mixed = c->precision == NS3D_PRECISION_MIXED;
     {
@@ -438,7 +439,7 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
     for (int n = 0; n < NVISC; ++n) a.V[n] = c->visc[n];
     a.b = c->b;
     a.ref = mixed_ref(c);
-    a.record = record ? 1 : 0;
+    a.record = u0mode;
     a.ihd[0] = 1.0 / c->grid.dx;
     a.ihd[1] = 1.0 / c->grid.dy;
     a.ihd[2] = 1.0 / c->grid.dz;
@@ -500,22 +501,36 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, bool reco
 // refill once before they return.  The mixed method hands stages 1 and 2's
 // primitives to the next viscous kernel in FP32, and stage 3's too with
 // q32_last (a device time loop's step that another step follows).
+//
+// through (a device time loop's step that another step follows, in place of
+// record): U^n stays in the state arrays until stage 3 and is itself the
+// stages' U0, so no U0 copy is stored.  Stage 1 reads U^n and writes U^(1)
+// into the RHS arrays (which no kernel of the device loops reads otherwise),
+// stage 2 updates them in place to U^(2), and stage 3 reads U^(2) there and
+// writes U^(n+1) over U^n, each cell's U0 read before that cell is written.
+// The results are those of the recorded step; the U0 arrays keep their
+// previous contents and the RHS arrays hold U^(2) afterwards.
 int record0(ns3d_ctx *c);
 int bnd_rk3(ns3d_ctx *c, const double *d_dt, double dt);
 
 int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool ghosts = true,
-        bool q32_last = false) {
+        bool q32_last = false, bool through = false) {
     int rc;
     if (c->bounded) {
-        if (record && (rc = record0(c))) return rc;
+        if ((record || through) && (rc = record0(c))) return rc;
         return bnd_rk3(c, d_dt, dt);
     }
-    for (int stage = 1; stage <= 3; ++stage)
-        if ((rc = fused_stage(c, stage, d_dt, dt, record && stage == 1, ghosts && stage == 3, nullptr,
+    double *const *A = conserved(c);
+    double *const *B = rhs(c);
+    const StageIO io[3] = {{A, B, A}, {B, B, A}, {B, A, A}};
+    for (int stage = 1; stage <= 3; ++stage) {
+        const int u0mode = stage > 1 ? U0_READ : through ? U0_INPUT : record ? U0_RECORD : U0_READ;
+        if ((rc = fused_stage(c, stage, d_dt, dt, u0mode, ghosts && stage == 3, through ? &io[stage - 1] : nullptr,
                               stage < 3 || q32_last))) {
             c->prims_fp32 = false;
             return rc;
         }
+    }
     return NS3D_OK;
 }
 
@@ -1072,8 +1087,8 @@ int ns3d_adjoint_step(ns3d_ctx *c, double dt, double *const *adj) {
     double *q1[5] = {c->dq[0], c->dq[1], c->dq[2], nullptr, c->dq[3]};
     double *q2[5] = {c->q2[0], c->q2[1], c->q2[2], nullptr, c->q2[3]};
     const StageIO st1{conserved(c), c->s1, conserved(c), nullptr, q1}, st2{c->s1, c->xg, conserved(c), q1, q2};
-    if ((rc = fused_stage(c, 1, nullptr, dt, false, false, &st1)) ||
-        (rc = fused_stage(c, 2, nullptr, dt, false, false, &st2)))
+    if ((rc = fused_stage(c, 1, nullptr, dt, U0_INPUT, false, &st1)) ||
+        (rc = fused_stage(c, 2, nullptr, dt, U0_READ, false, &st2)))
         return rc;
     return adjoint_reverse(c, dt, adj, c->s1, c->xg, q1, q2);
 }
@@ -1086,12 +1101,12 @@ int ns3d_step_stages(ns3d_ctx *c, double dt, double *const *U1, double *const *U
     // the stages written straight into the caller's arrays (interior cells),
     // U0 recorded by the stage-1 x pencil
     const StageIO st1{conserved(c), U1, recorded(c)}, st2{U1, U2, recorded(c)}, st3{U2, conserved(c), recorded(c)};
-    if ((rc = fused_stage(c, 1, nullptr, dt, true, false, &st1, true)) ||
-        (rc = fused_stage(c, 2, nullptr, dt, false, false, &st2, true))) {
+    if ((rc = fused_stage(c, 1, nullptr, dt, U0_RECORD, false, &st1, true)) ||
+        (rc = fused_stage(c, 2, nullptr, dt, U0_READ, false, &st2, true))) {
         c->prims_fp32 = false;
         return rc;
     }
-    return fused_stage(c, 3, nullptr, dt, false, true, &st3);
+    return fused_stage(c, 3, nullptr, dt, U0_READ, true, &st3);
 }
 
 int ns3d_adjoint_step_stages(ns3d_ctx *c, double dt, double *const *adj, const double *const *U1,
@@ -1547,12 +1562,14 @@ int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
     int rc;
     for (int s = 0; s < nsteps; ++s) {
         double *d_dt = d_dt_hist ? d_dt_hist + s : c->scal;
-        // recordConservedTo0 fused into the first stage; ghost layers
-        // refilled after the last step only, which alone hands out FP64
-        // primitives (the mixed method's steps before it pass theirs on in
-        // FP32; the CFL reduction reads the conserved state)
+        // recordConservedTo0 fused into the first stage of the last step;
+        // the steps before it keep U^n in place as their U0 and store no
+        // copy.  Ghost layers refilled after the last step only, which alone
+        // hands out FP64 primitives (the mixed method's steps before it pass
+        // theirs on in FP32; the CFL reduction reads the conserved state)
+        const bool last = s == nsteps - 1;
         if ((rc = cfl(c, d_dt))) return rc;
-        if ((rc = rk3(c, d_dt, 0.0, true, s == nsteps - 1, s < nsteps - 1))) return rc;
+        if ((rc = rk3(c, d_dt, 0.0, last, last, !last, !last))) return rc;
     }
     return NS3D_OK;
 }

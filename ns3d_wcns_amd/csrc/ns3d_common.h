@@ -100,6 +100,12 @@ inline int march_seg(long layer_blocks, int N, int min_len = PENCIL_SEG_MIN) {
     return (N + S - 1) / S;
 }
 
+// The recorded state U0 of stage 1 (U1 = U0 + dt RHS): U0_READ loads the
+// U0 arrays; U0_RECORD takes the stage's own state and also stores it as U0
+// for the interior (recordConservedTo0 fused into the stage); U0_INPUT takes
+// the stage's own state and stores nothing.  Stages 2 and 3 always load U0.
+enum { U0_READ = 0, U0_RECORD = 1, U0_INPUT = 2 };
+
 // Arguments of the fused pencil kernels of one stage.
 struct PencilArgs {
     const double *U[5];       // ghosted current state
@@ -117,7 +123,7 @@ struct PencilArgs {
     const double *dt;         // device scalar, or nullptr to use dtv
     double dtv;
     int stage;                // 0: store the RHS; 1..3: RK3 stage
-    int record;               // stage 1: also store U0 = U (recordConservedTo0) for the interior
+    int record;               // stage 1: where U0 comes from (U0_READ / U0_RECORD / U0_INPUT)
     int seg;                  // y / z segment length
     int line_threads;         // x pencil: threads per line
     float *Q32[4];            // mixed, stages 1-2: compact FP32 u, v, w, T - 1 written instead of Q (else null)

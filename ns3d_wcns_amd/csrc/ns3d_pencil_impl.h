@@ -147,7 +147,7 @@ constexpr bool kPairReconX = !CHAR && (INTERP == I_WENO5 || INTERP == I_WENO5_JS
 // Last of the three: adds the x part to the partial RHS of the z and y
 // pencils and stores the RHS (stage 0) or updates the line in place (stage
 // 1..3; the block holds its whole line in LDS and no other block reads it).
-// With a.record at stage 1 the current line is also stored as U0.
+// With a.record == U0_RECORD at stage 1 the current line is also stored as U0.
 // ONE: every thread owns exactly one face and one cell (N == threads per
 // line); its partial RHS and U0 are then loaded into registers before the
 // face work, so that their latency is hidden by it.
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
     double *Vl = Fl + 5 * N;
     const long row = ((long)(k + NG) * b.sy + (j + NG)) * b.sx;
     const long orow = b.oidx(0, j, k);
-    const bool need_u0 = a.stage > 1 || (a.stage == 1 && !a.record);
+    const bool need_u0 = a.stage > 1 || (a.stage == 1 && a.record == U0_READ);
     double sp[5], zp[5];
     if constexpr (ONE) {
         if (valid) {
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
 #pragma unroll
             for (int n = 0; n < 5; ++n) a.RHS[n][g] = r[n];
         } else {
-            if (a.stage == 1 && a.record) {
+            if (a.stage == 1 && a.record == U0_RECORD) {
 #pragma unroll
                 for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
             }
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(512) k_pencil_xp(PencilArgs a, SchemeCoef<R> c
     double *Xl = Fl + 5 * NF;
     const long row = ((long)(k + NG) * b.sy + (j + NG)) * b.sx;
     const long orow = b.oidx(0, j, k);
-    const bool need_u0 = a.stage > 1 || (a.stage == 1 && !a.record);
+    const bool need_u0 = a.stage > 1 || (a.stage == 1 && a.record == U0_READ);
     double sp[5], zp[5];
     if (valid) {
 #pragma unroll
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(512) k_pencil_xp(PencilArgs a, SchemeCoef<R> c
 #pragma unroll
         for (int n = 0; n < 5; ++n) a.RHS[n][g] = r[n];
     } else {
-        if (a.stage == 1 && a.record) {
+        if (a.stage == 1 && a.record == U0_RECORD) {
 #pragma unroll
             for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
         }
