@@ -132,15 +132,6 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o
     __builtin_nontemporal_store(T, &a.Q[4][g]);
 }
 
-// The y / z marches stage the window's next cell in LDS one iteration ahead
-// (global-to-LDS loads); NS3D_MARCH_DMA=0 at build time loads it in the
-// iteration that uses it instead.
-#ifndef NS3D_MARCH_DMA
-#define NS3D_MARCH_DMA 1
-#endif
-template <int INTERP, bool CHAR, class R>
-constexpr bool kMarchDma = NS3D_MARCH_DMA != 0;
-
 // Component-wise WCNS5 / WENO5-JS: the marches form both reconstructions of a
 // cell at once (recon_pair), in FP64 and in the FP32 faces of the mixed
 // method (which then still fits three waves per SIMD without scratch).
@@ -442,45 +433,6 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
         u[3] = (R)a.U[P3][g];
         u[4] = mixed ? (R)(a.U[4][g] - a.ref.E_ref) : (R)a.U[4][g];
     };
-    // The window's next cell, staged one iteration ahead in LDS by the
-    // global-to-LDS loads (no registers held while the face work runs): two
-    // buffers of 5 variables x 2 dwords x 64 lanes per wave; lane t writes
-    // and reads only its own dwords, so no barrier is needed.  Addresses: a
-    // wave-uniform row base (scalar registers) plus the lane's byte offset.
-    constexpr bool DMA = kMarchDma<INTERP, CHAR, R>;
-    __shared__ uint32_t pfb[DMA ? 2 * 4 * 5 * 2 * 64 : 1];
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const int lane = threadIdx.x;
-    auto stage_cell = [&](int c, int buf) {
-        const int ou = blockIdx.y * 4 + wid;
-        const long g = (DIR == 1 ? b.gidx(blockIdx.x * 64, 0, ou) : b.gidx(blockIdx.x * 64, ou, 0)) +
-                       (long)wrap(c, N) * gst;
-        const unsigned loff = (unsigned)lane * 8u;
-        const int vars[5] = {0, P1, P2, P3, 4};
-#pragma unroll
-        for (int n = 0; n < 5; ++n) {
-            // the row base as a scalar value, so that the lane offset is the
-            // only vector operand of the address
-            const unsigned long long rb = reinterpret_cast<unsigned long long>(a.U[vars[n]] + g);
-            const unsigned rhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(rb >> 32));
-            const unsigned rlo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)rb);
-            const char *row = reinterpret_cast<const char *>(((unsigned long long)rhi << 32) | rlo);
-            const char *src = row + loff;
-            uint32_t *dst = pfb + ((buf * 4 + wid) * 5 + n) * 128;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
-                                             (__attribute__((address_space(3))) void *)dst, 4, 0, 0);
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + 4),
-                                             (__attribute__((address_space(3))) void *)(dst + 64), 4, 0, 0);
-        }
-    };
-    auto take_cell = [&](int buf, R *u) {
-#pragma unroll
-        for (int n = 0; n < 5; ++n) {
-            const uint32_t *p = pfb + ((buf * 4 + wid) * 5 + n) * 128 + lane;
-            const double d = __builtin_bit_cast(double, ((unsigned long long)p[64] << 32) | p[0]);
-            u[n] = (mixed && n == 4) ? (R)(d - a.ref.E_ref) : (R)d;
-        }
-    };
     // Cell-sum weights: face f -> cells f-2..f+3 (-kHalfW, slots 1..6),
     // viscous flux of cell f -> cells f-3..f+3 (central weights / 60, slots
     // 0..6, none for the cell itself).  FP64 component-wise MDCD: pre-scaled
@@ -536,30 +488,22 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
     for (int q = 0; q < 7; ++q)
 #pragma unroll
         for (int n = 0; n < 5; ++n) acc[q][n] = 0.0;
-    if constexpr (DMA) stage_cell(c0, 0);
     for (int f = c0 - 3; f <= c1 + 2; ++f) {
-        // Without the LDS staging the window's new cell is loaded in the
-        // iteration that uses it: a register prefetch one iteration ahead
-        // kept the next cell's five values live through the face work, which
-        // spilled the FP64 marches at two waves per SIMD.
-        const int buf = (f - c0 + 3) & 1;
+        // The window's new cell is loaded in the iteration that uses it: a
+        // prefetch one iteration ahead kept the next cell's five values live
+        // through the face work, which spilled the FP64 marches at two waves
+        // per SIMD; the other resident waves cover the load's latency.
         {
             R nxt[5];
-            if constexpr (DMA) take_cell(buf, nxt);
-            else load(f + 3, nxt);
+            load(f + 3, nxt);
 #pragma unroll
             for (int n = 0; n < 5; ++n) w[n][W - 1] = nxt[n];
         }
-        // the staged reads above stay ahead of the next cell's staging
-        if constexpr (DMA) __builtin_amdgcn_sched_barrier(0);
         double vv[4];
         {
             const long o = obase + (long)wrap(f, N) * ost;
 #pragma unroll
             for (int n = 0; n < 4; ++n) vv[n] = visc_at<R>(a.V[kViscSlot[DIR][n]], o);
-        }
-        if constexpr (DMA) {
-            if (f < c1 + 2) stage_cell(f + 4, buf ^ 1);
         }
         if (f <= c1 + 1) {  // the last iteration only completes cell c1 - 1
             R pc[2][5];
