@@ -24,6 +24,10 @@ struct Box {
 };
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+// threadIdx.x of the calling wave's first lane (1-D blocks of whole waves,
+// all lanes active): a scalar value, so that indices derived from it are
+// formed on the scalar unit instead of once per lane
+__device__ __forceinline__ int wave_first_thread() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63; }
 
 // Reference state subtracted before the FP32 arithmetic of the mixed method
 // (zero for the plain method): E' = E - E_ref, and the pressure in the

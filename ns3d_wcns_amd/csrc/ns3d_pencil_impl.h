@@ -157,10 +157,12 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
     const Box &b = a.b;
     const int N = b.nx, W = N + 2 * NG;
     const int P = a.line_threads, Lb = blockDim.x / P;
-    const int ll = threadIdx.x / P, t = threadIdx.x % P;
-    const long line = (long)blockIdx.x * Lb + ll;
-    const bool valid = line < (long)b.ny * b.nz;
-    const int j = valid ? (int)(line % b.ny) : 0, k = valid ? (int)(line / b.ny) : 0;
+    // P is a multiple of 64, so each wave lies within one line: the line
+    // index is wave-uniform and its divisions run on the scalar unit
+    const int ll = wave_first_thread() / P, t = (int)threadIdx.x - ll * P;
+    const int line = (int)blockIdx.x * Lb + ll;
+    const bool valid = line < b.ny * b.nz;
+    const int j = valid ? line % b.ny : 0, k = valid ? line / b.ny : 0;
     double *Ul = lds + (long)ll * (5 * W + 9 * N);  // U[5][W], F[5][N], V[4][N]
     double *Fl = Ul + 5 * W;
     double *Vl = Fl + 5 * N;
@@ -298,10 +300,11 @@ __global__ void __launch_bounds__(512) k_pencil_xp(PencilArgs a, SchemeCoef<R> c
     const Box &b = a.b;
     const int N = b.nx, W = N + 2 * NG, NF = N + 5, NX = N + 1;
     const int Lb = blockDim.x / N;
-    const int ll = threadIdx.x / N, t = threadIdx.x % N;
-    const long line = (long)blockIdx.x * Lb + ll;
-    const bool valid = line < (long)b.ny * b.nz;
-    const int j = valid ? (int)(line % b.ny) : 0, k = valid ? (int)(line / b.ny) : 0;
+    // N is a multiple of 64: the line index is wave-uniform (scalar unit)
+    const int ll = wave_first_thread() / N, t = (int)threadIdx.x - ll * N;
+    const int line = (int)blockIdx.x * Lb + ll;
+    const bool valid = line < b.ny * b.nz;
+    const int j = valid ? line % b.ny : 0, k = valid ? line / b.ny : 0;
     double *Ul = lds + (long)ll * xpair_line_doubles(N);  // U[5][W], V[4][W], F[5][NF], X[5][NX]
     double *Vl = Ul + 5 * W;
     double *Fl = Vl + 4 * W;
