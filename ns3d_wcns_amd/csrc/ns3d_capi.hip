@@ -66,6 +66,9 @@ struct ns3d_ctx {
     void *fft_plan;
     double *partial;
     double *scal;  // [0] dt, [1] reduction result
+    // CFL_SLOTS maxima that a device time loop's stage-3 x pencils fold the
+    // new state's CFL speeds into (zero between uses)
+    unsigned long long *cfl_slots;
     bool timing;
     bool visc_fp32;  // visc[] holds the mixed method's FP32 fluxes (last writer: a mixed fused stage)
     // the primitives of the state are the compact FP32 u, v, w, T - 1 of
@@ -412,7 +415,7 @@ struct StageIO {
 // and stage 3 of a device time loop's steps other than the last): the x
 // pencil hands the primitives to the next stage's viscous kernel in FP32.
 int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, int u0mode = U0_READ, bool ghosts = true,
-                const StageIO *io = nullptr, bool q32 = false) {
+                const StageIO *io = nullptr, bool q32 = false, bool fold_cfl = false) {
     const bool mixed = c->precision == NS3D_PRECISION_MIXED;
     {
         ViscArgs va = visc_args(c, false, true);
@@ -449,6 +452,10 @@ int fused_stage(ns3d_ctx *c, int stage, const double *d_dt, double dt, int u0mod
     a.dtv = dt;
     a.stage = stage;
     for (int q = 0; q < 4; ++q) a.Q32[q] = q32 && mixed && stage >= 1 ? prims32(c, q) : nullptr;
+    // stage 3 of a device time loop's step that another step follows: the x
+    // pencil folds the new state's CFL speeds into the slots the next step's
+    // dt is taken from
+    a.cfl_slots = fold_cfl && stage == 3 ? c->cfl_slots : nullptr;
     const SchemeCoef<double> c64 = coef64(c->P);
     const SchemeCoef<float> c32 = coef32(c->P);
     const int interp = c->P.interpolation;
@@ -514,7 +521,7 @@ int record0(ns3d_ctx *c);
 int bnd_rk3(ns3d_ctx *c, const double *d_dt, double dt);
 
 int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool ghosts = true,
-        bool q32_last = false, bool through = false) {
+        bool q32_last = false, bool through = false, bool fold_cfl = false) {
     int rc;
     if (c->bounded) {
         if ((record || through) && (rc = record0(c))) return rc;
@@ -526,7 +533,7 @@ int rk3(ns3d_ctx *c, const double *d_dt, double dt, bool record = false, bool gh
     for (int stage = 1; stage <= 3; ++stage) {
         const int u0mode = stage > 1 ? U0_READ : through ? U0_INPUT : record ? U0_RECORD : U0_READ;
         if ((rc = fused_stage(c, stage, d_dt, dt, u0mode, ghosts && stage == 3, through ? &io[stage - 1] : nullptr,
-                              stage < 3 || q32_last))) {
+                              stage < 3 || q32_last, fold_cfl))) {
             c->prims_fp32 = false;
             return rc;
         }
@@ -698,6 +705,23 @@ int cfl(ns3d_ctx *c, double *d_out) {
     Timed t(c, NS3D_K_CFL);
     launch_cfl(a, c->stream);
     NS3D_LAUNCHED("cfl");
+    return NS3D_OK;
+}
+
+// The same dt from the maxima the previous step's stage-3 x pencils folded
+// into c->cfl_slots (bit-identical: the same speeds of the same stored
+// state, and a maximum does not depend on the order), clearing the slots.
+int cfl_folded(ns3d_ctx *c, double *d_out) {
+    CflArgs a{};
+    a.partial = reinterpret_cast<double *>(c->cfl_slots);
+    a.dt_out = d_out;
+    a.b = c->b;
+    a.gamma = c->P.gamma;
+    a.dx = c->grid.dx;
+    a.cfl = c->P.cfl;
+    Timed t(c, NS3D_K_CFL);
+    launch_cfl_final_slots(a, c->stream);
+    NS3D_LAUNCHED("cfl (folded)");
     return NS3D_OK;
 }
 
@@ -1150,7 +1174,7 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     // whose sizes are powers of two (128 MiB at 256^3) do not all start on
     // the same HBM channel when a kernel touches the same index of many.
     const size_t cstride = field_stride(c->ncell), istride = field_stride(c->nint);
-    const size_t total = NS3D_NUM_CELL_FIELDS * cstride + (25 + NVISC) * istride + npart + 16;
+    const size_t total = NS3D_NUM_CELL_FIELDS * cstride + (25 + NVISC) * istride + npart + 16 + CFL_SLOTS;
     if (hipMalloc(&c->slab, total * sizeof(double)) != hipSuccess) {
         (void)hipGetLastError();
         delete c;
@@ -1166,6 +1190,8 @@ int ns3d_create(ns3d_ctx **out, const ns3d_grid *grid, const ns3d_params *params
     c->partial = p;
     p += npart;
     c->scal = p;
+    p += 16;
+    c->cfl_slots = reinterpret_cast<unsigned long long *>(p);
     c->grad_slab = nullptr;
     c->adj_slab = nullptr;
     c->obs_work = nullptr;
@@ -1560,6 +1586,7 @@ int ns3d_time_advance(ns3d_ctx *c, int max_steps, double dt_fixed, double total_
 int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
     NS3D_CTX(c);
     int rc;
+    bool folded = false;
     for (int s = 0; s < nsteps; ++s) {
         double *d_dt = d_dt_hist ? d_dt_hist + s : c->scal;
         // recordConservedTo0 fused into the first stage of the last step;
@@ -1568,8 +1595,13 @@ int ns3d_advance_steps(ns3d_ctx *c, int nsteps, double *d_dt_hist) {
         // hands out FP64 primitives (the mixed method's steps before it pass
         // theirs on in FP32; the CFL reduction reads the conserved state)
         const bool last = s == nsteps - 1;
-        if ((rc = cfl(c, d_dt))) return rc;
-        if ((rc = rk3(c, d_dt, 0.0, last, last, !last, !last))) return rc;
+        // steps after the first take dt from the CFL maxima that the previous
+        // step's stage-3 x pencils folded in (periodic boxes), instead of a
+        // separate pass over the state
+        if ((rc = folded ? cfl_folded(c, d_dt) : cfl(c, d_dt))) return rc;
+        const bool fold = !last && !c->bounded;
+        if ((rc = rk3(c, d_dt, 0.0, last, last, !last, !last, fold))) return rc;
+        folded = fold;
     }
     return NS3D_OK;
 }

@@ -29,6 +29,18 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 // formed on the scalar unit instead of once per lane
 __device__ __forceinline__ int wave_first_thread() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63; }
 
+// |u| + a with the reference's x-velocity formula (compute_timestep,
+// timestep.cpp:12-20).  Evaluated without contraction so that
+// dt = cfl * min(1e9, dx / smax) reproduces the reference's bits (correctly
+// rounded division is monotone, so min_i dx/s_i = dx / max_i s_i).
+__device__ __forceinline__ double cfl_speed(double rho, double rhou, double E, double gamma) {
+#pragma clang fp contract(off)
+    const double u = rhou / rho;
+    const double p = (gamma - 1.) * (E - 0.5 * rho * u * u);
+    const double a = sqrt(gamma * p / rho);
+    return fabs(u) + a;
+}
+
 // Reference state subtracted before the FP32 arithmetic of the mixed method
 // (zero for the plain method): E' = E - E_ref, and the pressure in the
 // momentum flux is carried as p - p_ref.
@@ -131,7 +143,14 @@ struct PencilArgs {
     int seg;                  // y / z segment length
     int line_threads;         // x pencil: threads per line
     float *Q32[4];            // mixed, stages 1-2: compact FP32 u, v, w, T - 1 written instead of Q (else null)
+    unsigned long long *cfl_slots;  // stage 3: CFL speed maxima of the new state (atomic max of the bit
+                                    // patterns into CFL_SLOTS slots), for the next step's dt; else null
 };
+
+// Slots of the CFL maxima that the stage-3 x pencils fold into (each wave
+// takes slot (its index in the launch) mod CFL_SLOTS; non-negative doubles
+// order as their bit patterns do).
+constexpr int CFL_SLOTS = 2048;
 
 template <int RS>
 void launch_pencils(int dir, int interp, bool chr, int precision, const PencilArgs &a, const SchemeCoef<double> &c64,

@@ -125,6 +125,7 @@ void launch_halo(const HaloArgs &h, hipStream_t s);
 // Blocks of the CFL reduction's first pass (block maxima written to a.partial).
 long cfl_block_count(const Box &b);
 void launch_cfl(const CflArgs &a, hipStream_t s);
+void launch_cfl_final_slots(const CflArgs &a, hipStream_t s);
 void launch_ke(const KeArgs &a, double *out, hipStream_t s);
 void launch_interp_batch(int interp, const SchemeCoef<double> &c, const double *st, int n, double flag, double *out,
                          hipStream_t s);

@@ -355,18 +355,8 @@ __global__ void k_halo(HaloArgs h) {
 }
 
 // ---------------------------------------------------------------- CFL
-// max over the interior of |u| + a with the reference's x-velocity formula
-// (compute_timestep, timestep.cpp:12-20); NaN candidates are skipped, as
-// std::min skips them there.  Evaluated without contraction so that
-// dt = cfl * min(1e9, dx / smax) reproduces the reference's bits (correctly
-// rounded division is monotone, so min_i dx/s_i = dx / max_i s_i).
-__device__ __forceinline__ double cfl_speed(double rho, double rhou, double E, double gamma) {
-#pragma clang fp contract(off)
-    const double u = rhou / rho;
-    const double p = (gamma - 1.) * (E - 0.5 * rho * u * u);
-    const double a = sqrt(gamma * p / rho);
-    return fabs(u) + a;
-}
+// max over the interior of |u| + a (cfl_speed, ns3d_common.h); NaN
+// candidates are skipped, as std::min skips them in timestep.cpp.
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -417,10 +407,14 @@ __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
 }
 
 // Second pass: max of the block maxima, then dt = cfl * min(1e9, dx / smax).
-__global__ void __launch_bounds__(256) k_cfl_final(CflArgs a, long nblk) {
+__global__ void __launch_bounds__(256) k_cfl_final(CflArgs a, long nblk, bool clear) {
     __shared__ double part[4];
     double s = 0.0;
-    for (long q = threadIdx.x; q < nblk; q += 256) s = fmax(s, a.partial[q]);
+    for (long q = threadIdx.x; q < nblk; q += 256) {
+        s = fmax(s, a.partial[q]);
+        // slots of the stage-3 x pencils' atomic maxima start the next step at 0
+        if (clear) a.partial[q] = 0.0;
+    }
     s = wave_max(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -548,7 +542,12 @@ long cfl_block_count(const Box &b) { return reduce_grid(((long)b.ny * b.nz + 3) 
 void launch_cfl(const CflArgs &a, hipStream_t s) {
     const unsigned g = (unsigned)cfl_block_count(a.b);
     hipLaunchKernelGGL(k_cfl, dim3(g), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, (long)g);
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, (long)g, false);
+}
+// dt from the CFL maxima that the previous step's stage-3 x pencils folded
+// into a.partial (CFL_SLOTS slots), which are cleared for the next fold
+void launch_cfl_final_slots(const CflArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, s, a, (long)CFL_SLOTS, true);
 }
 void launch_ke(const KeArgs &a, double *out, hipStream_t s) {
     const unsigned g = reduce_grid((long)a.b.ny * a.b.nz);

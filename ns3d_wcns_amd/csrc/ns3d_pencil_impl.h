@@ -82,8 +82,8 @@ __device__ __forceinline__ double get_partial(const double *P, long o) {
 // trip; writes the new state and the primitives (range_kutta.cpp:53-91,
 // field_structures.h:444-488).
 template <class R>
-__device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o, const double *u0v,
-                                             const double *ucur, const double *r) {
+__device__ __forceinline__ double stage_update(const PencilArgs &a, long g, long o, const double *u0v,
+                                               const double *ucur, const double *r) {
     const double dt = a.dt ? *a.dt : a.dtv;
     double un[5];
 #pragma unroll
@@ -99,11 +99,15 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o
     const double u = un[1] * ir, v = un[2] * ir, w = un[3] * ir;
     const double kin = 0.5 * (u * u + v * v + w * w);
     const double p = (a.gamma - 1.0) * rr * (un[4] * ir - kin);
+    const double ru = rr * u, En = rr * (p / ((a.gamma - 1.0) * rr) + kin);
     a.Unew[0][g] = rr;
-    a.Unew[1][g] = rr * u;
+    a.Unew[1][g] = ru;
     a.Unew[2][g] = rr * v;
     a.Unew[3][g] = rr * w;
-    a.Unew[4][g] = rr * (p / ((a.gamma - 1.0) * rr) + kin);
+    a.Unew[4][g] = En;
+    // the CFL speed of the stored state, as the CFL reduction forms it from
+    // these same values (step 3 of a device time loop folds it)
+    const double cs = a.cfl_slots ? cfl_speed(rr, ru, En, a.gamma) : 0.0;
     // the primitives are next read by the following stage's viscous kernel:
     // streamed past the caches (nontemporal), as are the partial sums and U0
     // below and the viscous fluxes, each read once by one later kernel.  The
@@ -122,7 +126,7 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o
             __builtin_nontemporal_store((float)v, a.Q32[1] + o);
             __builtin_nontemporal_store((float)w, a.Q32[2] + o);
             __builtin_nontemporal_store((float)(T - 1.0), a.Q32[3] + o);
-            return;
+            return cs;
         }
     }
     __builtin_nontemporal_store(u, &a.Q[0][g]);
@@ -130,6 +134,20 @@ __device__ __forceinline__ void stage_update(const PencilArgs &a, long g, long o
     __builtin_nontemporal_store(w, &a.Q[2][g]);
     if (a.stage == 3) __builtin_nontemporal_store(p, &a.Q[3][g]);
     __builtin_nontemporal_store(T, &a.Q[4][g]);
+    return cs;
+}
+
+// The wave's maximum CFL speed (NaN skipped, as the CFL reduction does)
+// folded into its slot by an atomic maximum of the bit patterns.  Called by
+// every lane of the wave.
+__device__ __forceinline__ void cfl_fold(unsigned long long *slots, double v) {
+    double s = v == v ? v : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
+    if ((threadIdx.x & 63) == 0) {
+        const unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        atomicMax(&slots[wave % CFL_SLOTS], (unsigned long long)__double_as_longlong(s));
+    }
 }
 
 // Component-wise WCNS5 / WENO5-JS: the marches form both reconstructions of a
@@ -239,6 +257,7 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
     }
     __syncthreads();
     if (!valid) return;
+    double cmax = 0.0;  // CFL speed maximum of the thread's cells (stage 3 folding)
     for (int c = t; c < N; c += P) {
         int fi[7];
 #pragma unroll
@@ -274,10 +293,12 @@ __global__ void __launch_bounds__(512) k_pencil_x(PencilArgs a, SchemeCoef<R> cf
 #pragma unroll
                 for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
             }
-            stage_update<R>(a, g, orow + c, u0, u, r);
+            const double cs = stage_update<R>(a, g, orow + c, u0, u, r);
+            cmax = cs == cs ? fmax(cmax, cs) : cmax;
         }
         if constexpr (ONE) break;
     }
+    if (a.cfl_slots) cfl_fold(a.cfl_slots, cmax);
 }
 
 // LDS doubles per line of the paired x pencil: U and V on the line with three
@@ -401,7 +422,8 @@ __global__ void __launch_bounds__(512) k_pencil_xp(PencilArgs a, SchemeCoef<R> c
 #pragma unroll
             for (int n = 0; n < 5; ++n) a.U0[n][g] = u[n];
         }
-        stage_update<R>(a, g, orow + c, u0, u, r);
+        const double cs = stage_update<R>(a, g, orow + c, u0, u, r);
+        if (a.cfl_slots) cfl_fold(a.cfl_slots, cs);
     }
 }
 
