@@ -262,8 +262,10 @@ def test_time_advance_clipping(gpu):
 
 
 def test_advance_steps_matches_time_advance(gpu):
-    """The device-only loop (CFL fused with recordConservedTo0, dt kept on the
-    device) takes the same steps as the host-scheduled time_advance."""
+    """The device-only loop (dt kept on the device; every step after the
+    first takes it from the CFL maxima that the previous step's stage-3 x
+    pencil folded in) takes the same steps as the host-scheduled
+    time_advance, whose steps run the separate CFL pass."""
     import torch
 
     prim, ma = state(16, 0.3, seed=23)
@@ -282,6 +284,32 @@ def test_advance_steps_matches_time_advance(gpu):
     # the device loop records U0 for the interior cells only (ghost cells of
     # U0 are never read)
     for name in ("rho0", "rhou0", "E0"):
+        assert np.array_equal(a.interior(a.download(name)), b.interior(b.download(name))), name
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("precision,char", [(0, False), (1, False), (0, True)], ids=["fp64", "mixed", "fp64-char"])
+def test_folded_cfl_dt_bit_exact(gpu, precision, char):
+    """dt from the CFL maxima folded into the stage-3 x pencil equals, bit for
+    bit, the separate CFL pass over the same states: x lines of 64 cells
+    (the paired x pencil, or the general one characteristic-wise), several
+    lines per block, in both precisions."""
+    import torch
+
+    n = (64, 40, 36)
+    prim, ma = state(n, 0.35, noise=0.02, seed=31)
+    P = params(ma, char=char)
+    a = DeviceField(box(n), P, precision=precision)
+    b = DeviceField(box(n), P, precision=precision)
+    a.initialize_primitive(*prim)
+    b.initialize_primitive(*prim)
+    dts = a.time_advance(6)
+    d_hist = torch.zeros(6, dtype=torch.float64, device="cuda")
+    b.advance_steps(6, d_hist.data_ptr())
+    b.synchronize()
+    assert np.array_equal(d_hist.cpu().numpy(), dts)
+    for name in CONS:
         assert np.array_equal(a.interior(a.download(name)), b.interior(b.download(name))), name
     a.close()
     b.close()
