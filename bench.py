@@ -646,6 +646,9 @@ def main():
         "compute": "FP64 vector FMA/add/mul (v_fma_f64); no MFMA instruction is issued; the MI355X FP64 "
                    "vector and FP64 matrix peaks are both 78.6 TFLOP/s",
         "hbm_GBps_measured_traffic": round(traffic / (per_launch[dom] * 1e-3) / 1e9, 1) if traffic else None,
+        # the same kernel against the HBM roof on its measured bytes: the x pencil runs on memory (cutting a
+        # sixth of its VALU instructions left its time unchanged), so this is the roof it presses on
+        "hbm_frac_measured_traffic": round(traffic / (per_launch[dom] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4) if traffic else None,
         # rocprofv3 counters of the same kernel (profiles/<kernel>_valu_<n>.md): share of cycles the VALU
         # is busy, fraction of the one-wave-instruction-per-CU-per-cycle issue rate, counted FP64 rate
         "valu_busy_pct": valu.get("valu_busy_pct"),
