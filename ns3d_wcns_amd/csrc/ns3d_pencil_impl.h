@@ -27,6 +27,7 @@
 #pragma once
 #include "ns3d_common.h"
 
+
 namespace ns3d {
 
 // Cell-sum coefficient of face f for cell c = f - 2 + q, q = 0..5 (the
@@ -530,6 +531,17 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
 #pragma unroll
             for (int n = 0; n < 4; ++n) vv[n] = visc_at<R>(a.V[kViscSlot[DIR][n]], o);
         }
+        // Wave priorities (paired marches): a wave drops to the lowest issue
+        // priority once its loads for this iteration are out, while it waits
+        // for them and reconstructs the new cell, and takes a higher one for
+        // the Riemann flux, the sums and the next iteration's loads.  The two
+        // waves of a SIMD then alternate instead of both stalling on their
+        // loads at the same time.
+        if constexpr (PAIR) {
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_setprio(0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         if (f <= c1 + 1) {  // the last iteration only completes cell c1 - 1
             R pc[2][5];
             if constexpr (CHAR) {
@@ -544,6 +556,12 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
                 R UR[5], yn[5];
 #pragma unroll
                 for (int n = 0; n < 5; ++n) recon_pair<INTERP>(w[n], yn[n], UR[n]);  // cell f + 1
+                // the wave whose new cell has arrived and been reconstructed
+                // takes the issue priority for the rest of its iteration
+                // (see the loop head)
+                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_s_setprio(2);
+                __builtin_amdgcn_sched_barrier(0);
                 riemann<RS, XS>(cf, yl, UR, Fr);
 #pragma unroll
                 for (int n = 0; n < 5; ++n) yl[n] = yn[n];
