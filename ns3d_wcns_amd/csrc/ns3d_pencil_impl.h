@@ -28,6 +28,8 @@
 #include "ns3d_common.h"
 
 
+
+
 namespace ns3d {
 
 // Cell-sum coefficient of face f for cell c = f - 2 + q, q = 0..5 (the
@@ -531,17 +533,14 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
 #pragma unroll
             for (int n = 0; n < 4; ++n) vv[n] = visc_at<R>(a.V[kViscSlot[DIR][n]], o);
         }
-        // Wave priorities (paired marches): a wave drops to the lowest issue
-        // priority once its loads for this iteration are out, while it waits
-        // for them and reconstructs the new cell, and takes a higher one for
-        // the Riemann flux, the sums and the next iteration's loads.  The two
-        // waves of a SIMD then alternate instead of both stalling on their
-        // loads at the same time.
-        if constexpr (PAIR) {
-            __builtin_amdgcn_sched_barrier(0);
-            __builtin_amdgcn_s_setprio(0);
-            __builtin_amdgcn_sched_barrier(0);
-        }
+        // Wave priorities: a wave drops to the lowest issue priority once its
+        // loads for this iteration are out, while it waits for them and
+        // reconstructs the new cell, and takes a higher one for the Riemann
+        // flux, the sums and the next iteration's loads.  The waves of a SIMD
+        // then alternate instead of stalling on their loads at the same time.
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(0);
+        __builtin_amdgcn_sched_barrier(0);
         if (f <= c1 + 1) {  // the last iteration only completes cell c1 - 1
             R pc[2][5];
             if constexpr (CHAR) {
@@ -567,6 +566,11 @@ __device__ __forceinline__ void pencil_m(PencilArgs a, SchemeCoef<R> cf) {
                 for (int n = 0; n < 5; ++n) yl[n] = yn[n];
             } else {
                 face_flux<INTERP, RS, CHAR, XS>(cf, w, pc, Fr);
+                // (the face's interpolation and flux are one call here: the
+                // priority rises after both)
+                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_s_setprio(2);
+                __builtin_amdgcn_sched_barrier(0);
             }
             double F[5];
             F[0] = (double)Fr[0];
