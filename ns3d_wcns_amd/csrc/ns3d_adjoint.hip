@@ -10,12 +10,12 @@
 //   k_tl_assemble tangent RHS: half-node differences + viscous divergence.
 // Adjoint (hand-written reverse mode, ns3d_adjoint.h; component-wise WCNS5 /
 // WENO5-JS with HLLC / Rusanov):
-//   k_adj_lam_march  z-marching plane tiles of lambda: face adjoints of the
-//                 three directions (the transposed half-node difference) and
-//                 viscous flux adjoints (the transposed divergence);
-//   k_adj_q_march  z-marching plane tiles of u, v, w, T: their gradients and
-//                 the reverse of stresses / heat flux -> gradient and
-//                 (u, v, w, T) adjoints;
+//   k_adj_lamq_march  z-marching plane tiles of lambda and of u, v, w, T:
+//                 face adjoints of the three directions (the transposed
+//                 half-node difference), viscous flux adjoints (the
+//                 transposed divergence), then the gradients and the reverse
+//                 of stresses / heat flux -> gradient and (u, v, w, T)
+//                 adjoints;
 //   per direction, marching along the direction (lanes across it):
 //   k_adj_riemann both reconstructions of each cell (shared smoothness
 //                 indicators) and the reverse of the Riemann flux of each
@@ -750,40 +750,59 @@ struct PlaneMap {
     __device__ long plane(int k) const { return (long)wrap(k, b.nz) * ((long)b.sx * b.sy); }
 };
 
-// Lambda part: face adjoints Fb = -(1/h) sum_m kW[m] lambda[c + 3 - m] of the
-// three directions (the transposed half-node difference) and the viscous-flux
-// adjoints Vb = -(transposed divergence of lambda), stored in SB[0..8] for the
-// second kernel (SB is rewritten later by the Riemann reverse).
-__global__ void __launch_bounds__(256) k_adj_lam_march(AdjArgs a) {
+// z-marching plane tiles of lambda and of (u, v, w, T), one plane per
+// iteration.  Lambda part: the face adjoints FB = -(1/h) sum_m kW[m]
+// lambda[c + 3 - m] of the three directions (the transposed half-node
+// difference) and the viscous-flux adjoints Vb = -(transposed divergence of
+// lambda).  Primitive part: the gradients of u, v, w, T (6th-order central)
+// and the reverse of the local viscous-flux map with Vb -> gradient adjoints
+// Gb and (u, v, w, T) adjoints qb.  Vb stays in registers between the two
+// parts; two waves per SIMD (238 VGPRs) keep both register rings unspilled.
+__global__ void __launch_bounds__(256, 2) k_adj_lamq_march(AdjArgs a) {
     constexpr int HX = 70, HY = 10;
-    __shared__ double tile[5][HY][HX];
+    __shared__ double tl[5][HY][HX];
+    __shared__ double tq[4][HY][HX];
     const Box &b = a.b;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4, i = i0 + tx, j = j0 + ty;
     const bool own = i < b.nx && j < b.ny;
     const int k0 = blockIdx.z * a.seg, k1 = min(k0 + a.seg, b.nz);
     const PlaneMap pm(b, i0, j0, tx, ty);
-    double wl[5][7];
+    const double *Qv[4] = {a.Q[0], a.Q[1], a.Q[2], a.Q[4]};
+    double wl[5][7], wq[4][7];
 #pragma unroll
-    for (int m = 0; m < 6; ++m)
+    for (int m = 0; m < 6; ++m) {
 #pragma unroll
         for (int n = 0; n < 5; ++n) wl[n][m] = a.X[n][pm.col + pm.plane(k0 - 3 + m)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wq[q][m] = Qv[q][pm.col + pm.plane(k0 - 3 + m)];
+    }
     for (int k = k0; k < k1; ++k) {
 #pragma unroll
         for (int n = 0; n < 5; ++n) wl[n][6] = a.X[n][pm.col + pm.plane(k + 3)];
-        double hv[2][5];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wq[q][6] = Qv[q][pm.col + pm.plane(k + 3)];
+        double hl[2][5], hq[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) hl[h][n] = pm.hv[h] ? a.X[n][pm.hc[h] + pm.plane(k)] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) hq[h][q] = pm.hv[h] ? Qv[q][pm.hc[h] + pm.plane(k)] : 0.0;
+        }
+        __syncthreads();  // the previous plane's tiles have been read
+#pragma unroll
+        for (int n = 0; n < 5; ++n) tl[n][ty + 3][tx + 3] = wl[n][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tq[q][ty + 3][tx + 3] = wq[q][3];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
+            if (pm.hv[h]) {
 #pragma unroll
-            for (int n = 0; n < 5; ++n) hv[h][n] = pm.hv[h] ? a.X[n][pm.hc[h] + pm.plane(k)] : 0.0;
-        __syncthreads();  // the previous plane's tile has been read
+                for (int n = 0; n < 5; ++n) tl[n][pm.hj[h] + 3][pm.hi[h] + 3] = hl[h][n];
 #pragma unroll
-        for (int n = 0; n < 5; ++n) tile[n][ty + 3][tx + 3] = wl[n][3];
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-            if (pm.hv[h])
-#pragma unroll
-                for (int n = 0; n < 5; ++n) tile[n][pm.hj[h] + 3][pm.hi[h] + 3] = hv[h][n];
+                for (int q = 0; q < 4; ++q) tq[q][pm.hj[h] + 3][pm.hi[h] + 3] = hq[h][q];
+            }
         __syncthreads();
         if (own) {
             const long o = b.oidx(i, j, k);
@@ -795,14 +814,13 @@ __global__ void __launch_bounds__(256) k_adj_lam_march(AdjArgs a) {
                     double lv[7];
 #pragma unroll
                     for (int m = 0; m < 7; ++m)
-                        lv[m] = d == 0 ? tile[n][ty + 3][tx + m] : d == 1 ? tile[n][ty + m][tx + 3] : wl[n][m];
+                        lv[m] = d == 0 ? tl[n][ty + 3][tx + m] : d == 1 ? tl[n][ty + m][tx + 3] : wl[n][m];
                     double sum = 0.0;
 #pragma unroll
                     for (int m = 0; m < 6; ++m) sum += kW[m] * lv[6 - m];
                     __builtin_nontemporal_store(-sum * a.ih[d], &a.FB[d][n][o]);
                     if (n > 0) lg[n - 1][d] = d6c(lv[0], lv[1], lv[2], lv[4], lv[5], lv[6], a.i60[d]);
                 }
-            // transposed divergence (D6 is antisymmetric on the periodic box)
             double Vb[NVISC];
             Vb[VXX] = -lg[0][0];
             Vb[VYY] = -lg[1][1];
@@ -813,65 +831,15 @@ __global__ void __launch_bounds__(256) k_adj_lam_march(AdjArgs a) {
             Vb[VEX] = -lg[3][0];
             Vb[VEY] = -lg[3][1];
             Vb[VEZ] = -lg[3][2];
-#pragma unroll
-            for (int n = 0; n < NVISC; ++n) __builtin_nontemporal_store(Vb[n], &a.SB[n][o]);
-        }
-#pragma unroll
-        for (int m = 0; m < 6; ++m)
-#pragma unroll
-            for (int n = 0; n < 5; ++n) wl[n][m] = wl[n][m + 1];
-    }
-}
-
-// Primitive part: the gradients of u, v, w, T (6th-order central) and the
-// reverse of the local viscous-flux map with Vb from SB[0..8] -> gradient
-// adjoints Gb and (u, v, w, T) adjoints qb.
-__global__ void __launch_bounds__(256) k_adj_q_march(AdjArgs a) {
-    constexpr int HX = 70, HY = 10;
-    __shared__ double tile[4][HY][HX];
-    const Box &b = a.b;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4, i = i0 + tx, j = j0 + ty;
-    const bool own = i < b.nx && j < b.ny;
-    const int k0 = blockIdx.z * a.seg, k1 = min(k0 + a.seg, b.nz);
-    const PlaneMap pm(b, i0, j0, tx, ty);
-    const double *Qv[4] = {a.Q[0], a.Q[1], a.Q[2], a.Q[4]};
-    double wq[4][7];
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) wq[q][m] = Qv[q][pm.col + pm.plane(k0 - 3 + m)];
-    for (int k = k0; k < k1; ++k) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) wq[q][6] = Qv[q][pm.col + pm.plane(k + 3)];
-        double hv[2][4];
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) hv[h][q] = pm.hv[h] ? Qv[q][pm.hc[h] + pm.plane(k)] : 0.0;
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) tile[q][ty + 3][tx + 3] = wq[q][3];
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-            if (pm.hv[h])
-#pragma unroll
-                for (int q = 0; q < 4; ++q) tile[q][pm.hj[h] + 3][pm.hi[h] + 3] = hv[h][q];
-        __syncthreads();
-        if (own) {
-            const long o = b.oidx(i, j, k);
             double g[4][3];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                g[q][0] = d6c(tile[q][ty + 3][tx], tile[q][ty + 3][tx + 1], tile[q][ty + 3][tx + 2],
-                              tile[q][ty + 3][tx + 4], tile[q][ty + 3][tx + 5], tile[q][ty + 3][tx + 6], a.i60[0]);
-                g[q][1] = d6c(tile[q][ty][tx + 3], tile[q][ty + 1][tx + 3], tile[q][ty + 2][tx + 3],
-                              tile[q][ty + 4][tx + 3], tile[q][ty + 5][tx + 3], tile[q][ty + 6][tx + 3], a.i60[1]);
+                g[q][0] = d6c(tq[q][ty + 3][tx], tq[q][ty + 3][tx + 1], tq[q][ty + 3][tx + 2], tq[q][ty + 3][tx + 4],
+                              tq[q][ty + 3][tx + 5], tq[q][ty + 3][tx + 6], a.i60[0]);
+                g[q][1] = d6c(tq[q][ty][tx + 3], tq[q][ty + 1][tx + 3], tq[q][ty + 2][tx + 3], tq[q][ty + 4][tx + 3],
+                              tq[q][ty + 5][tx + 3], tq[q][ty + 6][tx + 3], a.i60[1]);
                 g[q][2] = d6c(wq[q][0], wq[q][1], wq[q][2], wq[q][4], wq[q][5], wq[q][6], a.i60[2]);
             }
-            double Vb[NVISC];
-#pragma unroll
-            for (int n = 0; n < NVISC; ++n) Vb[n] = __builtin_nontemporal_load(&a.SB[n][o]);
             double gb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, qb[4] = {0, 0, 0, 0};
             visc_local_vjp(a.mu, a.S_ref, a.cp_pr, wq[0][3], wq[1][3], wq[2][3], wq[3][3], g, Vb, gb, qb);
 #pragma unroll
@@ -882,9 +850,12 @@ __global__ void __launch_bounds__(256) k_adj_q_march(AdjArgs a) {
             }
         }
 #pragma unroll
-        for (int m = 0; m < 6; ++m)
+        for (int m = 0; m < 6; ++m) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) wl[n][m] = wl[n][m + 1];
 #pragma unroll
             for (int q = 0; q < 4; ++q) wq[q][m] = wq[q][m + 1];
+        }
     }
 }
 
@@ -1074,8 +1045,7 @@ void launch_adj_rhs(const AdjArgs &a, int interp, int rs, hipStream_t s, const L
     // the face and viscous gradient adjoints first, then the y and z
     // directions (y writes the interior of out, z accumulates), then the x
     // direction with the viscous part (ghosts of out are not written)
-    launch_adj_march<2, k_adj_lam_march>(a, s);
-    launch_adj_march<2, k_adj_q_march>(a, s);
+    launch_adj_march<2, k_adj_lamq_march>(a, s);
     adj_dir<1>(a, interp, rs, s);
     adj_dir<2>(a, interp, rs, s);
     adj_x(a, interp, rs, s, tail);
